@@ -1,0 +1,299 @@
+"""Seeded synthetic EVPFFT inputs in the reference's own file formats.
+
+Writes the three files a ``forge input.in`` run reads:
+
+* ``input.in``  -- the namelist parsed by ``GetNameList`` (src/io.c:96-164);
+* ``Cu.sx``     -- the single-crystal file parsed by ``PlasticInit`` at fixed line
+  positions (src/init.c:52-231); one slip mode, so the hardening matrix comes from
+  the mode that is used (the reference takes it from the *last* mode, Q9);
+* ``ms.txt``    -- the microstructure read by ``InitMicroStruct``
+  (src/init.c:253-266): ``phi1 Phi phi2 x y z grain phase`` per line, Bunge angles in
+  degrees, 1-based x-major indices with z fastest.
+
+Randomness comes from SplitMix64 (Steele, Lea & Flood 2014) with a recorded 64-bit
+seed, so a configuration is reproduced bit-for-bit from its seed.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from typing import Optional
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+# The 12 {111}<110> systems, written as the reference's Cu_p.sx lists them
+# (test_run/Cu_p.sx:38-49): plane normal n, then slip direction b.
+FCC_SLIP = [
+    ((1, 1, -1), (0, 1, 1)), ((1, 1, -1), (1, 0, 1)), ((1, 1, -1), (1, -1, 0)),
+    ((1, -1, -1), (0, 1, -1)), ((1, -1, -1), (1, 0, 1)), ((1, -1, -1), (1, 1, 0)),
+    ((1, -1, 1), (0, 1, 1)), ((1, -1, 1), (1, 0, -1)), ((1, -1, 1), (1, 1, 0)),
+    ((1, 1, 1), (0, 1, -1)), ((1, 1, 1), (1, 0, -1)), ((1, 1, 1), (1, -1, 0)),
+]
+
+
+class SplitMix64:
+    """SplitMix64 generator; ``uniform()`` returns 53-bit doubles in [0, 1)."""
+
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+    def uniforms(self, n: int) -> np.ndarray:
+        return np.array([self.uniform() for _ in range(n)], dtype=np.float64)
+
+
+@dataclasses.dataclass
+class Material:
+    """Cubic elastic constants (MPa) and power-law/Voce parameters of one phase."""
+
+    c11: float = 168400.0
+    c12: float = 121400.0
+    c44: float = 75400.0
+    nu: float = 0.35
+    nrs: float = 10.0       # rate exponent n
+    gam0: float = 1.0       # reference shear rate (1/s)
+    tau0f: float = 16.0     # initial CRSS, forward (MPa)
+    tau0b: float = 16.0     # initial CRSS, backward (MPa)
+    tau1: float = 10.0      # Voce tau1 (MPa)
+    thet0: float = 200.0    # initial hardening rate (MPa)
+    thet1: float = 20.0     # final hardening rate (MPa)
+    hself: float = 1.0
+    hlat: float = 1.0
+
+
+@dataclasses.dataclass
+class Config:
+    """One synthetic EVPFFT job."""
+
+    n: int = 32
+    grains: int = 64
+    seed: int = 0x5EED0001
+    increments: int = 10
+    dt: float = 1e-3
+    err: float = 1e-6
+    iter_max: int = 2000
+    pore_fraction: float = 0.0
+    pore_radius: int = 3
+    material: Material = dataclasses.field(default_factory=Material)
+    update_flag: int = 1
+    hard_flag: int = 1
+    elast_bc: int = 1
+    velgrad_bc_flag: tuple = (0, 0, 1, 1, 1, 1)
+    velgrad_bc: tuple = (0.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+    stress_bc_flag: tuple = (1, 1, 0, 0, 0, 0)
+    stress_bc: tuple = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    single_orientation: Optional[tuple] = None  # (phi1, Phi, phi2) for every grain
+
+
+# The configurations the loop is measured and checked on (sizes, grains, loading).
+def config_c1() -> Config:
+    """32^3 periodic voxels, 64 Voronoi grains, 10 increments."""
+    return Config(n=32, grains=64, seed=0x5EED0001, increments=10)
+
+
+def config_c2() -> Config:
+    """128^3, 512 grains, 2 % spherical pores of radius 3, 50 increments."""
+    return Config(n=128, grains=512, seed=0x5EED0002, increments=50, pore_fraction=0.02)
+
+
+def config_c3() -> Config:
+    """256^3, 4096 grains, 20 increments."""
+    return Config(n=256, grains=4096, seed=0x5EED0003, increments=20)
+
+
+def config_c4() -> Config:
+    """64^3, 128 grains, 20 increments (adjoint calibration)."""
+    return Config(n=64, grains=128, seed=0x5EED0004, increments=20)
+
+
+def voronoi(n: int, seeds: np.ndarray) -> np.ndarray:
+    """Periodic Voronoi labels (0-based) of voxel points (i, j, k), x-major, z fastest.
+
+    Nearest seed under the minimum-image distance; ties go to the lower seed index.
+    """
+    g = seeds.shape[0]
+    ids = np.empty(n * n * n, dtype=np.int32)
+    coords = np.arange(n, dtype=np.float64)
+    zz = coords[None, :]
+    for x in range(n):
+        best = np.full((n, n), np.inf)
+        lab = np.zeros((n, n), dtype=np.int32)
+        dx = np.abs(x - seeds[:, 0])
+        dx = np.minimum(dx, n - dx)
+        for s in range(g):
+            dy = np.abs(coords - seeds[s, 1])
+            dy = np.minimum(dy, n - dy)
+            dz = np.abs(zz - seeds[s, 2])
+            dz = np.minimum(dz, n - dz)
+            d = dx[s] * dx[s] + dy[:, None] ** 2 + dz ** 2
+            m = d < best
+            best = np.where(m, d, best)
+            lab = np.where(m, s, lab)
+        ids[x * n * n:(x + 1) * n * n] = lab.reshape(-1)
+    return ids
+
+
+def voronoi_fast(n: int, seeds: np.ndarray) -> np.ndarray:
+    """Same labels as :func:`voronoi` via a periodic k-d tree (large grids)."""
+    from scipy.spatial import cKDTree
+
+    tree = cKDTree(np.mod(seeds, n), boxsize=n)
+    ax = np.arange(n, dtype=np.float64)
+    out = np.empty(n ** 3, dtype=np.int32)
+    yy, zz = np.meshgrid(ax, ax, indexing="ij")
+    plane = np.stack([np.zeros(n * n), yy.ravel(), zz.ravel()], axis=1)
+    for x in range(n):
+        plane[:, 0] = x
+        d, i = tree.query(plane, k=2)
+        # exact ties resolved towards the lower seed index, as in voronoi()
+        tie = d[:, 0] == d[:, 1]
+        lab = np.where(tie, np.minimum(i[:, 0], i[:, 1]), i[:, 0])
+        out[x * n * n:(x + 1) * n * n] = lab
+    return out
+
+
+def pores(n: int, frac: float, radius: int, rng: SplitMix64) -> np.ndarray:
+    """Boolean gas mask: non-overlapping periodic digital spheres |x-c|^2 <= r^2."""
+    mask = np.zeros((n, n, n), dtype=bool)
+    if frac <= 0.0:
+        return mask.reshape(-1)
+    r = radius
+    off = np.arange(-r, r + 1)
+    ox, oy, oz = np.meshgrid(off, off, off, indexing="ij")
+    ball = (ox ** 2 + oy ** 2 + oz ** 2) <= r * r
+    bx, by, bz = ox[ball], oy[ball], oz[ball]
+    target = frac * n ** 3
+    tries = 0
+    while mask.sum() < target and tries < 100000:
+        tries += 1
+        c = [int(rng.uniform() * n) for _ in range(3)]
+        px = (c[0] + bx) % n
+        py = (c[1] + by) % n
+        pz = (c[2] + bz) % n
+        # reject overlapping or touching spheres
+        gx = (c[0] + np.arange(-r - 1, r + 2)) % n
+        gy = (c[1] + np.arange(-r - 1, r + 2)) % n
+        gz = (c[2] + np.arange(-r - 1, r + 2)) % n
+        if mask[np.ix_(gx, gy, gz)].any():
+            continue
+        mask[px, py, pz] = True
+    return mask.reshape(-1)
+
+
+def microstructure(cfg: Config):
+    """(euler_deg[N^3,3], grain[N^3], phase[N^3]) of a configuration."""
+    n = cfg.n
+    rng = SplitMix64(cfg.seed)
+    seeds = rng.uniforms(3 * cfg.grains).reshape(cfg.grains, 3) * n
+    u = rng.uniforms(3 * cfg.grains).reshape(cfg.grains, 3)
+    if cfg.single_orientation is not None:
+        eul = np.tile(np.asarray(cfg.single_orientation, dtype=np.float64), (cfg.grains, 1))
+    else:
+        eul = np.stack([360.0 * u[:, 0], np.degrees(np.arccos(2.0 * u[:, 1] - 1.0)),
+                        360.0 * u[:, 2]], axis=1)
+    # round to the precision written to the file so in-memory and file inputs agree
+    eul = np.round(eul, 8)
+    if cfg.grains == 1:
+        lab = np.zeros(n ** 3, dtype=np.int32)
+    elif n ** 3 * cfg.grains <= 4_000_000:
+        lab = voronoi(n, seeds)
+    else:
+        lab = voronoi_fast(n, seeds)
+    gas = pores(n, cfg.pore_fraction, cfg.pore_radius, rng)
+    grain = (lab + 1).astype(np.int32)
+    phase = np.ones(n ** 3, dtype=np.int32)
+    grain[gas] = cfg.grains + 1
+    phase[gas] = 2
+    euler = eul[lab]
+    euler[gas] = 0.0
+    return euler, grain, phase
+
+
+def write_ms(path: str, n: int, euler: np.ndarray, grain: np.ndarray, phase: np.ndarray):
+    """Write the 8-column microstructure file (each line < 80 characters)."""
+    idx = np.arange(n ** 3)
+    xs = idx // (n * n) + 1
+    ys = (idx // n) % n + 1
+    zs = idx % n + 1
+    cols = np.column_stack([euler, xs, ys, zs, grain, phase])
+    with open(path, "w") as f:
+        np.savetxt(f, cols, fmt=["%.8f", "%.8f", "%.8f", "%d", "%d", "%d", "%d", "%d"],
+                   delimiter=" ")
+
+
+def write_sx(path: str, m: Material):
+    """Single-mode cubic .sx file laid out on the line positions PlasticInit reads."""
+    lines = [
+        "# Crystal system (XtalSys):", "CUBIC", "",
+        "# Crystal axex (XtalAxis):", "1.0 1.0 1.0", "",
+        "# Total number of modes listed in the file (N_modes_max):", "1", "",
+        "# Number of modes to be used in the calculation (N_modes):", "1", "",
+        "# Labels of the modes to be used (iMode):", "1", "",
+        "# Mode# 1 -- <110>(111) SLIP",
+        "\tnsmx\tnrsx\tgamd0x\ttwshx\tisectwx",
+        f"\t12\t\t{m.nrs!r}\t\t{m.gam0!r}\t\t0.0\t\t0",
+        "\ttau0xf\ttau0xb\ttau1x\tthet0\tthet1",
+        f"\t{m.tau0f!r}\t\t{m.tau0b!r}\t\t{m.tau1!r}\t\t{m.thet0!r}\t{m.thet1!r}",
+        "\thselfx\thlatex",
+        f"\t{m.hself!r}\t\t{m.hlat!r}",
+        "\tSlip (n-b)",
+    ]
+    for nv, bv in FCC_SLIP:
+        lines.append("\t%d %d %d\t\t%d %d %d" % (*nv, *bv))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_input(path: str, cfg: Config, sx_name="Cu.sx", ms_name="ms.txt"):
+    """Namelist in the layout of test_run/input.in (power-law keys only)."""
+    m = cfg.material
+    two = cfg.pore_fraction > 0.0
+    lines = [
+        "# synthetic EVPFFT job (seed 0x%X)" % cfg.seed,
+        "N_phases\t%d" % (2 if two else 1),
+        "CellDim\t%d %d %d" % (cfg.n, cfg.n, cfg.n),
+        "initial_ms\t%s" % ms_name,
+        "Type_phases\t0\t1",
+        "ElastConst_PhaseI\t%r\t%r\t%r\t%r" % (m.c11, m.c12, m.c44, m.nu),
+        "Slip_PhaseI\t%s" % sx_name,
+        "ElastConst_PhaseII\t0.0\t0.0\t0.0\t0.0",
+        "Slip_PhaseII\t%s" % sx_name,
+        "VelGrad_BC_Flag\t" + " ".join(str(int(v)) for v in cfg.velgrad_bc_flag),
+        "VelGrad_BC\t" + " ".join(repr(float(v)) for v in cfg.velgrad_bc),
+        "Stress_BC_Flag\t" + " ".join(str(int(v)) for v in cfg.stress_bc_flag),
+        "Stress_BC\t" + " ".join(repr(float(v)) for v in cfg.stress_bc),
+        "ElastBC\t%d" % cfg.elast_bc,
+        "TimeStep %r" % cfg.dt,
+        "N_steps\t%d" % cfg.increments,
+        "Err\t%r" % cfg.err,
+        "IterMax\t%d" % cfg.iter_max,
+        "PrintControl\t0 2000",
+        "Update_Flag\t%d" % cfg.update_flag,
+        "Hard_Flag\t%d" % cfg.hard_flag,
+        "Tex_Flag\t0",
+    ]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_job(dirpath: str, cfg: Config) -> str:
+    """Write input.in, Cu.sx and ms.txt of ``cfg`` into ``dirpath``; return input.in path."""
+    os.makedirs(dirpath, exist_ok=True)
+    euler, grain, phase = microstructure(cfg)
+    write_ms(os.path.join(dirpath, "ms.txt"), cfg.n, euler, grain, phase)
+    write_sx(os.path.join(dirpath, "Cu.sx"), cfg.material)
+    inp = os.path.join(dirpath, "input.in")
+    write_input(inp, cfg)
+    return inp

@@ -1,0 +1,274 @@
+"""ctypes front end of the CPU oracle (oracle/evp_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker of the HIP path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "lib", "liboracle.so")
+_lib = None
+
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+# fields of orc_job_field()
+SIG, EPS, EDOT, DISGRAD, VELGRAD, TRANMAT, CRSS, GAMDOT, GAMACUM = range(9)
+
+# layout of orc_job_macro()
+MACRO = dict(TimeTot=0, e_vm=1, evmp=2, d_vm=3, dvmp=4, s_vm=5, s_vm1=6, Err_e=7,
+             Err_s=8, Wgt_ph1=9, green_time=55, green_iters=56, newton_iters=57,
+             istep=58, status=59)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.orc_job_setup.restype = C.c_void_p
+        L.orc_job_setup.argtypes = [C.c_char_p, IP]
+        L.orc_job_free.argtypes = [C.c_void_p]
+        L.orc_job_run.argtypes = [C.c_void_p, C.c_int, IP, IP]
+        L.orc_job_nvox.argtypes = [C.c_void_p]
+        L.orc_job_dims.argtypes = [C.c_void_p, IP]
+        L.orc_job_field.restype = DP
+        L.orc_job_field.argtypes = [C.c_void_p, C.c_int]
+        L.orc_job_grain.restype = IP
+        L.orc_job_grain.argtypes = [C.c_void_p]
+        L.orc_job_phase.restype = IP
+        L.orc_job_phase.argtypes = [C.c_void_p]
+        L.orc_job_macro.argtypes = [C.c_void_p, DP]
+        L.orc_job_setup_info.argtypes = [C.c_void_p, DP, DP, DP, DP]
+        L.orc_job_cgr.argtypes = [C.c_void_p, DP]
+        L.orc_job_green.argtypes = [C.c_void_p, DP, DP, DP, DP]
+        L.orc_job_newton_iters.restype = C.c_double
+        L.orc_job_newton_iters.argtypes = [C.c_void_p]
+        L.orc_fft3d.argtypes = [DP, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_lu_inv66.argtypes = [DP]
+        L.orc_vonmises.restype = C.c_double
+        L.orc_vonmises.argtypes = [DP]
+        L.orc_strain_rate.argtypes = [DP, DP, DP, C.c_int, DP, DP, DP, DP, DP]
+        L.orc_harden_voxel.argtypes = [C.c_int, DP, C.c_double, DP, DP, DP, DP, DP]
+        L.orc_update_stress_voxel.argtypes = [DP, DP, C.c_double, DP, DP, DP, C.c_int, DP,
+                                              DP, DP, DP, DP, DP, DP, DP, IP]
+        L.orc_rotate_schmid.argtypes = [DP, C.c_int, DP, DP]
+        L.orc_orientation.argtypes = [DP, DP]
+        L.orc_euler_to_mat.argtypes = [C.c_double] * 3 + [DP]
+        for nm in ("orc_v6_to_t2", "orc_t2_to_v6", "orc_c2_to_t4", "orc_t4_to_c2",
+                   "orc_v5_to_t2", "orc_t2_to_v5"):
+            getattr(L, nm).argtypes = [DP, DP]
+        L.orc_basis.argtypes = [DP]
+        L.orc_ten4_transform_inv.argtypes = [DP, DP, DP]
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray, t=DP):
+    return a.ctypes.data_as(t)
+
+
+def f64(shape):
+    return np.zeros(shape, dtype=np.float64)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code):
+        super().__init__(f"oracle status {code}")
+        self.code = code
+
+
+class Job:
+    """A whole EVPFFT job on the CPU: setup from input.in, then increments."""
+
+    def __init__(self, input_path: str):
+        L = lib()
+        st = C.c_int(0)
+        self.h = L.orc_job_setup(os.fsencode(input_path), C.byref(st))
+        if not self.h:
+            raise OracleError(st.value)
+        d = (C.c_int * 3)()
+        L.orc_job_dims(self.h, d)
+        self.dims = tuple(d)
+        self.n = L.orc_job_nvox(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_job_free(self.h)
+            self.h = None
+
+    def run(self, n: int, forced=None) -> np.ndarray:
+        it = np.zeros(n, dtype=np.int32)
+        f = None if forced is None else np.ascontiguousarray(forced, dtype=np.int32)
+        st = lib().orc_job_run(self.h, n, ptr(it, IP), None if f is None else ptr(f, IP))
+        if st:
+            raise OracleError(st)
+        return it
+
+    def field(self, which: int) -> np.ndarray:
+        p = lib().orc_job_field(self.h, which)
+        per = {CRSS: 24, GAMDOT: 12, GAMACUM: 1}.get(which, 9)
+        a = np.ctypeslib.as_array(p, shape=(self.n * per,)).copy()
+        if per == 9:
+            return a.reshape(self.n, 3, 3)
+        if per == 24:
+            return a.reshape(self.n, 12, 2)
+        return a.reshape(self.n, per) if per > 1 else a
+
+    def grain(self):
+        return np.ctypeslib.as_array(lib().orc_job_grain(self.h), shape=(self.n,)).copy()
+
+    def phase(self):
+        return np.ctypeslib.as_array(lib().orc_job_phase(self.h), shape=(self.n,)).copy()
+
+    def macro(self) -> dict:
+        o = f64(64)
+        lib().orc_job_macro(self.h, ptr(o))
+        d = {k: float(o[v]) for k, v in MACRO.items()}
+        d["SigAvg"] = o[10:19].reshape(3, 3).copy()
+        d["SigAvg1"] = o[19:28].reshape(3, 3).copy()
+        d["DisGradAvg"] = o[28:37].reshape(3, 3).copy()
+        d["EpsAvg"] = o[37:46].reshape(3, 3).copy()
+        d["EdotAvg"] = o[46:55].reshape(3, 3).copy()
+        return d
+
+    def setup_info(self):
+        c066, s066, c0, s0 = f64(36), f64(36), f64(81), f64(81)
+        lib().orc_job_setup_info(self.h, ptr(c066), ptr(s066), ptr(c0), ptr(s0))
+        return (c066.reshape(6, 6), s066.reshape(6, 6), c0.reshape(3, 3, 3, 3),
+                s0.reshape(3, 3, 3, 3))
+
+    def cgr(self):
+        a = f64(self.n * 36)
+        lib().orc_job_cgr(self.h, ptr(a))
+        return a.reshape(self.n, 6, 6)
+
+    def green(self, ksr: np.ndarray, ksi: np.ndarray):
+        ksr = np.ascontiguousarray(ksr, dtype=np.float64)
+        ksi = np.ascontiguousarray(ksi, dtype=np.float64)
+        orr, oi = np.zeros_like(ksr), np.zeros_like(ksi)
+        lib().orc_job_green(self.h, ptr(ksr), ptr(ksi), ptr(orr), ptr(oi))
+        return orr, oi
+
+
+# ---- unit-level wrappers ----
+def basis():
+    B = f64((6, 3, 3))
+    lib().orc_basis(ptr(B))
+    return B
+
+
+def _unary(name, x, out_shape):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    o = f64(out_shape)
+    getattr(lib(), name)(ptr(x), ptr(o))
+    return o
+
+
+def v6_to_t2(v): return _unary("orc_v6_to_t2", v, (3, 3))
+def t2_to_v6(t): return _unary("orc_t2_to_v6", t, (6,))
+def v5_to_t2(v): return _unary("orc_v5_to_t2", v, (3, 3))
+def t2_to_v5(t): return _unary("orc_t2_to_v5", t, (5,))
+def c2_to_t4(c): return _unary("orc_c2_to_t4", c, (3, 3, 3, 3))
+def t4_to_c2(t): return _unary("orc_t4_to_c2", t, (6, 6))
+
+
+def euler_to_mat(t1, ph, t2):
+    a = f64((3, 3))
+    lib().orc_euler_to_mat(t1, ph, t2, ptr(a))
+    return a
+
+
+def ten4_transform_inv(a, c2):
+    c1 = f64((3, 3, 3, 3))
+    lib().orc_ten4_transform_inv(ptr(c1), ptr(np.ascontiguousarray(a, dtype=np.float64)),
+                                 ptr(np.ascontiguousarray(c2, dtype=np.float64)))
+    return c1
+
+
+def lu_inv66(c):
+    c = np.array(c, dtype=np.float64, order="C")
+    st = lib().orc_lu_inv66(ptr(c))
+    if st:
+        raise OracleError(st)
+    return c
+
+
+def vonmises(t):
+    return lib().orc_vonmises(ptr(np.ascontiguousarray(t, dtype=np.float64)))
+
+
+def fft3d(x: np.ndarray, sign: int) -> np.ndarray:
+    """Unnormalised 3-D DFT of a complex array with exponent sign ``sign``."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    nx, ny, nz = a.shape
+    lib().orc_fft3d(a.ctypes.data_as(DP), nx, ny, nz, sign)
+    return a
+
+
+def strain_rate(stress6, sc, crss, nsr, gam0):
+    nsys = sc.shape[0]
+    edot, dedot, gd = f64(6), f64((6, 6)), f64(12)
+    lib().orc_strain_rate(ptr(np.ascontiguousarray(stress6, dtype=np.float64)), ptr(edot),
+                          ptr(dedot), nsys, ptr(np.ascontiguousarray(sc, dtype=np.float64)),
+                          ptr(np.ascontiguousarray(crss, dtype=np.float64)),
+                          ptr(np.ascontiguousarray(nsr, dtype=np.float64)),
+                          ptr(np.ascontiguousarray(gam0, dtype=np.float64)), ptr(gd))
+    return edot, dedot, gd[:nsys]
+
+
+def harden_voxel(gamdot, dt, hard, tau, thet, crss, gamacum):
+    crss = np.array(crss, dtype=np.float64, order="C")
+    g = np.array([gamacum], dtype=np.float64)
+    nsys = len(gamdot)
+    H = f64((12, 12))
+    H[:nsys, :nsys] = hard
+    lib().orc_harden_voxel(nsys, ptr(np.ascontiguousarray(gamdot, dtype=np.float64)), dt,
+                           ptr(H), ptr(np.ascontiguousarray(tau, dtype=np.float64)),
+                           ptr(np.ascontiguousarray(thet, dtype=np.float64)), ptr(crss), ptr(g))
+    return crss, float(g[0])
+
+
+def update_stress_voxel(sg66, c066, dt, sig, eps, disgrad, sc, crss, nsr, gam0):
+    """Returns (status, sig_out, edot, gamdot, err_s, err_e, n_iter)."""
+    sig = np.array(sig, dtype=np.float64, order="C")
+    edot, gd = f64((3, 3)), f64(12)
+    es, ee = f64(1), f64(1)
+    nit = np.zeros(1, dtype=np.int32)
+    st = lib().orc_update_stress_voxel(
+        ptr(np.ascontiguousarray(sg66, dtype=np.float64)),
+        ptr(np.ascontiguousarray(c066, dtype=np.float64)), dt, ptr(sig),
+        ptr(np.ascontiguousarray(eps, dtype=np.float64)),
+        ptr(np.ascontiguousarray(disgrad, dtype=np.float64)), sc.shape[0],
+        ptr(np.ascontiguousarray(sc, dtype=np.float64)),
+        ptr(np.ascontiguousarray(crss, dtype=np.float64)),
+        ptr(np.ascontiguousarray(nsr, dtype=np.float64)),
+        ptr(np.ascontiguousarray(gam0, dtype=np.float64)), ptr(edot), ptr(gd), ptr(es),
+        ptr(ee), ptr(nit, IP))
+    return st, sig, edot, gd[:sc.shape[0]], float(es[0]), float(ee[0]), int(nit[0])
+
+
+def rotate_schmid(tranmat, schm_xt):
+    nsys = schm_xt.shape[0]
+    out = f64((nsys, 5))
+    lib().orc_rotate_schmid(ptr(np.ascontiguousarray(tranmat, dtype=np.float64)), nsys,
+                            ptr(np.ascontiguousarray(schm_xt, dtype=np.float64)), ptr(out))
+    return out
+
+
+def orientation(tran, rot):
+    tran = np.array(tran, dtype=np.float64, order="C")
+    rot = np.array(rot, dtype=np.float64, order="C")
+    lib().orc_orientation(ptr(tran), ptr(rot))
+    return tran, rot
