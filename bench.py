@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the EVPFFT Green-operator loop on one MI355X.
+
+Workload (the 128^3 configuration): 128^3 periodic voxels, 512 periodic
+Voronoi grains with uniform SO(3) texture, 2 % of the volume in spherical
+pores (radius 3, zero stiffness), FCC Cu (C11/C12/C44 = 168.4/121.4/75.4 GPa,
+12 {111}<110> systems, n = 10, gamma0 = 1/s, Voce tau0/tau1/theta0/theta1 =
+16/10/200/20 MPa), uniaxial tension along z at 1/s with free lateral
+stresses, dt = 1e-3 s, Green-loop tolerance 1e-6.  Inputs are written in the
+reference's file formats by the seeded generator and read by the drop-in.
+
+A step is one loading increment (prologue, Green loop to convergence,
+epilogue).  `--warmup W` increments run on the job first, then the job is
+reset to its post-setup state and increments 1..K are timed, so the timed
+steps are the first K increments of the 50-increment run.  Both the mixed
+(FP32 FFT/Green, FP64 Newton + FP64 correction sweeps) and the all-FP64 GPU
+loops run; `value` is the mixed loop's voxel-iterations per second:
+    N^3 * (Green iterations) / (wall time of the K increments).
+
+Prints one JSON line (rank 0, N = 1 only).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "m-themes_amd"))
+
+from pyevp import lib, synth  # noqa: E402
+
+METRIC = "voxel-iters/s at 128³, mixed vs FP64 GPU at 1e-6 err; adjoint/forward time"
+
+# FP64 operations of the per-voxel Newton kernel (counted from newton.hip,
+# an FMA counts as 2): per voxel and launch (loads -> 6-vectors, norms,
+# stores) / per Newton iteration (rate law with its 5x5 derivative, residual,
+# Jacobian, gradient, LU solve, step control) / per line-search trial (rate
+# law without derivative, residual, merit function).
+NEWTON_FLOPS_VOXEL = 100
+NEWTON_FLOPS_ITER = 2006
+NEWTON_FLOPS_TRIAL = 606
+# HBM bytes of the Newton kernel per crystal voxel and launch: reads sigma,
+# strain, plastic strain (3 x 48), Schmid tensors (480), CRSS (96, forward =
+# backward), phase + orientation index (8); writes sigma, edot (2 x 48),
+# gamdot (96).  Gas voxels: phase (4), sigma/edot writes (96), sigma re-read (48).
+NEWTON_BYTES_CRYSTAL = 3 * 48 + 480 + 96 + 8 + 2 * 48 + 96
+NEWTON_BYTES_GAS = 4 + 96 + 48
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(inp, precision, steps, warmup):
+    job = lib.GpuJob(inp, precision)
+    if warmup > 0:
+        job.run(warmup)
+        job.reset()
+    job.set_timing(True)
+    job.synchronize()
+    t0 = time.perf_counter()
+    it, stats = job.run(steps)
+    job.synchronize()
+    wall = time.perf_counter() - t0
+    out = dict(wall=wall, iters=[int(i) for i in it], stats=stats,
+               timing={k: job.timing(k) for k in range(5)}, macro=job.macro(),
+               sig=job.state()["Sig"])
+    job.close()
+    return out
+
+
+def cpu_baseline(inp, n_iter):
+    """The oracle (C restatement, OpenMP over voxels) on a bounded sample of the
+    same workload: the first `n_iter` Green iterations of increment 1."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    J = O.Job(inp)
+    J.run(1, forced=[n_iter])
+    m = J.macro()
+    n = int(np.prod(J.dims))
+    return dict(value=n * n_iter / m["green_time"], unit="voxel-iters/s", cores=O.threads(),
+                kind="port",
+                sample=f"128^3 job, increment 1, first {n_iter} Green iterations "
+                       f"({m['green_time']:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4, help="timed increments")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="grid edge (default: the 128^3 job)")
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--cpu-iters", type=int, default=3,
+                    help="Green iterations of the CPU baseline sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs the whole grid "
+              "on one GPU (no multi-GPU path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    cfg = synth.config_c2()
+    if args.n != 128:
+        cfg.n = args.n
+        cfg.grains = max(8, 512 * args.n ** 3 // 128 ** 3)
+    work = tempfile.mkdtemp(prefix="evpbench_")
+    t = time.perf_counter()
+    inp = synth.write_job(work, cfg)
+    log(f"bench: wrote {cfg.n}^3 job ({cfg.grains} grains, {cfg.pore_fraction:.0%} pores) "
+        f"in {time.perf_counter() - t:.1f} s")
+    N = cfg.n ** 3
+    res = {}
+    for name, prec in (("fp64", lib.FP64), ("mixed", lib.MIXED)):
+        if args.precision not in ("both", name):
+            continue
+        log(f"bench: {name}: warmup {args.warmup}, timing {args.steps} increments")
+        r = run_gpu(inp, prec, args.steps, args.warmup)
+        its = sum(r["iters"])
+        r["vips"] = N * its / r["wall"]
+        log(f"bench: {name}: iterations {r['iters']} ({sum(s['fp64_iters'] for s in r['stats'])} "
+            f"FP64), {r['wall']:.2f} s, {r['vips']:.4g} voxel-iters/s")
+        res[name] = r
+    main_key = "mixed" if "mixed" in res else "fp64"
+    R = res[main_key]
+
+    # roofline of the dominant kernel: the per-voxel Newton update
+    ms, launches = R["timing"][0]
+    n_gas = None
+    g, p = None, None
+    nit = sum(s["newton_iters"] for s in R["stats"])
+    ntr = sum(s["ls_trials"] for s in R["stats"])
+    j = lib.GpuJob(inp, lib.FP64)
+    g, p = j.ids()
+    j.close()
+    n_cryst = int(np.sum(p == 1))
+    n_gas = N - n_cryst
+    flops = NEWTON_FLOPS_VOXEL * n_cryst * launches + NEWTON_FLOPS_ITER * nit + NEWTON_FLOPS_TRIAL * ntr
+    bytes_ = (NEWTON_BYTES_CRYSTAL * n_cryst + NEWTON_BYTES_GAS * n_gas) * launches
+    t_s = ms * 1e-3
+    tf = flops / t_s / 1e12
+    gbs = bytes_ / t_s / 1e9
+    if tf / PEAK_FP64_TFS >= gbs / PEAK_HBM_GBS:
+        roof = dict(bound="mfma", achieved=tf, peak=PEAK_FP64_TFS, unit="TFLOP/s",
+                    frac=tf / PEAK_FP64_TFS, traffic=None, kernel="k_newton",
+                    avg_launch_ms=ms / max(launches, 1), hbm_gbs=gbs,
+                    note="FP64 vector ALU bound; peak is the MI355X FP64 figure (78.6 TFLOP/s), "
+                         "no MFMA instructions are used")
+    else:
+        roof = dict(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s",
+                    frac=gbs / PEAK_HBM_GBS, traffic=None, kernel="k_newton",
+                    avg_launch_ms=ms / max(launches, 1), tflops=tf)
+    kern = {}
+    names = ["newton", "fft_forward", "green", "fft_inverse_update", "increment_pro_epilogue"]
+    for k, nm in enumerate(names):
+        kms, kn = R["timing"][k]
+        kern[nm] = dict(ms_total=round(kms, 3), launches=kn, ms_avg=round(kms / max(kn, 1), 4))
+
+    out = {
+        "metric": METRIC,
+        "value": R["vips"],
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * R["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if main_key == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"EVPFFT {cfg.n}^3, {cfg.grains} Voronoi grains, "
+                               f"{cfg.pore_fraction:.0%} pores, increments 1..{args.steps} of 50",
+                   "grid": cfg.n, "grains": cfg.grains, "pores": cfg.pore_fraction,
+                   "precision": main_key, "green_tol": cfg.err, "seed": hex(cfg.seed)},
+        "green_iters": R["iters"],
+        "kernels_ms": kern,
+        "roofline": roof,
+    }
+    if "fp64" in res:
+        out["fp64"] = {"value": res["fp64"]["vips"], "ms_per_step": 1e3 * res["fp64"]["wall"] / args.steps,
+                       "green_iters": res["fp64"]["iters"]}
+    if "fp64" in res and "mixed" in res:
+        a, b = res["fp64"]["sig"], res["mixed"]["sig"]
+        out["mixed_vs_fp64"] = {
+            "speedup_voxel_iters": res["mixed"]["vips"] / res["fp64"]["vips"],
+            "speedup_time": res["fp64"]["wall"] / res["mixed"]["wall"],
+            "stress_rel_err": float(np.max(np.abs(a - b)) / np.max(np.abs(a))),
+            "sig_zz_rel_err": float(abs(res["mixed"]["macro"]["sig_avg"][2, 2] -
+                                        res["fp64"]["macro"]["sig_avg"][2, 2]) /
+                                    abs(res["fp64"]["macro"]["sig_avg"][2, 2])),
+        }
+    if args.cpu_iters > 0:
+        log(f"bench: cpu baseline: oracle, {args.cpu_iters} Green iterations")
+        out["cpu_baseline"] = cpu_baseline(inp, args.cpu_iters)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,77 @@
+/*
+ * evp.h -- drop-in entry points of the M-THEMES EVPFFT solver, backed by the
+ * MI355X Green-operator loop (evpgpu.h).
+ *
+ * The first seven functions replace the reference's public interface
+ * (src/evp.h:572-596,615), which a `forge input.in` run calls from main
+ * (src/evp.c:7-32):
+ *   GetNameList   src/io.c:96-164     namelist parser, same keys and quirks
+ *   PrintNameList src/io.c:166-211
+ *   OpenFiles     src/io.c:213-230    vm.out, err.out
+ *   CloseFiles    src/io.c:232-245
+ *   SetupJob      src/init.c:727-1205 (power-law build)
+ *   Evolution     src/evolution.c:550-1729 (reconstructed power-law driver)
+ *   DestroyJob    src/init.c:1207-1399
+ * They keep the reference's calling convention (global state, no
+ * arguments).  On an error they print the reference's message and exit with
+ * its code, as PError/MPI_Abort does (src/evp.h:62-63), unless
+ * evp_set_abort_on_error(0) was called; then they return and evp_status()
+ * holds the code.
+ *
+ * The evp_job_* functions are the same steps on an explicit handle, for
+ * callers that hold several jobs or want status codes.
+ */
+#ifndef EVP_DROPIN_H
+#define EVP_DROPIN_H
+
+#include "evpgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- the reference's interface (C linkage) ---- */
+int GetNameList(char **argv);
+void PrintNameList(void);
+void OpenFiles(void);
+void CloseFiles(void);
+void SetupJob(void);
+void Evolution(void);
+void DestroyJob(void);
+
+void evp_set_abort_on_error(int on);
+int evp_status(void);
+/* precision of the Green loop for SetupJob: EVPGPU_FP64 (default) or EVPGPU_MIXED */
+void evp_set_precision(int precision);
+
+/* ---- handle API ---- */
+typedef struct evp_job evp_job;
+
+/* Parse input_path (namelist, .sx files, microstructure), set the job up on
+ * the GPU.  Relative file names are looked up in the working directory, then
+ * next to input_path.                                                          */
+evp_job *evp_job_open(const char *input_path, int precision, int *status);
+void evp_job_close(evp_job *job);
+evpgpu_ctx *evp_job_ctx(evp_job *job);
+int evp_job_dims(const evp_job *job, int dims[3]);
+int evp_job_nsteps(const evp_job *job);
+/* run n increments; iters[k] gets the Green iterations of increment k,
+ * stats (may be NULL) the per-increment statistics.                           */
+int evp_job_run(evp_job *job, int n, int forced_iters, int *iters, evpgpu_stats *stats);
+int evp_job_macro(evp_job *job, evpgpu_macro *m);
+/* grain, phase: int32[N] as read from the microstructure file */
+int evp_job_ids(const evp_job *job, int *grain, int *phase);
+/* number of distinct grain IDs of non-gas voxels (gID_list, src/init.c:286-352)
+ * and, if ids != NULL, the sorted IDs                                          */
+int evp_job_grain_list(const evp_job *job, int *ids);
+/* setup products: c066, s066 (36), c0, s0 (81) */
+int evp_job_setup_info(const evp_job *job, double *c066, double *s066, double *c0, double *s0);
+/* restore the state right after setup (fields, CRSS, orientations, macros) */
+int evp_job_reset(evp_job *job);
+/* vm.out row of the last increment (TIME EVM EVMP DVM DVMP SVM SVM_1) */
+int evp_job_vm_row(const evp_job *job, double row[7]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

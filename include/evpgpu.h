@@ -1,0 +1,158 @@
+/*
+ * evpgpu.h -- C ABI of the MI355X (gfx950) EVPFFT Green-operator loop.
+ *
+ * This is the device seam under the drop-in entry points of evp.h: one
+ * context holds the whole voxel grid resident in HBM and runs the loading
+ * increments of src/evolution.c:1318-1547 (prologue, Green fixed-point loop,
+ * epilogue) with hand-written HIP kernels.  It replaces, on one GPU, the
+ * FFTW-MPI slab loop of src/evolution.c:1343-1463 and the per-voxel work of
+ * src/evolution.c:44-310, src/kinematics.c:430-463, src/constitutive.c:196-326.
+ *
+ * Conventions
+ *   - Plain C types only; every call returns an int status (EVPGPU_OK = 0, or
+ *     the reference's abort code where one exists: 35 singular LU,
+ *     1203 line-search round-off), instead of calling MPI_Abort.
+ *   - Host arrays are caller-owned, FP64 (grain/phase int32), structure of
+ *     arrays: component-major, then voxel index v = (x*ny + y)*nz + z
+ *     (the reference's pIDX, src/evp.h:53).
+ *       symmetric 2nd-order fields (6 comps): 11 22 33 23 13 12
+ *         (the component order of the .iout writers, src/io.c:400-437)
+ *       full 3x3 fields (9 comps): row-major a*3+b
+ *       crss: (s*2 + d), d = 0 forward / 1 backward CRSS of system s
+ *   - Device memory is library-owned.  One context per host thread; all work
+ *     is ordered on the context's stream.  No exceptions cross the ABI.
+ */
+#ifndef EVPGPU_H
+#define EVPGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    EVPGPU_OK = 0,
+    EVPGPU_ERR_SINGULAR = 35,     /* LU row max < 1e-5, src/kinematics.c:306-313 */
+    EVPGPU_ERR_LINESEARCH = 1203, /* slope >= 0, src/evolution.c:210-214          */
+    EVPGPU_ERR_ARG = -1,          /* bad argument                                 */
+    EVPGPU_ERR_HIP = -2,          /* HIP runtime failure                          */
+    EVPGPU_ERR_SIZE = -3,         /* grid size not supported (powers of 2, 8..512) */
+    EVPGPU_ERR_NOMEM = -4,        /* device allocation failed                     */
+    EVPGPU_ERR_STATE = -5,        /* call out of order                            */
+    EVPGPU_ERR_NODEV = -6         /* no HIP device                                */
+};
+
+enum { EVPGPU_FP64 = 0, EVPGPU_MIXED = 1 };
+
+#define EVPGPU_NSYSMX 12
+
+typedef struct evpgpu_ctx evpgpu_ctx;
+
+/* One phase as PlasticInit leaves it (src/init.c:52-231). */
+typedef struct {
+    int is_gas;                    /* Type_phases[ph] != 0 */
+    int nsys;                      /* nSYS[ph] */
+    double nrs[EVPGPU_NSYSMX];     /* rate exponent n per system (nSRS) */
+    double gam0[EVPGPU_NSYSMX];    /* reference shear rate */
+    double tau[EVPGPU_NSYSMX][3];  /* tau0 forward, tau0 backward, tau1 */
+    double thet[EVPGPU_NSYSMX][2]; /* theta0, theta1 */
+    double hard[EVPGPU_NSYSMX][EVPGPU_NSYSMX]; /* latent/self hardening */
+    double schm_xt[EVPGPU_NSYSMX][5];          /* Schmid tensors, crystal axes, 5-D basis */
+    double bn[EVPGPU_NSYSMX][3], bb[EVPGPU_NSYSMX][3]; /* unit normals, directions */
+} evpgpu_phase;
+
+/* Loading and macro constants (src/init.c:363-378, 864-900). */
+typedef struct {
+    double dt;                     /* TimeStep */
+    double err_tol;                /* Err */
+    int iter_max;                  /* IterMax */
+    int update_flag, hard_flag, elast_bc;
+    int velgrad_bc_flag[6], stress_bc_flag[6];
+    double velgrad_bc[6];          /* already multiplied by the flags */
+    double stress_bc[6];
+    double c066[36], s066[36];     /* <C_gr> and its inverse, 6x6 B-basis */
+    double c0[81], s0[81];         /* the same as 4th-order tensors */
+    double wgt_ph1;                /* volume fraction of phase 1 */
+    double d_vm0;                  /* von Mises of dev(sym Udot) */
+} evpgpu_loading;
+
+/* Macroscopic state carried between increments (src/evolution.c:550-1729). */
+typedef struct {
+    int istep;                     /* index of the next increment */
+    double time_tot, e_vm, evmp, d_vm, dvmp, s_vm, s_vm1, err_e, err_s;
+    double sig_avg[9], sig_avg1[9];
+    double disgrad_avg[9];         /* DisGradAvg: elastic guess of this increment */
+    double disgrad_avg_t[9];       /* DisGradAvg_t (= DisGradAvg_actual at t)     */
+    double eps_avg[9], edot_avg[9];
+} evpgpu_macro;
+
+typedef struct {
+    int iters;                     /* Green iterations of the increment (all precisions) */
+    int fp64_iters;                /* of which FP64 correction sweeps (mixed mode) */
+    double newton_iters;           /* sum over voxels and iterations */
+    double ls_trials;              /* line-search trial evaluations (NR_quality calls) */
+    double seconds;                /* Green-loop wall time */
+    int stopped;                   /* 1 if the creep stop (e_vm > 0.3) ended the run */
+} evpgpu_stats;
+
+const char *evpgpu_strerror(int status);
+
+/* Create a context for an nx*ny*nz grid (each a power of 2 in [8, 512]). */
+int evpgpu_create(evpgpu_ctx **ctx, int nx, int ny, int nz, int precision);
+int evpgpu_destroy(evpgpu_ctx *ctx);
+int evpgpu_set_precision(evpgpu_ctx *ctx, int precision);
+
+int evpgpu_set_phases(evpgpu_ctx *ctx, const evpgpu_phase *phases, int nphases);
+int evpgpu_set_loading(evpgpu_ctx *ctx, const evpgpu_loading *ld);
+
+/* grain, phase: int32[N]; orient: int32[N] index into sgr (n_orient x 36, the
+ * per-orientation LU_inv_66(C_gr), src/evolution.c:71-74), -1 for gas.        */
+int evpgpu_upload_microstructure(evpgpu_ctx *ctx, const int *grain, const int *phase,
+                                 const int *orient, int n_orient, const double *sgr);
+
+/* Full voxel state.  Any pointer may be NULL on download to skip that field. */
+int evpgpu_upload_state(evpgpu_ctx *ctx, const double *sig6, const double *disgrad9,
+                        const double *eps6, const double *edot6, const double *tranmat9,
+                        const double *crss24, const double *gamacum);
+int evpgpu_download_state(evpgpu_ctx *ctx, double *sig6, double *disgrad9, double *eps6,
+                          double *edot6, double *tranmat9, double *crss24, double *gamacum,
+                          double *gamdot12);
+
+int evpgpu_set_macro(evpgpu_ctx *ctx, const evpgpu_macro *m);
+int evpgpu_get_macro(evpgpu_ctx *ctx, evpgpu_macro *m);
+
+/* One loading increment: src/evolution.c:1318-1547 plus hardening.
+ * forced_iters > 0 runs exactly that many Green iterations (parity protocol). */
+int evpgpu_increment(evpgpu_ctx *ctx, int forced_iters, evpgpu_stats *stats);
+
+/* Kernel timing with HIP events on the context's stream (off by default).
+ * which: 0 per-voxel Newton update, 1 forward FFT, 2 Green step, 3 inverse FFT
+ * with strain update, 4 increment prologue/epilogue.  Reports the summed
+ * elapsed milliseconds and the launch count since the last reset.            */
+int evpgpu_set_timing(evpgpu_ctx *ctx, int on);
+/* Wait for all work of the context's stream and of the device. */
+int evpgpu_synchronize(evpgpu_ctx *ctx);
+int evpgpu_get_timing(evpgpu_ctx *ctx, int which, double *ms, long long *launches);
+
+/* ---- building blocks on device-resident or host data (tests, tools) ---- */
+/* R2C 3-D DFT (exponent sign -1) of a real host field -> half spectrum
+ * [nx][ny][nz/2+1] complex interleaved; C2R inverse (sign +1, unnormalised). */
+int evpgpu_fft_r2c_host(evpgpu_ctx *ctx, const double *in, double *out_cplx);
+int evpgpu_fft_c2r_host(evpgpu_ctx *ctx, const double *in_cplx, double *out);
+/* Gamma-hat : sigma-hat on a host half spectrum of the 6 stress components
+ * (6 x [nx][ny][nz/2+1] complex) -> 9 components of the displacement-gradient
+ * increment in the same layout (row-major a*3+b).                           */
+int evpgpu_green_host(evpgpu_ctx *ctx, const double *sig_hat6, double *e_hat9);
+/* Per-voxel stress update (src/evolution.c:44-310) on a host batch of n voxels
+ * of the context's phases: inputs lambda (sig6), strain (6, symmetric part of the
+ * displacement gradient), plastic strain eps6, Schmid schm60 (s*5+j, sample axes),
+ * crss24, phase; sgr/orient as for the microstructure.  Outputs sig6 (in place),
+ * edot6, gamdot12, err2 (err_s, err_e per voxel) and niter.                   */
+int evpgpu_newton_host(evpgpu_ctx *ctx, int n, const int *phase, const int *orient,
+                       const double *sgr, int n_orient, double *sig6, const double *strain6,
+                       const double *eps6, const double *schm60, const double *crss24,
+                       double *edot6, double *gamdot12, double *err2, int *niter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,803 @@
+// evpgpu.hip -- C ABI of the MI355X EVPFFT loop: context, state transfer and
+// the loading increment (src/evolution.c:1318-1547 + reconstructed harden).
+//
+// Per Green iteration (src/evolution.c:1355-1462) the device runs
+//   R2C(sigma, 6 comps) -> Gamma_eff (in place, antisym part accumulated)
+//   -> C2R fused with  E += sym(dDisGradAvg) + re/N
+//   -> per-voxel Newton update + block partials -> one deterministic row sum
+// and the host reads 15 doubles back to finish get_smacro (src/evolution.c:
+// 473-547), the error normalisation and the loop test.  In mixed precision the
+// FFT/Green/C2R steps run in FP32 until the loop converges, then FP64 sweeps
+// run until the FP64 residual converges again (at least one sweep).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "evpgpu_impl.h"
+
+// ------------------------------------------------------------ host tensors --
+namespace {
+
+const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
+
+void basis(double B[6][3][3])
+{
+    memset(B, 0, sizeof(double) * 54);
+    B[1][0][0] = -1. * RSQ6; B[1][1][1] = -1. * RSQ6; B[1][2][2] = 2. * RSQ6;
+    B[0][0][0] = -1. * RSQ2; B[0][1][1] = RSQ2;
+    B[2][1][2] = RSQ2; B[2][2][1] = RSQ2;
+    B[3][0][2] = RSQ2; B[3][2][0] = RSQ2;
+    B[4][0][1] = RSQ2; B[4][1][0] = RSQ2;
+    B[5][0][0] = RSQ3; B[5][1][1] = RSQ3; B[5][2][2] = RSQ3;
+}
+
+// chg_basis opt 2 / chg_basis5 opt 1 restricted to n components
+void t2_to_v(const double t[9], double *v, int n)
+{
+    double B[6][3][3];
+    basis(B);
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) s += t[a * 3 + b] * B[i][a][b];
+        v[i] = s;
+    }
+}
+void v_to_t2(const double *v, double t[9], int n)
+{
+    double B[6][3][3];
+    basis(B);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double s = 0.0;
+            for (int i = 0; i < n; i++) s += v[i] * B[i][a][b];
+            t[a * 3 + b] = s;
+        }
+}
+
+// VonMises (src/kinematics.c:465-484)
+double vonmises(const double t[9])
+{
+    double tr = t[0] + t[4] + t[8], vm = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double d = (t[a * 3 + b] + t[b * 3 + a]) / 2. - (double)(a == b) * tr / 3.0;
+            vm += d * d;
+        }
+    return sqrt(2. / 3. * vm);
+}
+
+void sym6_to_full(const double s[6], double t[9])
+{
+    t[0] = s[0]; t[4] = s[1]; t[8] = s[2];
+    t[5] = t[7] = s[3];
+    t[2] = t[6] = s[4];
+    t[1] = t[3] = s[5];
+}
+
+double now_s()
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+bool pow2_ok(int n) { return n >= 8 && n <= 256 && (n & (n - 1)) == 0; }
+
+template <typename T> int dalloc(T **p, size_t n)
+{
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return EVPGPU_ERR_NOMEM;
+    }
+    return EVPGPU_OK;
+}
+
+NewtonParams newton_params(const evpgpu_ctx *c)
+{
+    NewtonParams P;
+    memset(&P, 0, sizeof(P));
+    memcpy(P.c066, c->ld.c066, sizeof(P.c066));
+    P.dt = c->ld.dt;
+    for (int p = 0; p < 2; p++) {
+        if (p >= c->nphases) {
+            P.is_gas[p] = 1;
+            continue;
+        }
+        const evpgpu_phase &ph = c->ph[p];
+        P.is_gas[p] = ph.is_gas;
+        P.nsys[p] = ph.nsys;
+        int integral = ph.nsys > 0;
+        for (int s = 0; s < ph.nsys; s++) {
+            P.nrs[p][s] = ph.nrs[s];
+            P.gam0[p][s] = ph.gam0[s];
+            double n = ph.nrs[s];
+            if (!(n == floor(n) && n >= 1 && n <= 64 && n == ph.nrs[0])) integral = 0;
+        }
+        P.int_n[p] = integral ? (int)ph.nrs[0] : 0;
+    }
+    P.crss_sym = c->crss_sym;
+    return P;
+}
+
+// get_smacro (src/evolution.c:473-547) on the reduced sums; Q3 fixed
+void smacro(evpgpu_ctx *c, const double *red)
+{
+    const double wgt = 1.0 / (double)c->N;
+    double s[6], s1[6];
+    for (int k = 0; k < 6; k++) {
+        s[k] = red[3 + k] * wgt;
+        s1[k] = red[9 + k] * (wgt / c->ld.wgt_ph1);
+    }
+    sym6_to_full(s, c->m.sig_avg);
+    sym6_to_full(s1, c->m.sig_avg1);
+    for (int i = 0; i < 6; i++) {
+        int ii = IJV[0][i], jj = IJV[1][i];
+        double d = 0.0;
+        if (c->ld.velgrad_bc_flag[i] == 0)
+            for (int k = 0; k < 6; k++) {
+                int kk = IJV[0][k], ll = IJV[1][k];
+                d += c->ld.s0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * c->ld.stress_bc_flag[k] *
+                     (c->scauchy[kk * 3 + ll] - c->m.sig_avg[kk * 3 + ll]);
+            }
+        c->dD[ii * 3 + jj] = d;
+    }
+    for (int i = 0; i < 9; i++) c->dD_acum[i] += c->dD[i];
+    double v6[6], dev[9];
+    t2_to_v(c->m.sig_avg, v6, 6);
+    v_to_t2(v6, dev, 5);
+    double tr = dev[0] + dev[4] + dev[8], a = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double d = dev[i * 3 + j] - (i == j ? tr / 3.0 : 0.0);
+            a += (i == j) ? d * d : dev[i * 3 + j] * dev[i * 3 + j];
+        }
+    c->m.s_vm = sqrt(3. / 2. * a);
+    t2_to_v(c->m.sig_avg1, v6, 6);
+    v_to_t2(v6, dev, 5);
+    a = 0.0;
+    for (int i = 0; i < 9; i++) a += dev[i] * dev[i];
+    c->m.s_vm1 = sqrt(3. / 2. * a);
+}
+
+inline void tmark(evpgpu_ctx *c, int k, int end)
+{
+    if (c->timing) (void)hipEventRecord(c->ev[k][end], c->stream);
+}
+
+// add the elapsed times of the events recorded since the last collect
+void tcollect(evpgpu_ctx *c, const int *which, int n)
+{
+    if (!c->timing) return;
+    for (int i = 0; i < n; i++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[which[i]][0], c->ev[which[i]][1]) == hipSuccess) {
+            c->t_ms[which[i]] += ms;
+            c->t_n[which[i]]++;
+        }
+    }
+}
+
+// one Green iteration in precision prec; leaves errors and macros updated
+int green_iteration(evpgpu_ctx *c, int prec, double *newton_iters, double *ls_trials)
+{
+    tmark(c, 1, 0);
+    int st = fft_forward(c, c->sig, 6, prec);
+    if (st) return st;
+    tmark(c, 1, 1);
+    tmark(c, 2, 0);
+    st = green_apply(c, prec, 1);
+    if (st) return st;
+    tmark(c, 2, 1);
+    double add[6];
+    const double *d = c->dD;
+    add[0] = d[0]; add[1] = d[4]; add[2] = d[8];
+    add[3] = (d[5] + d[7]) / 2.0;
+    add[4] = (d[2] + d[6]) / 2.0;
+    add[5] = (d[1] + d[3]) / 2.0;
+    for (int i = 0; i < 9; i++) c->dD_applied[i] += d[i];
+    tmark(c, 3, 0);
+    st = fft_inverse_update(c, 6, prec, c->E, add, 1.0 / (double)c->N);
+    if (st) return st;
+    tmark(c, 3, 1);
+    NewtonParams P = newton_params(c);
+    int nblk = 0;
+    tmark(c, 0, 0);
+    st = newton_launch(c, P, c->sig, c->E, c->epsp, c->edot, c->schm, c->crss, c->gamdot, c->phase,
+                       c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr, nullptr);
+    if (st) return st;
+    tmark(c, 0, 1);
+    st = reduce_partials(c, c->partials, nblk, NRED, c->red);
+    if (st) return st;
+    EVP_CHECK(hipMemcpyAsync(c->h_red, c->red, NRED * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EVP_CHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EVP_CHECK(hipStreamSynchronize(c->stream));
+    {
+        const int w[4] = {0, 1, 2, 3};
+        tcollect(c, w, 4);
+    }
+    if (*c->h_status) return *c->h_status;
+    const double wgt = 1.0 / (double)c->N;
+    c->m.err_e = c->h_red[1] * wgt;
+    c->m.err_s = c->h_red[0] * wgt;
+    *newton_iters += c->h_red[2];
+    *ls_trials += c->h_red[15];
+    smacro(c, c->h_red);
+    if (fabs(c->m.e_vm) < 1E-5) c->m.err_e = 0.0;
+    else c->m.err_e /= c->m.e_vm;
+    c->m.err_s /= c->m.s_vm;
+    return EVPGPU_OK;
+}
+
+int ready(const evpgpu_ctx *c)
+{
+    return c && c->have_phases && c->have_loading && c->have_ms && c->have_state;
+}
+
+} // namespace
+
+// ------------------------------------------------------------------- ABI --
+extern "C" {
+
+const char *evpgpu_strerror(int s)
+{
+    switch (s) {
+    case EVPGPU_OK: return "ok";
+    case EVPGPU_ERR_SINGULAR: return "singular 6x6 matrix in LU decomposition (35)";
+    case EVPGPU_ERR_LINESEARCH: return "roundoff problem during line searching in NR (1203)";
+    case EVPGPU_ERR_ARG: return "invalid argument";
+    case EVPGPU_ERR_HIP: return "HIP runtime error";
+    case EVPGPU_ERR_SIZE: return "unsupported grid size (each dimension a power of 2 in [8, 256])";
+    case EVPGPU_ERR_NOMEM: return "device allocation failed";
+    case EVPGPU_ERR_STATE: return "call out of order (phases, loading, microstructure, state)";
+    case EVPGPU_ERR_NODEV: return "no HIP device";
+    }
+    return "unknown status";
+}
+
+int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
+{
+    if (!out) return EVPGPU_ERR_ARG;
+    *out = nullptr;
+    if (!pow2_ok(nx) || !pow2_ok(ny) || !pow2_ok(nz)) return EVPGPU_ERR_SIZE;
+    if (precision != EVPGPU_FP64 && precision != EVPGPU_MIXED) return EVPGPU_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EVPGPU_ERR_NODEV;
+    evpgpu_ctx *c = (evpgpu_ctx *)calloc(1, sizeof(evpgpu_ctx));
+    EVP_CHECK(hipGetDevice(&c->device));
+    c->nx = nx; c->ny = ny; c->nz = nz; c->nzh = nz / 2 + 1;
+    c->N = (long long)nx * ny * nz;
+    c->NK = (long long)nx * ny * c->nzh;
+    c->precision = precision;
+    int st = EVPGPU_OK;
+    if (hipStreamCreate(&c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+    const long long N = c->N;
+    // the Newton kernel runs 64-lane blocks and writes NRED partials per block;
+    // the epilogue (256-lane blocks, NRED_EPI partials) needs less
+    long long nblk = (N + 63) / 64;
+    if (!st) st = dalloc(&c->sig, 6 * N);
+    if (!st) st = dalloc(&c->E, 6 * N);
+    if (!st) st = dalloc(&c->W, 3 * N);
+    if (!st) st = dalloc(&c->epsp, 6 * N);
+    if (!st) st = dalloc(&c->edot, 6 * N);
+    if (!st) st = dalloc(&c->tran, 9 * N);
+    if (!st) st = dalloc(&c->crss, 24 * N);
+    if (!st) st = dalloc(&c->gamacum, N);
+    if (!st) st = dalloc(&c->gamdot, 12 * N);
+    if (!st) st = dalloc(&c->schm, 60 * N);
+    if (!st) st = dalloc(&c->phase, N);
+    if (!st) st = dalloc(&c->grain, N);
+    if (!st) st = dalloc(&c->orient, N);
+    if (!st) st = dalloc((double2 **)&c->spec, 6 * c->NK);
+    if (!st) st = dalloc(&c->acc, 3 * c->NK);
+    if (!st) st = dalloc(&c->rwork, 3 * N);
+    if (!st) st = dalloc(&c->partials, nblk * NRED);
+    if (!st) st = dalloc(&c->red, NRED);
+    if (!st) st = dalloc(&c->d_status, 1);
+    if (!st && hipHostMalloc((void **)&c->h_red, NRED * sizeof(double), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
+    if (!st && hipHostMalloc((void **)&c->h_status, sizeof(int), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
+    if (!st) st = fft_twiddles(c);
+    if (!st) {
+        // fields start at zero (gamdot of gas voxels is never written)
+        st = fill_zero(c, c->gamdot, 12 * N);
+        if (!st) st = fill_zero(c, c->schm, 60 * N);
+        if (!st && hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+        if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+    }
+    if (st) {
+        evpgpu_destroy(c);
+        return st;
+    }
+    *out = c;
+    return EVPGPU_OK;
+}
+
+int evpgpu_destroy(evpgpu_ctx *c)
+{
+    if (!c) return EVPGPU_OK;
+    void *ptrs[] = {c->sig, c->E, c->W, c->epsp, c->edot, c->tran, c->crss, c->gamacum, c->gamdot,
+                    c->schm, c->phase, c->grain, c->orient, c->sgr, c->spec, c->acc, c->rwork,
+                    c->partials, c->red, c->d_status, c->tw64[0], c->tw64[1], c->tw64[2],
+                    c->tw32[0], c->tw32[1], c->tw32[2]};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
+        if (ptrs[i]) (void)hipFree(ptrs[i]);
+    for (int k = 0; k < 5; k++)
+        for (int e = 0; e < 2; e++)
+            if (c->ev[k][e]) (void)hipEventDestroy(c->ev[k][e]);
+    if (c->h_red) (void)hipHostFree(c->h_red);
+    if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    free(c);
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_timing(evpgpu_ctx *c, int on)
+{
+    if (!c) return EVPGPU_ERR_ARG;
+    if (on && !c->ev[0][0])
+        for (int k = 0; k < 5; k++)
+            for (int e = 0; e < 2; e++) EVP_CHECK(hipEventCreate(&c->ev[k][e]));
+    c->timing = on ? 1 : 0;
+    memset(c->t_ms, 0, sizeof(c->t_ms));
+    memset(c->t_n, 0, sizeof(c->t_n));
+    return EVPGPU_OK;
+}
+
+int evpgpu_synchronize(evpgpu_ctx *c)
+{
+    if (!c) return EVPGPU_ERR_ARG;
+    EVP_CHECK(hipStreamSynchronize(c->stream));
+    EVP_CHECK(hipDeviceSynchronize());
+    return EVPGPU_OK;
+}
+
+int evpgpu_get_timing(evpgpu_ctx *c, int which, double *ms, long long *launches)
+{
+    if (!c || which < 0 || which > 4) return EVPGPU_ERR_ARG;
+    if (ms) *ms = c->t_ms[which];
+    if (launches) *launches = c->t_n[which];
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_precision(evpgpu_ctx *c, int precision)
+{
+    if (!c || (precision != EVPGPU_FP64 && precision != EVPGPU_MIXED)) return EVPGPU_ERR_ARG;
+    c->precision = precision;
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_phases(evpgpu_ctx *c, const evpgpu_phase *ph, int n)
+{
+    if (!c || !ph || n < 1 || n > 2) return EVPGPU_ERR_ARG;
+    for (int p = 0; p < n; p++)
+        if (ph[p].nsys < 0 || ph[p].nsys > EVPGPU_NSYSMX) return EVPGPU_ERR_ARG;
+    memcpy(c->ph, ph, sizeof(evpgpu_phase) * n);
+    c->nphases = n;
+    c->have_phases = 1;
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_loading(evpgpu_ctx *c, const evpgpu_loading *ld)
+{
+    if (!c || !ld) return EVPGPU_ERR_ARG;
+    c->ld = *ld;
+    // Udot = VoigtToFull(VelGrad_BC) (src/kinematics.c:5-12), Scauchy likewise
+    const double *v = ld->velgrad_bc, *s = ld->stress_bc;
+    double U[9] = {v[0], v[5], v[4], v[5], v[1], v[3], v[4], v[3], v[2]};
+    double S[9] = {s[0], s[5], s[4], s[5], s[1], s[3], s[4], s[3], s[2]};
+    memcpy(c->udot, U, sizeof(U));
+    memcpy(c->scauchy, S, sizeof(S));
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            c->udot_s[a * 3 + b] = (U[a * 3 + b] + U[b * 3 + a]) / 2.;
+            c->udot_a[a * 3 + b] = (U[a * 3 + b] - U[b * 3 + a]) / 2.;
+        }
+    c->creep_flag = 0;
+    for (int i = 0; i < 6; i++) c->creep_flag += ld->stress_bc_flag[i];
+    c->creep_flag /= 6;
+    c->have_loading = 1;
+    return EVPGPU_OK;
+}
+
+int evpgpu_upload_microstructure(evpgpu_ctx *c, const int *grain, const int *phase, const int *orient,
+                                 int n_orient, const double *sgr)
+{
+    if (!c || !grain || !phase || !orient || n_orient < 1 || !sgr) return EVPGPU_ERR_ARG;
+    for (long long v = 0; v < c->N; v++)
+        if (orient[v] >= n_orient || (orient[v] < 0 && phase[v] >= 1 && phase[v] <= c->nphases &&
+                                      !c->ph[phase[v] - 1].is_gas))
+            return EVPGPU_ERR_ARG;
+    // gas voxels carry orientation -1; the kernel never reads their table row,
+    // but keep the index valid
+    int *o2 = (int *)malloc(sizeof(int) * c->N);
+    for (long long v = 0; v < c->N; v++) o2[v] = orient[v] < 0 ? 0 : orient[v];
+    if (c->sgr) (void)hipFree(c->sgr);
+    int st = dalloc(&c->sgr, 36LL * n_orient);
+    if (!st && hipMemcpy(c->sgr, sgr, sizeof(double) * 36 * n_orient, hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st && hipMemcpy(c->grain, grain, sizeof(int) * c->N, hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st && hipMemcpy(c->phase, phase, sizeof(int) * c->N, hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st && hipMemcpy(c->orient, o2, sizeof(int) * c->N, hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
+    free(o2);
+    if (st) return st;
+    c->n_orient = n_orient;
+    c->have_ms = 1;
+    return EVPGPU_OK;
+}
+
+int evpgpu_upload_state(evpgpu_ctx *c, const double *sig6, const double *dg9, const double *eps6,
+                        const double *edot6, const double *tran9, const double *crss24,
+                        const double *gamacum)
+{
+    if (!c || !sig6 || !dg9 || !eps6 || !edot6 || !tran9 || !crss24 || !gamacum) return EVPGPU_ERR_ARG;
+    if (!c->have_phases || !c->have_ms) return EVPGPU_ERR_STATE;
+    const long long N = c->N;
+    double *E = (double *)malloc(sizeof(double) * 9 * N);
+    double *W = E + 6 * N;
+    for (long long v = 0; v < N; v++) {
+        const double *d = dg9;
+        double g[9];
+        for (int k = 0; k < 9; k++) g[k] = d[k * N + v];
+        E[0 * N + v] = g[0];
+        E[1 * N + v] = g[4];
+        E[2 * N + v] = g[8];
+        E[3 * N + v] = (g[5] + g[7]) / 2.0;
+        E[4 * N + v] = (g[2] + g[6]) / 2.0;
+        E[5 * N + v] = (g[1] + g[3]) / 2.0;
+        W[0 * N + v] = (g[5] - g[7]) / 2.0;
+        W[1 * N + v] = (g[2] - g[6]) / 2.0;
+        W[2 * N + v] = (g[1] - g[3]) / 2.0;
+    }
+    int sym = 1;
+    for (long long i = 0; i < 12 * N && sym; i++) {
+        long long s = i / N, v = i % N;
+        if (crss24[(2 * s) * N + v] != crss24[(2 * s + 1) * N + v]) sym = 0;
+    }
+    c->crss_sym = sym;
+    int st = EVPGPU_OK;
+#define H2D(dst, src, n) \
+    if (!st && hipMemcpy(dst, src, sizeof(double) * (n), hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP
+    H2D(c->sig, sig6, 6 * N);
+    H2D(c->E, E, 6 * N);
+    H2D(c->W, W, 3 * N);
+    H2D(c->epsp, eps6, 6 * N);
+    H2D(c->edot, edot6, 6 * N);
+    H2D(c->tran, tran9, 9 * N);
+    H2D(c->crss, crss24, 24 * N);
+    H2D(c->gamacum, gamacum, N);
+#undef H2D
+    free(E);
+    if (st) return st;
+    // Schmid tensors from the current orientations (initialised at setup, Q12)
+    st = schmid_launch(c);
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st) c->have_state = 1;
+    return st;
+}
+
+int evpgpu_download_state(evpgpu_ctx *c, double *sig6, double *dg9, double *eps6, double *edot6,
+                          double *tran9, double *crss24, double *gamacum, double *gamdot12)
+{
+    if (!c) return EVPGPU_ERR_ARG;
+    const long long N = c->N;
+    EVP_CHECK(hipStreamSynchronize(c->stream));
+#define D2H(dst, src, n) \
+    if (dst) EVP_CHECK(hipMemcpy(dst, src, sizeof(double) * (n), hipMemcpyDeviceToHost))
+    D2H(sig6, c->sig, 6 * N);
+    D2H(eps6, c->epsp, 6 * N);
+    D2H(edot6, c->edot, 6 * N);
+    D2H(tran9, c->tran, 9 * N);
+    D2H(crss24, c->crss, 24 * N);
+    D2H(gamacum, c->gamacum, N);
+    D2H(gamdot12, c->gamdot, 12 * N);
+#undef D2H
+    if (dg9) {
+        double *E = (double *)malloc(sizeof(double) * 9 * N), *W = E + 6 * N;
+        if (hipMemcpy(E, c->E, sizeof(double) * 6 * N, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(W, c->W, sizeof(double) * 3 * N, hipMemcpyDeviceToHost) != hipSuccess) {
+            free(E);
+            return EVPGPU_ERR_HIP;
+        }
+        for (long long v = 0; v < N; v++) {
+            double e[6], w[3];
+            for (int k = 0; k < 6; k++) e[k] = E[k * N + v];
+            for (int k = 0; k < 3; k++) w[k] = W[k * N + v];
+            double g[9] = {e[0], e[5] + w[2], e[4] + w[1], e[5] - w[2], e[1],
+                           e[3] + w[0], e[4] - w[1], e[3] - w[0], e[2]};
+            for (int k = 0; k < 9; k++) dg9[k * N + v] = g[k];
+        }
+        free(E);
+    }
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_macro(evpgpu_ctx *c, const evpgpu_macro *m)
+{
+    if (!c || !m) return EVPGPU_ERR_ARG;
+    c->m = *m;
+    memcpy(c->disgrad_avg_actual, m->disgrad_avg_t, sizeof(c->disgrad_avg_actual));
+    return EVPGPU_OK;
+}
+
+int evpgpu_get_macro(evpgpu_ctx *c, evpgpu_macro *m)
+{
+    if (!c || !m) return EVPGPU_ERR_ARG;
+    *m = c->m;
+    return EVPGPU_OK;
+}
+
+int evpgpu_increment(evpgpu_ctx *c, int forced, evpgpu_stats *stats)
+{
+    if (!ready(c)) return EVPGPU_ERR_STATE;
+    evpgpu_stats S;
+    memset(&S, 0, sizeof(S));
+    const double dt = c->ld.dt;
+    evpgpu_macro &m = c->m;
+    if (c->creep_flag == 1 && m.e_vm > 0.3) {
+        S.stopped = 1;
+        if (stats) *stats = S;
+        return EVPGPU_OK;
+    }
+    // prologue: DisGrad += Udot*dt (creep: DisGrad = DisGradAvg_actual)
+    double pro[9];
+    const double *src = c->creep_flag == 1 ? c->disgrad_avg_actual : nullptr;
+    double U[9];
+    for (int i = 0; i < 9; i++) U[i] = src ? src[i] : c->udot[i] * dt;
+    pro[0] = U[0]; pro[1] = U[4]; pro[2] = U[8];
+    pro[3] = (U[5] + U[7]) / 2.0; pro[4] = (U[2] + U[6]) / 2.0; pro[5] = (U[1] + U[3]) / 2.0;
+    pro[6] = (U[5] - U[7]) / 2.0; pro[7] = (U[2] - U[6]) / 2.0; pro[8] = (U[1] - U[3]) / 2.0;
+    tmark(c, 4, 0);
+    int st = prologue_launch(c, pro, c->creep_flag == 1, nullptr);
+    if (st) return st;
+    if (m.istep == 1 || c->ld.update_flag == 1) {
+        st = schmid_launch(c);
+        if (st) return st;
+    }
+    st = fill_zero(c, (double *)c->acc, 6 * c->NK);
+    if (st) return st;
+    tmark(c, 4, 1);
+    EVP_CHECK(hipStreamSynchronize(c->stream));
+    {
+        const int w[1] = {4};
+        tcollect(c, w, 1);
+    }
+    memset(c->dD, 0, sizeof(c->dD));
+    memset(c->dD_acum, 0, sizeof(c->dD_acum));
+    memset(c->dD_applied, 0, sizeof(c->dD_applied));
+    int iter = 0, it64 = 0;
+    m.err_e = 2. * c->ld.err_tol;
+    m.err_s = 2. * c->ld.err_tol;
+    const double tol = c->ld.err_tol;
+    double nit = 0.0, ntr = 0.0;
+    double t0 = now_s();
+    auto more = [&](void) {
+        return (iter < c->ld.iter_max) && (fmax(fabs(m.err_s), fabs(m.err_e)) > tol);
+    };
+    if (forced > 0) {
+        while (iter < forced) {
+            int prec = (c->precision == EVPGPU_MIXED && iter < forced - 1) ? EVPGPU_MIXED : EVPGPU_FP64;
+            iter++;
+            if (prec == EVPGPU_FP64) it64++;
+            st = green_iteration(c, prec, &nit, &ntr);
+            if (st) return st;
+        }
+    } else {
+        if (c->precision == EVPGPU_MIXED) {
+            while (more()) {
+                iter++;
+                st = green_iteration(c, EVPGPU_MIXED, &nit, &ntr);
+                if (st) return st;
+            }
+            // FP64 residual-correction sweeps: at least one, until converged
+            do {
+                iter++;
+                it64++;
+                st = green_iteration(c, EVPGPU_FP64, &nit, &ntr);
+                if (st) return st;
+            } while (more());
+        } else {
+            while (more()) {
+                iter++;
+                it64++;
+                st = green_iteration(c, EVPGPU_FP64, &nit, &ntr);
+                if (st) return st;
+            }
+        }
+    }
+    S.seconds = now_s() - t0;
+    S.iters = iter;
+    S.fp64_iters = it64;
+    S.newton_iters = nit;
+    S.ls_trials = ntr;
+
+    // epilogue (src/evolution.c:1469-1547)
+    // antisymmetric displacement-gradient increment: uniform part + IFFT of the
+    // accumulated antisymmetric Green output
+    double m3[3];
+    {
+        const double *a = c->udot_a, *d = c->dD_applied;
+        m3[0] = a[5] * dt + (d[5] - d[7]) / 2.0;
+        m3[1] = a[2] * dt + (d[2] - d[6]) / 2.0;
+        m3[2] = a[1] * dt + (d[1] - d[3]) / 2.0;
+        if (c->creep_flag == 1) m3[0] = m3[1] = m3[2] = 0.0;
+    }
+    tmark(c, 4, 0);
+    st = fft_inverse_real(c, c->acc, 3, c->rwork, 1.0 / (double)c->N);
+    if (st) return st;
+    st = antisym_add_launch(c, m3);
+    if (st) return st;
+    double actual[9], vg[9];
+    for (int i = 0; i < 9; i++) {
+        actual[i] = m.disgrad_avg[i] + c->dD_acum[i];
+        vg[i] = (actual[i] - m.disgrad_avg_t[i]) / dt;
+    }
+    memcpy(c->disgrad_avg_actual, actual, sizeof(actual));
+    m.e_vm = vonmises(actual);
+    m.d_vm = vonmises(vg);
+    m.time_tot += dt;
+    memcpy(m.disgrad_avg_t, actual, sizeof(actual));
+    EpilogueParams E;
+    memset(&E, 0, sizeof(E));
+    E.dt = dt;
+    memcpy(E.udot_a, c->udot_a, sizeof(E.udot_a));
+    E.update_orient = (c->ld.update_flag == 1 && m.istep > 1);
+    E.harden = (c->ld.hard_flag == 1 && m.istep > 1);
+    E.wgt = 1.0 / (double)c->N;
+    for (int p = 0; p < 2; p++) {
+        E.is_gas[p] = p < c->nphases ? c->ph[p].is_gas : 1;
+        if (p >= c->nphases) continue;
+        const evpgpu_phase &ph = c->ph[p];
+        E.nsys[p] = ph.nsys;
+        memcpy(E.hard[p], ph.hard, sizeof(E.hard[p]));
+        memcpy(E.tau[p], ph.tau, sizeof(E.tau[p]));
+        memcpy(E.thet[p], ph.thet, sizeof(E.thet[p]));
+        memcpy(E.bn[p], ph.bn, sizeof(E.bn[p]));
+        memcpy(E.bb[p], ph.bb, sizeof(E.bb[p]));
+    }
+    int nb = 0;
+    st = epilogue_launch(c, E, c->partials, &nb);
+    if (st) return st;
+    st = reduce_partials(c, c->partials, nb, NRED_EPI, c->red);
+    if (st) return st;
+    tmark(c, 4, 1);
+    EVP_CHECK(hipMemcpyAsync(c->h_red, c->red, NRED_EPI * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EVP_CHECK(hipStreamSynchronize(c->stream));
+    {
+        const int w[1] = {4};
+        tcollect(c, w, 1);
+    }
+    sym6_to_full(c->h_red, m.eps_avg);
+    sym6_to_full(c->h_red + 6, m.edot_avg);
+    double ev = 0.0, dv = 0.0;
+    for (int i = 0; i < 9; i++) {
+        ev += m.eps_avg[i] * m.eps_avg[i];
+        dv += m.edot_avg[i] * m.edot_avg[i];
+    }
+    m.evmp = sqrt(2. / 3. * ev);
+    m.dvmp = sqrt(2. / 3. * dv);
+    for (int i = 0; i < 9; i++) m.disgrad_avg[i] = m.disgrad_avg_t[i] + c->udot[i] * dt;
+    m.istep++;
+    if (stats) *stats = S;
+    return EVPGPU_OK;
+}
+
+// ------------------------------------------------------ building blocks --
+int evpgpu_fft_r2c_host(evpgpu_ctx *c, const double *in, double *out)
+{
+    if (!c || !in || !out) return EVPGPU_ERR_ARG;
+    EVP_CHECK(hipMemcpy(c->rwork, in, sizeof(double) * c->N, hipMemcpyHostToDevice));
+    int st = fft_forward(c, c->rwork, 1, c->precision);
+    if (st) return st;
+    EVP_CHECK(hipStreamSynchronize(c->stream));
+    if (c->precision == EVPGPU_MIXED) {
+        float *tmp = (float *)malloc(sizeof(float) * 2 * c->NK);
+        EVP_CHECK(hipMemcpy(tmp, c->spec, sizeof(float) * 2 * c->NK, hipMemcpyDeviceToHost));
+        for (long long i = 0; i < 2 * c->NK; i++) out[i] = tmp[i];
+        free(tmp);
+    } else {
+        EVP_CHECK(hipMemcpy(out, c->spec, sizeof(double) * 2 * c->NK, hipMemcpyDeviceToHost));
+    }
+    return EVPGPU_OK;
+}
+
+int evpgpu_fft_c2r_host(evpgpu_ctx *c, const double *in, double *out)
+{
+    if (!c || !in || !out) return EVPGPU_ERR_ARG;
+    if (c->precision == EVPGPU_MIXED) {
+        float *tmp = (float *)malloc(sizeof(float) * 2 * c->NK);
+        for (long long i = 0; i < 2 * c->NK; i++) tmp[i] = (float)in[i];
+        EVP_CHECK(hipMemcpy(c->spec, tmp, sizeof(float) * 2 * c->NK, hipMemcpyHostToDevice));
+        free(tmp);
+    } else {
+        EVP_CHECK(hipMemcpy(c->spec, in, sizeof(double) * 2 * c->NK, hipMemcpyHostToDevice));
+    }
+    int st = fill_zero(c, c->rwork, c->N);
+    if (!st) st = fft_inverse_update(c, 1, c->precision, c->rwork, nullptr, 1.0);
+    if (st) return st;
+    EVP_CHECK(hipStreamSynchronize(c->stream));
+    EVP_CHECK(hipMemcpy(out, c->rwork, sizeof(double) * c->N, hipMemcpyDeviceToHost));
+    return EVPGPU_OK;
+}
+
+int evpgpu_green_host(evpgpu_ctx *c, const double *in6, double *out9)
+{
+    if (!c || !in6 || !out9 || !c->have_loading) return EVPGPU_ERR_ARG;
+    double2 *o9 = nullptr;
+    int st = dalloc(&o9, 9 * c->NK);
+    if (st) return st;
+    if (hipMemcpy(c->spec, in6, sizeof(double2) * 6 * c->NK, hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st) st = green_full9(c, (double2 *)c->spec, o9);
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st && hipMemcpy(out9, o9, sizeof(double2) * 9 * c->NK, hipMemcpyDeviceToHost) != hipSuccess) st = EVPGPU_ERR_HIP;
+    (void)hipFree(o9);
+    return st;
+}
+
+int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient, const double *sgr,
+                       int n_orient, double *sig6, const double *strain6, const double *eps6,
+                       const double *schm60, const double *crss24, double *edot6, double *gamdot12,
+                       double *err2, int *niter)
+{
+    if (!c || n < 1 || !phase || !orient || !sgr || !sig6 || !strain6 || !eps6 || !schm60 || !crss24 ||
+        !edot6 || !gamdot12 || !err2 || !niter || !c->have_phases || !c->have_loading)
+        return EVPGPU_ERR_ARG;
+    for (int v = 0; v < n; v++)
+        if (orient[v] < 0 || orient[v] >= n_orient || phase[v] < 1 || phase[v] > c->nphases) return EVPGPU_ERR_ARG;
+    int *dph = nullptr, *dor = nullptr, *dni = nullptr, *dst = nullptr;
+    double *dsg = nullptr, *dsig = nullptr, *dstr = nullptr, *deps = nullptr, *dsc = nullptr,
+           *dcr = nullptr, *ded = nullptr, *dgd = nullptr, *der = nullptr, *dpart = nullptr;
+    int st = EVPGPU_OK;
+    if (!st) st = dalloc(&dph, n);
+    if (!st) st = dalloc(&dor, n);
+    if (!st) st = dalloc(&dni, n);
+    if (!st) st = dalloc(&dst, 1);
+    if (!st) st = dalloc(&dsg, 36LL * n_orient);
+    if (!st) st = dalloc(&dsig, 6LL * n);
+    if (!st) st = dalloc(&dstr, 6LL * n);
+    if (!st) st = dalloc(&deps, 6LL * n);
+    if (!st) st = dalloc(&dsc, 60LL * n);
+    if (!st) st = dalloc(&dcr, 24LL * n);
+    if (!st) st = dalloc(&ded, 6LL * n);
+    if (!st) st = dalloc(&dgd, 12LL * n);
+    if (!st) st = dalloc(&der, 2LL * n);
+    if (!st) st = dalloc(&dpart, (long long)((n + 63) / 64) * NRED);
+#define UP(d, s, cnt, T) \
+    if (!st && hipMemcpy(d, s, sizeof(T) * (cnt), hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP
+    UP(dph, phase, n, int);
+    UP(dor, orient, n, int);
+    UP(dsg, sgr, 36LL * n_orient, double);
+    UP(dsig, sig6, 6LL * n, double);
+    UP(dstr, strain6, 6LL * n, double);
+    UP(deps, eps6, 6LL * n, double);
+    UP(dsc, schm60, 60LL * n, double);
+    UP(dcr, crss24, 24LL * n, double);
+    UP(dgd, gamdot12, 12LL * n, double);
+#undef UP
+    if (!st && hipMemset(dst, 0, sizeof(int)) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st) {
+        NewtonParams P = newton_params(c);
+        P.crss_sym = 0;
+        int nb = 0;
+        st = newton_launch(c, P, dsig, dstr, deps, ded, dsc, dcr, dgd, dph, dor, dsg, n, dpart, &nb, dst,
+                           der, dni);
+        if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+    }
+    int kst = 0;
+#define DN(h, d, cnt, T) \
+    if (!st && hipMemcpy(h, d, sizeof(T) * (cnt), hipMemcpyDeviceToHost) != hipSuccess) st = EVPGPU_ERR_HIP
+    DN(&kst, dst, 1, int);
+    DN(sig6, dsig, 6LL * n, double);
+    DN(edot6, ded, 6LL * n, double);
+    DN(gamdot12, dgd, 12LL * n, double);
+    DN(err2, der, 2LL * n, double);
+    DN(niter, dni, n, int);
+#undef DN
+    void *ptrs[] = {dph, dor, dni, dst, dsg, dsig, dstr, deps, dsc, dcr, ded, dgd, der, dpart};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
+        if (ptrs[i]) (void)hipFree(ptrs[i]);
+    if (!st && kst) st = kst;
+    return st;
+}
+
+} // extern "C"

@@ -1,0 +1,146 @@
+// evpgpu_impl.h -- internal state and kernel launchers of libevpgpu (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+
+#include "../../include/evpgpu.h"
+
+#define EVP_CHECK(x)                                                   \
+    do {                                                               \
+        hipError_t e_ = (x);                                           \
+        if (e_ != hipSuccess) return EVPGPU_ERR_HIP;                   \
+    } while (0)
+
+// Lebensohn-Tome basis constants (src/kinematics.c:22-24)
+#define RSQ2 0.70710678118654744
+#define RSQ3 0.57735026918962584
+#define RSQ6 0.40824829046386304
+
+// ---------------------------------------------------------------- params --
+// Per-launch constants of the per-voxel stress update (passed by value: they
+// land in SGPRs / the kernel-argument segment, uniform across the wave).
+struct NewtonParams {
+    double c066[36];
+    double dt;
+    double nrs[2][EVPGPU_NSYSMX];
+    double gam0[2][EVPGPU_NSYSMX];
+    int nsys[2];
+    int is_gas[2];
+    int int_n[2];    // integer rate exponent of the phase, 0 if not integral
+    int crss_sym;    // 1: forward and backward CRSS are equal in every voxel
+};
+
+struct GreenParams {
+    double c0[81];
+    double s0[81];
+    int elast_bc;
+};
+
+struct EpilogueParams {
+    double dt;
+    double udot_a[9];
+    int update_orient;
+    int harden;
+    int nsys[2];
+    int is_gas[2];
+    double wgt;
+    double hard[2][EVPGPU_NSYSMX][EVPGPU_NSYSMX];
+    double tau[2][EVPGPU_NSYSMX][3];
+    double thet[2][EVPGPU_NSYSMX][2];
+    double bn[2][EVPGPU_NSYSMX][3];
+    double bb[2][EVPGPU_NSYSMX][3];
+    double nrs[2][EVPGPU_NSYSMX];
+    double gam0[2][EVPGPU_NSYSMX];
+    int int_n[2];
+    int crss_sym;
+};
+
+// number of doubles each Newton block writes to the partials array:
+// err_s, err_e, newton iterations, sum sig (6), sum sig over phase 1 (6),
+// line-search trials
+#define NRED 16
+// epilogue partials: sum eps*wgt (6), sum edot*wgt (6)
+#define NRED_EPI 12
+
+struct evpgpu_ctx {
+    int device;
+    int nx, ny, nz, nzh;
+    long long N, NK;
+    int precision;
+    hipStream_t stream;
+
+    int nphases;
+    evpgpu_phase ph[2];
+    evpgpu_loading ld;
+    int have_phases, have_loading, have_ms, have_state;
+    int crss_sym;
+
+    // per-voxel fields (FP64, SoA component-major)
+    double *sig;      // 6N
+    double *E;        // 6N  symmetric part of DisGrad
+    double *W;        // 3N  antisymmetric part of DisGrad: (23, 13, 12)
+    double *epsp;     // 6N  plastic strain (Eps)
+    double *edot;     // 6N
+    double *tran;     // 9N  TranMat_xt2sa
+    double *crss;     // 24N (s*2+d)
+    double *gamacum;  // N
+    double *gamdot;   // 12N
+    double *schm;     // 60N (s*5+j), sample axes, refreshed per increment
+    int *phase, *grain, *orient;
+    double *sgr;      // n_orient x 36
+    int n_orient;
+
+    // spectral buffers
+    void *spec;       // 6*NK complex (double2 or float2)
+    double2 *acc;     // 3*NK complex: antisymmetric e-hat summed over the increment
+    double *rwork;    // 3N real scratch
+    double2 *tw64[3];
+    float2 *tw32[3];
+
+    // reductions
+    double *partials; // max(nblk) * NRED
+    double *red;      // NRED (device)
+    double *h_red;    // NRED pinned host
+    int *d_status;
+    int *h_status;
+
+    // kernel timing (HIP events), see evpgpu_set_timing
+    int timing;
+    hipEvent_t ev[5][2];
+    double t_ms[5];
+    long long t_n[5];
+
+    // macro state (host)
+    evpgpu_macro m;
+    double dD[9], dD_acum[9], dD_applied[9];
+    double udot[9], udot_s[9], udot_a[9], scauchy[9];
+    int creep_flag;
+    double disgrad_avg_actual[9];
+};
+
+// ---- launchers (defined in the .hip files) ----
+// fft.hip
+int fft_forward(evpgpu_ctx *c, const double *real_in, int ncomp, int precision);
+int fft_inverse_update(evpgpu_ctx *c, int ncomp, int precision, double *target,
+                       const double *add_const, double scale);
+int fft_inverse_real(evpgpu_ctx *c, const double2 *cin, int ncomp, double *out, double scale);
+int fft_twiddles(evpgpu_ctx *c);
+// green.hip
+int green_apply(evpgpu_ctx *c, int precision, int accumulate);
+int green_full9(evpgpu_ctx *c, const double2 *in6, double2 *out9);
+// newton.hip
+int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const double *E,
+                  const double *epsp, double *edot, const double *schm, const double *crss,
+                  double *gamdot, const int *phase, const int *orient, const double *sgr,
+                  long long n, double *partials, int *nblk_out, int *status,
+                  double *err_vox, int *niter_vox);
+int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out);
+// fields.hip
+int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *macro_sym6);
+int schmid_launch(evpgpu_ctx *c);
+int antisym_add_launch(evpgpu_ctx *c, const double *dW3);
+int epilogue_launch(evpgpu_ctx *c, const EpilogueParams &P, double *partials, int *nblk);
+int fill_zero(evpgpu_ctx *c, double *p, long long n);

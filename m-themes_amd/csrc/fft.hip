@@ -1,0 +1,362 @@
+// fft.hip -- Stockham 3-D R2C / C2R FFT for gfx950, FP64 and FP32.
+//
+// Replaces FFTW-MPI's fftwnd_mpi (src/evolution.c:1367,1399; plans at
+// src/init.c:752-757) with whole-grid-resident transforms.  The reference runs
+// 9 full complex transforms each way per Green iteration on real data; here
+// the stress components go through R2C along z (half spectrum kz = 0..nz/2),
+// then C2C along y and x, and come back through C2R, which gives
+// Re(IFFT(.)) of the Hermitian spectrum the Green step produces.
+//
+// Each pass moves every line of one axis through LDS:
+//   * a workgroup owns TB = 16 lines whose elements are adjacent in memory
+//     (consecutive kz for the strided y/x passes, consecutive rows for z), so
+//     every global load/store is a run of TB complex values;
+//   * LDS rows are padded to L+1 entries so the transposed accesses of the
+//     load/store phases spread over the banks;
+//   * the butterflies are Stockham autosort radix-8 stages (radix-4/2 last),
+//     each thread holding whole butterflies in registers, one LDS round trip
+//     per stage, no bit reversal.
+// Twiddles come from per-length tables computed on the host in double.
+#include <math.h>
+
+#include "evpgpu_impl.h"
+
+namespace {
+
+template <typename R> struct CT;
+template <> struct CT<double> { typedef double2 T; };
+template <> struct CT<float> { typedef float2 T; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return {a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename C> __device__ __forceinline__ C cmi(C a)
+{
+    if (INV) return {-a.y, a.x};
+    return {a.y, -a.x};
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void bf2(C &a, C &b)
+{
+    C t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void bf4(C v[4])
+{
+    C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    C s13 = cadd(v[1], v[3]), d13 = cmi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void bf8(C v[8])
+{
+    typedef decltype(v[0].x) R;
+    const R h = (R)0.70710678118654752440;
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
+    bf4<INV>(e);
+    bf4<INV>(o);
+    // o[k] *= W8^k, W8 = exp(-+ 2 pi i / 8)
+    C w1 = INV ? C{h, h} : C{h, -h};
+    C w3 = INV ? C{-h, h} : C{-h, -h};
+    o[1] = cmul(o[1], w1);
+    o[2] = cmi<INV>(o[2]);
+    o[3] = cmul(o[3], w3);
+    for (int k = 0; k < 4; k++) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+template <bool INV, int RAD, typename C> __device__ __forceinline__ void bfly(C v[RAD])
+{
+    if constexpr (RAD == 2) bf2<INV>(v[0], v[1]);
+    else if constexpr (RAD == 4) bf4<INV>(v);
+    else bf8<INV>(v);
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+// radix of stage s for length L: radix 8 first, then the 4 or 2 left over
+constexpr int radix_at(int L, int s)
+{
+    int m = ilog2(L), q = m / 3, r = m % 3;
+    return s < q ? 8 : (r == 2 ? 4 : 2);
+}
+constexpr int nstages(int L)
+{
+    int m = ilog2(L);
+    return m / 3 + (m % 3 ? 1 : 0);
+}
+
+constexpr int TB = 16;      // lines per workgroup
+constexpr int NT = 256;     // threads per workgroup
+
+// One Stockham stage over TB lines of length L held in LDS (pitch L+1).
+template <bool INV, int L, int RAD, int NS, typename C>
+__device__ __forceinline__ void stage(C *lds, const C *tw)
+{
+    constexpr int P = L + 1;
+    constexpr int NB = L / RAD;                 // butterflies per line
+    constexpr int TOT = NB * TB;
+    constexpr int PER = (TOT + NT - 1) / NT;    // butterflies per thread
+    C v[PER][RAD];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int b = threadIdx.x + q * NT;
+        if (b < TOT) {
+            int t = b / NB, j = b % NB;
+#pragma unroll
+            for (int r = 0; r < RAD; r++) v[q][r] = lds[t * P + j + r * NB];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int b = threadIdx.x + q * NT;
+        if (b < TOT) {
+            int t = b / NB, j = b % NB;
+            int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; r++) {
+                    C w = tw[(r * k * (L / (NS * RAD))) % L];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            bfly<INV, RAD>(v[q]);
+            int d = (j / NS) * NS * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; r++) lds[t * P + d + r * NS] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool INV, int L, int S, int NS, typename C>
+__device__ __forceinline__ void stages(C *lds, const C *tw)
+{
+    if constexpr (S < nstages(L)) {
+        constexpr int RAD = radix_at(L, S);
+        stage<INV, L, RAD, NS>(lds, tw);
+        stages<INV, L, S + 1, NS * RAD>(lds, tw);
+    }
+}
+
+// ---- strided C2C pass: lines along an axis with element stride `es`.
+// Workgroup (bx, o): lines inner = bx*TB .. +TB of group o, group base
+// (o / O2) * S1 + (o % O2) * S2.
+template <typename R, int L, bool INV>
+__global__ __launch_bounds__(NT) void k_fft_strided(typename CT<R>::T *data, int ninner,
+                                                    long long es, int O2, long long S1,
+                                                    long long S2, const typename CT<R>::T *tw)
+{
+    typedef typename CT<R>::T C;
+    constexpr int P = L + 1;
+    __shared__ C lds[TB * P];
+    const int t0 = blockIdx.x * TB;
+    const int o = blockIdx.y;
+    const long long base = (long long)(o / O2) * S1 + (long long)(o % O2) * S2 + t0;
+    const int nt = min(TB, ninner - t0);
+    for (int e = threadIdx.x; e < L * TB; e += NT) {
+        int i = e / TB, t = e % TB;
+        if (t < nt) lds[t * P + i] = data[base + i * es + t];
+    }
+    __syncthreads();
+    stages<INV, L, 0, 1>(lds, tw);
+    for (int e = threadIdx.x; e < L * TB; e += NT) {
+        int i = e / TB, t = e % TB;
+        if (t < nt) data[base + i * es + t] = lds[t * P + i];
+    }
+}
+
+// ---- z pass, real -> half spectrum.  in: ncomp x N real (FP64), out: ncomp x NK.
+template <typename R, int L>
+__global__ __launch_bounds__(NT) void k_fft_z_r2c(const double *in, typename CT<R>::T *out,
+                                                  long long N, long long NK,
+                                                  const typename CT<R>::T *tw)
+{
+    typedef typename CT<R>::T C;
+    constexpr int P = L + 1, LH = L / 2 + 1;
+    __shared__ C lds[TB * P];
+    const int comp = blockIdx.y;
+    const long long line0 = (long long)blockIdx.x * TB;
+    const double *src = in + comp * N + line0 * L;
+    for (int e = threadIdx.x; e < L * TB; e += NT) {
+        int t = e / L, i = e % L;
+        lds[t * P + i] = C{(R)src[e], (R)0};
+    }
+    __syncthreads();
+    stages<false, L, 0, 1>(lds, tw);
+    C *dst = out + comp * NK + line0 * LH;
+    for (int e = threadIdx.x; e < LH * TB; e += NT) {
+        int t = e / LH, k = e % LH;
+        dst[e] = lds[t * P + k];
+    }
+}
+
+// ---- z pass, half spectrum -> real, Re(IFFT) of the Hermitian extension.
+// MODE 0: target[c] += addc[c] + re*scale  (the strain update of
+//         src/evolution.c:1400-1401);  MODE 1: target[c] = re*scale.
+template <typename R, int L, int MODE>
+__global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, double *target,
+                                                  long long N, long long NK, double a0, double a1,
+                                                  double a2, double a3, double a4, double a5,
+                                                  double scale, const typename CT<R>::T *tw)
+{
+    typedef typename CT<R>::T C;
+    constexpr int P = L + 1, LH = L / 2 + 1;
+    __shared__ C lds[TB * P];
+    const int comp = blockIdx.y;
+    const long long line0 = (long long)blockIdx.x * TB;
+    const C *src = in + comp * NK + line0 * LH;
+    for (int e = threadIdx.x; e < LH * TB; e += NT) {
+        int t = e / LH, k = e % LH;
+        C x = src[e];
+        lds[t * P + k] = x;
+        if (k > 0 && k < L / 2) lds[t * P + L - k] = C{x.x, -x.y};
+    }
+    __syncthreads();
+    stages<true, L, 0, 1>(lds, tw);
+    double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
+    double *dst = target + comp * N + line0 * L;
+    for (int e = threadIdx.x; e < L * TB; e += NT) {
+        int t = e / L, i = e % L;
+        double re = (double)lds[t * P + i].x;
+        if (MODE == 0) dst[e] += add + re * scale;
+        else dst[e] = re * scale;
+    }
+}
+
+template <typename R> const typename CT<R>::T *twp(evpgpu_ctx *c, int axis);
+template <> const double2 *twp<double>(evpgpu_ctx *c, int axis) { return c->tw64[axis]; }
+template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[axis]; }
+
+// dispatch on a runtime length
+#define FFT_DISPATCH(L_, CALL) \
+    switch (L_) {              \
+    case 8: CALL(8); break;    \
+    case 16: CALL(16); break;  \
+    case 32: CALL(32); break;  \
+    case 64: CALL(64); break;  \
+    case 128: CALL(128); break; \
+    case 256: CALL(256); break; \
+    case 512: CALL(512); break; \
+    default: return EVPGPU_ERR_SIZE; \
+    }
+
+template <typename R, bool INV>
+int pass_y(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
+{
+    dim3 g((c->nzh + TB - 1) / TB, ncomp * c->nx);
+#define CALLY(L) hipLaunchKernelGGL((k_fft_strided<R, L, INV>), g, dim3(NT), 0, c->stream, d, c->nzh, \
+                                    (long long)c->nzh, 1, (long long)c->ny * c->nzh, 0LL, twp<R>(c, 1))
+    FFT_DISPATCH(c->ny, CALLY);
+#undef CALLY
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+template <typename R, bool INV>
+int pass_x(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
+{
+    dim3 g((c->nzh + TB - 1) / TB, ncomp * c->ny);
+#define CALLX(L) hipLaunchKernelGGL((k_fft_strided<R, L, INV>), g, dim3(NT), 0, c->stream, d, c->nzh, \
+                                    (long long)c->ny * c->nzh, c->ny, (long long)c->NK,             \
+                                    (long long)c->nzh, twp<R>(c, 0))
+    FFT_DISPATCH(c->nx, CALLX);
+#undef CALLX
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+template <typename R>
+int fwd(evpgpu_ctx *c, const double *real_in, int ncomp, typename CT<R>::T *spec)
+{
+    dim3 g((unsigned)(c->nx * c->ny / TB), ncomp);
+#define CALLZ(L) hipLaunchKernelGGL((k_fft_z_r2c<R, L>), g, dim3(NT), 0, c->stream, real_in, spec, \
+                                    c->N, c->NK, twp<R>(c, 2))
+    FFT_DISPATCH(c->nz, CALLZ);
+#undef CALLZ
+    int st = pass_y<R, false>(c, spec, ncomp);
+    if (st) return st;
+    return pass_x<R, false>(c, spec, ncomp);
+}
+
+template <typename R, int MODE>
+int inv(evpgpu_ctx *c, typename CT<R>::T *spec, int ncomp, double *target, const double *addc,
+        double scale)
+{
+    int st = pass_x<R, true>(c, spec, ncomp);
+    if (st) return st;
+    st = pass_y<R, true>(c, spec, ncomp);
+    if (st) return st;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    if (addc)
+        for (int i = 0; i < ncomp && i < 6; i++) a[i] = addc[i];
+    dim3 g((unsigned)(c->nx * c->ny / TB), ncomp);
+#define CALLZI(L) hipLaunchKernelGGL((k_fft_z_c2r<R, L, MODE>), g, dim3(NT), 0, c->stream, spec, target, \
+                                     c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5], scale,        \
+                                     twp<R>(c, 2))
+    FFT_DISPATCH(c->nz, CALLZI);
+#undef CALLZI
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+} // namespace
+
+int fft_twiddles(evpgpu_ctx *c)
+{
+    int dims[3] = {c->nx, c->ny, c->nz};
+    for (int a = 0; a < 3; a++) {
+        int L = dims[a];
+        double2 *h64 = (double2 *)malloc(sizeof(double2) * L);
+        float2 *h32 = (float2 *)malloc(sizeof(float2) * L);
+        for (int m = 0; m < L; m++) {
+            // exp(-2 pi i m / L); the quarter-period values are exact
+            double ang = 2.0 * M_PI * (double)m / (double)L;
+            double cr = cos(ang), si = -sin(ang);
+            if (4 * m == L) { cr = 0.0; si = -1.0; }
+            if (2 * m == L) { cr = -1.0; si = 0.0; }
+            if (4 * m == 3 * L) { cr = 0.0; si = 1.0; }
+            h64[m] = make_double2(cr, si);
+            h32[m] = make_float2((float)cr, (float)si);
+        }
+        EVP_CHECK(hipMalloc(&c->tw64[a], sizeof(double2) * L));
+        EVP_CHECK(hipMalloc(&c->tw32[a], sizeof(float2) * L));
+        EVP_CHECK(hipMemcpy(c->tw64[a], h64, sizeof(double2) * L, hipMemcpyHostToDevice));
+        EVP_CHECK(hipMemcpy(c->tw32[a], h32, sizeof(float2) * L, hipMemcpyHostToDevice));
+        free(h64);
+        free(h32);
+    }
+    return EVPGPU_OK;
+}
+
+// forward transform of ncomp real FP64 fields into c->spec (precision chosen)
+int fft_forward(evpgpu_ctx *c, const double *real_in, int ncomp, int precision)
+{
+    if (precision == EVPGPU_MIXED) return fwd<float>(c, real_in, ncomp, (float2 *)c->spec);
+    return fwd<double>(c, real_in, ncomp, (double2 *)c->spec);
+}
+
+// inverse of c->spec (ncomp comps) accumulated into target: t += add[c] + re*scale
+int fft_inverse_update(evpgpu_ctx *c, int ncomp, int precision, double *target,
+                       const double *add_const, double scale)
+{
+    if (precision == EVPGPU_MIXED)
+        return inv<float, 0>(c, (float2 *)c->spec, ncomp, target, add_const, scale);
+    return inv<double, 0>(c, (double2 *)c->spec, ncomp, target, add_const, scale);
+}
+
+// inverse of an FP64 spectrum (destroyed) into out = re*scale
+int fft_inverse_real(evpgpu_ctx *c, const double2 *cin, int ncomp, double *out, double scale)
+{
+    return inv<double, 1>(c, (double2 *)cin, ncomp, out, nullptr, scale);
+}

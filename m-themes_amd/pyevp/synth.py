@@ -222,15 +222,33 @@ def microstructure(cfg: Config):
 
 
 def write_ms(path: str, n: int, euler: np.ndarray, grain: np.ndarray, phase: np.ndarray):
-    """Write the 8-column microstructure file (each line < 80 characters)."""
-    idx = np.arange(n ** 3)
-    xs = idx // (n * n) + 1
-    ys = (idx // n) % n + 1
-    zs = idx % n + 1
-    cols = np.column_stack([euler, xs, ys, zs, grain, phase])
+    """Write the 8-column microstructure file (each line < 80 characters).
+
+    Lines are ``phi1 Phi phi2 x y z grain phase`` with the angles to 8 decimals;
+    when every grain carries one orientation (the synthetic case) the angle text
+    is formatted once per grain.
+    """
+    N = n ** 3
+    idx = np.arange(N)
+    xs = (idx // (n * n) + 1).tolist()
+    ys = ((idx // n) % n + 1).tolist()
+    zs = (idx % n + 1).tolist()
+    g = grain.tolist()
+    p = phase.tolist()
+    uniq, first = np.unique(grain, return_index=True)
+    per_grain = np.all(euler[first][np.searchsorted(uniq, grain)] == euler)
     with open(path, "w") as f:
-        np.savetxt(f, cols, fmt=["%.8f", "%.8f", "%.8f", "%d", "%d", "%d", "%d", "%d"],
-                   delimiter=" ")
+        if per_grain:
+            txt = {int(u): "%.8f %.8f %.8f" % tuple(euler[i]) for u, i in zip(uniq, first)}
+            step = 1 << 18
+            for a in range(0, N, step):
+                b = min(N, a + step)
+                f.write("".join(f"{txt[g[v]]} {xs[v]} {ys[v]} {zs[v]} {g[v]} {p[v]}\n"
+                                for v in range(a, b)))
+        else:
+            cols = np.column_stack([euler, xs, ys, zs, grain, phase])
+            np.savetxt(f, cols, fmt=["%.8f", "%.8f", "%.8f", "%d", "%d", "%d", "%d", "%d"],
+                       delimiter=" ")
 
 
 def write_sx(path: str, m: Material):

@@ -840,40 +840,57 @@ static int parse_sx(orc_job *J, const char *name, int ph)
     return ORC_OK;
 }
 
-/* InitMicroStruct (src/init.c:233-396), single slab */
+/* InitMicroStruct (src/init.c:233-396), single slab.  The file is read in
+ * order; the per-voxel stiffness rotations then run in parallel and C066 is
+ * summed in voxel order afterwards, as the reference's loop does.            */
 static int read_ms(orc_job *J)
 {
     char buf[80] = {0};
     FILE *fp = open_rel(J, J->initial_ms);
     if (!fp) return ORC_ERR_FILE;
     int nph1 = 0, Ny = J->CellDim[1], Nz = J->CellDim[2];
-    double c066l[6][6] = {{0}};
+    double (*eul)[3] = calloc((size_t)J->Nxyz, sizeof(*eul));
     for (int v = 0; v < J->Nxyz; v++) {
         double ph = 0, th = 0, om = 0;
         int ii = 0, jj = 0, kk = 0, gr = 0, jp = 0;
         if (!fgets(buf, 80, fp)) buf[0] = 0;
         sscanf(buf, "%lf %lf %lf %d %d %d %d %d", &ph, &th, &om, &ii, &jj, &kk, &gr, &jp);
         int idx = ((ii - 1) * Ny + jj - 1) * Nz + kk - 1;
-        if (idx != v) { fclose(fp); return ORC_ERR_MSINDEX; }
+        if (idx != v || jp < 1 || jp > J->N_phases) {
+            fclose(fp);
+            free(eul);
+            return ORC_ERR_MSINDEX;
+        }
         if (jp == 1) nph1++;
         J->grain[v] = gr;
         J->phase[v] = jp;
-        if (jp < 1 || jp > J->N_phases) { fclose(fp); return ORC_ERR_MSINDEX; }
-        if (!J->Type_phases[jp - 1]) {
-            ph *= PI_REF / 180.;
-            th *= PI_REF / 180.;
-            om *= PI_REF / 180.;
-            double sa2xt[3][3], c4[3][3][3][3];
-            orc_euler_to_mat(ph, th, om, sa2xt);
-            for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++) J->TranMat[v][a][b] = sa2xt[b][a];
-            orc_ten4_transform_inv(c4, sa2xt, J->Cijkl[jp - 1]);
-            orc_t4_to_c2(c4, J->C_gr[v]);
-            for (int a = 0; a < 6; a++)
-                for (int b = 0; b < 6; b++) c066l[a][b] += J->C_gr[v][a][b];
-        }
+        eul[v][0] = ph;
+        eul[v][1] = th;
+        eul[v][2] = om;
     }
     fclose(fp);
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < J->Nxyz; v++) {
+        int jp = J->phase[v];
+        if (J->Type_phases[jp - 1]) continue;
+        double ph = eul[v][0], th = eul[v][1], om = eul[v][2];
+        ph *= PI_REF / 180.;
+        th *= PI_REF / 180.;
+        om *= PI_REF / 180.;
+        double sa2xt[3][3], c4[3][3][3][3];
+        orc_euler_to_mat(ph, th, om, sa2xt);
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) J->TranMat[v][a][b] = sa2xt[b][a];
+        orc_ten4_transform_inv(c4, sa2xt, J->Cijkl[jp - 1]);
+        orc_t4_to_c2(c4, J->C_gr[v]);
+    }
+    free(eul);
+    double c066l[6][6] = {{0}};
+    for (int v = 0; v < J->Nxyz; v++) {
+        if (J->Type_phases[J->phase[v] - 1]) continue;
+        for (int a = 0; a < 6; a++)
+            for (int b = 0; b < 6; b++) c066l[a][b] += J->C_gr[v][a][b];
+    }
     J->Wgt_ph1 = (1.0 * (double)nph1) / J->Nxyz;
     for (int a = 0; a < 6; a++)
         for (int b = 0; b < 6; b++) J->C066[a][b] = c066l[a][b] * J->WGT;
@@ -975,6 +992,8 @@ orc_job *orc_job_setup(const char *input_path, int *status)
     voigt_to_full(J->Stress_BC, J->Scauchy);
 
     /* CRSS (src/init.c:1135-1163); sg = LU_inv_66(C_gr) is loop-invariant */
+    int st_sg = ORC_OK;
+#pragma omp parallel for schedule(static)
     for (int v = 0; v < N; v++) {
         int jp = J->phase[v];
         J->gamacum[v] = 0.0;
@@ -984,10 +1003,15 @@ orc_job *orc_job_setup(const char *input_path, int *status)
                 J->crss[v][i][1] = J->tau[jp - 1][i][1];
             }
             memcpy(J->sg[v], J->C_gr[v], sizeof(J->sg[v]));
-            if ((st = orc_lu_inv66(J->sg[v]))) goto fail;
+            int s2 = orc_lu_inv66(J->sg[v]);
+            if (s2) {
+#pragma omp critical
+                st_sg = s2;
+            }
             orc_rotate_schmid(J->TranMat[v], J->nSYS[jp - 1], J->Schm_xt[jp - 1], J->Schm_gr[v]);
         }
     }
+    if ((st = st_sg)) goto fail;
     /* InitFields (src/init.c:444-515) */
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) J->DisGradAvg[a][b] = J->Udot[a][b] * J->TimeStep;
@@ -1359,6 +1383,15 @@ int orc_job_run(orc_job *J, int n, int *iters, const int *forced)
 }
 
 int orc_job_nvox(const orc_job *J) { return J->Nxyz; }
+
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 void orc_job_dims(const orc_job *J, int d[3]) { memcpy(d, J->CellDim, sizeof(J->CellDim)); }
 int *orc_job_grain(orc_job *J) { return J->grain; }
 int *orc_job_phase(orc_job *J) { return J->phase; }

@@ -42,6 +42,7 @@ def lib():
         L.orc_job_free.argtypes = [C.c_void_p]
         L.orc_job_run.argtypes = [C.c_void_p, C.c_int, IP, IP]
         L.orc_job_nvox.argtypes = [C.c_void_p]
+        L.orc_threads.argtypes = []
         L.orc_job_dims.argtypes = [C.c_void_p, IP]
         L.orc_job_field.restype = DP
         L.orc_job_field.argtypes = [C.c_void_p, C.c_int]
@@ -272,3 +273,7 @@ def orientation(tran, rot):
     rot = np.array(rot, dtype=np.float64, order="C")
     lib().orc_orientation(ptr(tran), ptr(rot))
     return tran, rot
+
+
+def threads() -> int:
+    return lib().orc_threads()

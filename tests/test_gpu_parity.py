@@ -1,0 +1,213 @@
+"""Parity of the HIP path (through the C ABI) against the oracle and numpy.
+
+Tolerances (FP64 path): FFTs 1e-12 of the spectrum's max; Green operator 1e-12;
+per-voxel Newton 1e-12 on stress with equal iteration counts; whole loop 1e-10
+normwise-relative on stress, strain rate and the macroscopic curve, with equal
+Green iteration counts per increment.  Mixed path: 1e-6 on stress.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pyevp import lib
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ---------------------------------------------------------------- FFT ----
+@pytest.mark.parametrize("dims", [(8, 8, 8), (16, 16, 16), (32, 16, 8), (8, 32, 64), (64, 64, 64),
+                                  (128, 128, 128), (256, 32, 16)])
+def test_fft_r2c_fp64_vs_numpy(dims):
+    x = np.random.default_rng(1).standard_normal(dims)
+    ctx = lib.Context(*dims)
+    assert rel(ctx.rfftn(x), np.fft.rfftn(x)) < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8), (32, 16, 8), (128, 128, 128)])
+def test_fft_c2r_fp64_vs_numpy(dims):
+    rng = np.random.default_rng(2)
+    nh = dims[2] // 2 + 1
+    # arbitrary half spectrum, including imaginary parts on the kz = 0 and nz/2 planes
+    X = rng.standard_normal(dims[:2] + (nh,)) + 1j * rng.standard_normal(dims[:2] + (nh,))
+    ctx = lib.Context(*dims)
+    n = np.prod(dims)
+    ref = np.fft.irfftn(X, s=dims) * n
+    assert rel(ctx.irfftn_unnormalised(X), ref) < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (128, 128, 128)])
+def test_fft_fp32_vs_numpy(dims):
+    x = np.random.default_rng(3).standard_normal(dims)
+    ctx = lib.Context(*dims, precision=lib.MIXED)
+    assert rel(ctx.rfftn(x), np.fft.rfftn(x)) < 2e-6
+    X = np.fft.rfftn(x)
+    assert rel(ctx.irfftn_unnormalised(X), x * np.prod(dims)) < 2e-6
+
+
+# -------------------------------------------------------------- Green ----
+def _loading_from(job):
+    c066, s066, c0, s0 = job.setup_info()
+    ld = lib.Loading()
+    ld.c066[:] = c066.ravel()
+    ld.s066[:] = s066.ravel()
+    ld.c0[:] = c0.ravel()
+    ld.s0[:] = s0.ravel()
+    ld.dt = 1e-3
+    return ld
+
+
+@pytest.mark.parametrize("elast_bc", [1, 0])
+@pytest.mark.parametrize("n", [8, 16])
+def test_green_vs_oracle(job_factory, n, elast_bc):
+    """GPU Gamma_eff on the half spectrum == Hermitian part of the oracle's
+    Gamma on the full spectrum (what Re(IFFT) keeps)."""
+    dims = (n, n, n)
+    inp = job_factory(f"green{n}_{elast_bc}", n=n, grains=4, seed=11, increments=1,
+                      elast_bc=elast_bc)
+    J = O.Job(inp)
+    rng = np.random.default_rng(5)
+    s6 = rng.standard_normal((6,) + dims)
+    full = np.empty(dims + (3, 3))
+    idx = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    for c, (a, b) in enumerate(idx):
+        full[..., a, b] = full[..., b, a] = s6[c]
+    F = np.fft.fftn(full, axes=(0, 1, 2))
+    orr, oi = J.green(F.real.reshape(-1, 3, 3), F.imag.reshape(-1, 3, 3))
+    X = (orr + 1j * oi).reshape(dims + (3, 3))
+    # Hermitian part H(k) = (X(k) + conj(X(-k)))/2
+    m = [(-np.arange(d)) % d for d in dims]
+    Xm = np.conj(X[m[0]][:, m[1]][:, :, m[2]])
+    H = 0.5 * (X + Xm)
+    nh = n // 2 + 1
+    ctx = lib.Context(*dims)
+    ld = _loading_from(J)
+    ld.elast_bc = elast_bc
+    ctx.set_loading(ld)
+    Sh = np.stack([np.fft.rfftn(s6[c]) for c in range(6)])
+    E = ctx.green(Sh)
+    for a in range(3):
+        for b in range(3):
+            assert rel(E[a * 3 + b], H[:, :, :nh, a, b]) < 1e-12, (a, b)
+
+
+# ------------------------------------------------------------- Newton ----
+def _newton_batch(J, gj, rng, perturb):
+    """Per-voxel inputs from an oracle state, with the strain perturbed so the
+    Newton solve has work to do."""
+    n = J.n
+    sig = J.field(O.SIG)
+    dg = J.field(O.DISGRAD)
+    eps = J.field(O.EPS)
+    tran = J.field(O.TRANMAT)
+    crss = J.field(O.CRSS)
+    ph = J.phase()
+    cgr = J.cgr()
+    c066, _, _, _ = J.setup_info()
+    strain = 0.5 * (dg + dg.transpose(0, 2, 1))
+    strain = strain + perturb * rng.standard_normal((n, 3, 3))
+    strain = 0.5 * (strain + strain.transpose(0, 2, 1))
+    return sig, strain, eps, tran, crss, ph, cgr, c066
+
+
+def test_newton_vs_oracle(job_factory):
+    inp = job_factory("newton8", n=8, grains=6, seed=21, increments=3, pore_fraction=0.05,
+                      pore_radius=1)
+    J = O.Job(inp)
+    J.run(3)
+    gj = lib.GpuJob(inp)
+    rng = np.random.default_rng(7)
+    sig, strain, eps, tran, crss, ph, cgr, c066 = _newton_batch(J, gj, rng, 2e-4)
+    xt_schm = None
+    # crystal voxels only
+    sel = np.where(ph == 1)[0]
+    m = len(sel)
+    # Schmid tensors in sample axes: rotate the crystal-axis tensors with the oracle
+    import oracle as Om
+    from pyevp import synth
+    nb = np.array([s[0] for s in synth.FCC_SLIP], float)
+    bb = np.array([s[1] for s in synth.FCC_SLIP], float)
+    nb /= np.linalg.norm(nb, axis=1)[:, None]
+    bb /= np.linalg.norm(bb, axis=1)[:, None]
+    xt_schm = np.array([Om.t2_to_v5(0.5 * (np.outer(nb[s], bb[s]) + np.outer(bb[s], nb[s])))
+                        for s in range(12)])
+    schm = np.array([Om.rotate_schmid(tran[v], xt_schm) for v in sel])       # m,12,5
+    sg = np.array([Om.lu_inv66(cgr[v]) for v in sel])
+    mat = synth.Material()
+    nsr = np.full(12, mat.nrs)
+    g0 = np.full(12, mat.gam0)
+    # oracle per voxel
+    ref = [Om.update_stress_voxel(sg[i], c066, 1e-3, sig[v], eps[v], strain[v], schm[i], crss[v],
+                                  nsr, g0) for i, v in enumerate(sel)]
+    assert all(r[0] == 0 for r in ref)
+    # GPU batch (component-major arrays)
+    def sym6(t):
+        return np.stack([t[:, 0, 0], t[:, 1, 1], t[:, 2, 2], t[:, 1, 2], t[:, 0, 2], t[:, 0, 1]])
+    st, s_out, ed, gd, err, nit = lib.Context.newton(
+        gj_ctx(gj), np.ones(m, np.int32), np.arange(m, dtype=np.int32), sg.reshape(m, 36),
+        sym6(sig[sel]), sym6(strain[sel]), sym6(eps[sel]),
+        schm.reshape(m, 60).T.copy(), crss[sel].reshape(m, 24).T.copy())
+    assert st == 0
+    s_ref = np.array([sym6(r[1][None])[:, 0] for r in ref]).T
+    e_ref = np.array([sym6(r[2][None])[:, 0] for r in ref]).T
+    g_ref = np.array([r[3] for r in ref]).T
+    assert np.array_equal(nit, np.array([r[6] for r in ref]))
+    assert rel(s_out, s_ref) < 1e-12
+    assert rel(ed, e_ref) < 1e-10
+    assert rel(gd, g_ref) < 1e-10
+    assert rel(err[:, 0], np.array([r[4] for r in ref])) < 1e-8
+
+
+class _CtxView:
+    def __init__(self, h):
+        self.h = h
+
+
+def gj_ctx(gj):
+    return _CtxView(gj.ctx)
+
+
+# --------------------------------------------------------- whole loop ----
+def _compare_jobs(J, gj, k, tol=1e-10):
+    it_o = J.run(k)
+    it_g, _ = gj.run(k)
+    assert list(it_g) == list(it_o), (it_g, it_o)
+    s = gj.state()
+    assert rel(s["Sig"], J.field(O.SIG)) < tol
+    assert rel(s["Edot"], J.field(O.EDOT)) < tol
+    assert rel(s["Eps"], J.field(O.EPS)) < tol
+    assert rel(s["DisGrad"], J.field(O.DISGRAD)) < tol
+    assert rel(s["TranMat"], J.field(O.TRANMAT)) < tol
+    assert rel(s["crss"], J.field(O.CRSS)) < tol
+    mo, mg = J.macro(), gj.macro()
+    assert rel(mg["sig_avg"], mo["SigAvg"]) < tol
+    for a, b in (("e_vm", "e_vm"), ("s_vm", "s_vm"), ("evmp", "evmp"), ("dvmp", "dvmp")):
+        assert abs(mg[a] - mo[b]) <= tol * max(abs(mo[b]), 1e-300), a
+    g, p = gj.ids()
+    assert np.array_equal(g, J.grain()) and np.array_equal(p, J.phase())
+
+
+def test_loop_parity_8(job_factory):
+    inp = job_factory("loop8", n=8, grains=4, seed=7, increments=5)
+    _compare_jobs(O.Job(inp), lib.GpuJob(inp), 5)
+
+
+def test_loop_parity_16_pores(job_factory):
+    inp = job_factory("loop16p", n=16, grains=8, seed=9, increments=3, pore_fraction=0.03,
+                      pore_radius=2)
+    _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
+
+
+def test_mixed_precision_stress_error(job_factory):
+    inp = job_factory("mixed16", n=16, grains=8, seed=13, increments=3)
+    a = lib.GpuJob(inp, lib.FP64)
+    b = lib.GpuJob(inp, lib.MIXED)
+    a.run(3)
+    it, stats = b.run(3)
+    assert all(s["fp64_iters"] >= 1 for s in stats)
+    sa, sb = a.state(), b.state()
+    assert rel(sb["Sig"], sa["Sig"]) < 1e-6
+    assert rel(b.macro()["sig_avg"], a.macro()["sig_avg"]) < 1e-6
