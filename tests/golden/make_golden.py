@@ -1,0 +1,51 @@
+"""Write the golden fixtures the GPU parity tests compare against.
+
+Runs the CPU oracle (oracle/) on
+  * the reference's own test_run/ inputs (input.in, Cu_p.sx, the 32^3
+    pore microstructure), all N_steps = 2 increments;
+  * a 32^3 synthetic job (64 Voronoi grains, seed 0x5EED0001), 3 increments;
+and stores per-increment Green iteration counts, macroscopic stress and the
+vm.out row, plus the final stress / strain-rate / displacement-gradient /
+CRSS fields on every 16th voxel.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "m-themes_amd"))
+
+import oracle as O  # noqa: E402
+from pyevp import synth  # noqa: E402
+
+STRIDE = 16
+
+
+def record(inp, nsteps, out):
+    J = O.Job(inp)
+    iters, sig_avg, vm = [], [], []
+    for _ in range(nsteps):
+        iters.append(int(J.run(1)[0]))
+        m = J.macro()
+        sig_avg.append(m["SigAvg"].ravel())
+        vm.append([m[k] for k in ("TimeTot", "e_vm", "evmp", "d_vm", "dvmp", "s_vm", "s_vm1")])
+    sel = np.arange(0, J.n, STRIDE)
+    np.savez_compressed(
+        out, iters=np.array(iters), sig_avg=np.array(sig_avg), vm=np.array(vm), sel=sel,
+        sig=J.field(O.SIG)[sel], edot=J.field(O.EDOT)[sel], disgrad=J.field(O.DISGRAD)[sel],
+        crss=J.field(O.CRSS)[sel], tranmat=J.field(O.TRANMAT)[sel], grain=J.grain(),
+        phase=J.phase())
+    print(out, iters, flush=True)
+
+
+if __name__ == "__main__":
+    record(os.path.join(HERE, "test_run", "input.in"), 2, os.path.join(HERE, "oracle_test_run.npz"))
+    with tempfile.TemporaryDirectory() as d:
+        inp = synth.write_job(d, synth.config_c1())
+        record(inp, 3, os.path.join(HERE, "oracle_c1_32.npz"))

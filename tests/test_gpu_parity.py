@@ -211,3 +211,67 @@ def test_mixed_precision_stress_error(job_factory):
     sa, sb = a.state(), b.state()
     assert rel(sb["Sig"], sa["Sig"]) < 1e-6
     assert rel(b.macro()["sig_avg"], a.macro()["sig_avg"]) < 1e-6
+
+
+# ------------------------------------------------------ golden fixtures ----
+def _golden_compare(inp, fx, nsteps, tol=1e-10):
+    import os
+    g = np.load(fx)
+    gj = lib.GpuJob(inp)
+    iters, sig_avg, vm = [], [], []
+    for _ in range(nsteps):
+        it, _ = gj.run(1)
+        iters.append(int(it[0]))
+        m = gj.macro()
+        sig_avg.append(m["sig_avg"].ravel())
+        vm.append([m[k] for k in ("time_tot", "e_vm", "evmp", "d_vm", "dvmp", "s_vm", "s_vm1")])
+    assert iters == list(g["iters"]), (iters, list(g["iters"]))
+    assert rel(np.array(sig_avg), g["sig_avg"]) < tol
+    assert rel(np.array(vm), g["vm"]) < tol
+    s = gj.state()
+    sel = g["sel"]
+    assert rel(s["Sig"][sel], g["sig"]) < tol
+    assert rel(s["Edot"][sel], g["edot"]) < tol
+    assert rel(s["DisGrad"][sel], g["disgrad"]) < tol
+    assert rel(s["crss"][sel], g["crss"]) < tol
+    assert rel(s["TranMat"][sel], g["tranmat"]) < tol
+    gr, ph = gj.ids()
+    assert np.array_equal(gr, g["grain"]) and np.array_equal(ph, g["phase"])
+
+
+def test_golden_reference_test_run():
+    """The reference's own test_run inputs (32^3, 24 % pores), both increments."""
+    import os
+    from conftest import GOLDEN
+    _golden_compare(os.path.join(GOLDEN, "test_run", "input.in"),
+                    os.path.join(GOLDEN, "oracle_test_run.npz"), 2)
+
+
+def test_golden_c1_32(tmp_path):
+    """32^3, 64 grains (the CPU-runnable configuration), first 3 increments."""
+    import os
+    from conftest import GOLDEN
+    from pyevp import synth
+    inp = synth.write_job(str(tmp_path), synth.config_c1())
+    _golden_compare(inp, os.path.join(GOLDEN, "oracle_c1_32.npz"), 3)
+
+
+def test_forge_dropin_writes_vm_out(tmp_path):
+    """`forge input.in` end to end: the namelist echo and the vm.out rows."""
+    import os
+    import shutil
+    import subprocess
+    from conftest import GOLDEN
+    for f in ("input.in", "Cu_p.sx", "FFT_microstructure_pore_197.txt"):
+        shutil.copy(os.path.join(GOLDEN, "test_run", f), tmp_path / f)
+    r = subprocess.run([os.path.join(lib.LIBDIR, "forge"), "input.in"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "NameList for input data" in r.stdout
+    rows = open(tmp_path / "vm.out").read().splitlines()
+    assert rows[0] == "TIME\tEVM\tEVMP\tDVM\tDVMP\tSVM\tSVMSVM_1"
+    vals = np.array([[float(x) for x in ln.split("\t")] for ln in rows[1:]])
+    g = np.load(os.path.join(GOLDEN, "oracle_test_run.npz"))
+    assert vals.shape == (2, 7)
+    # vm.out carries %e (7 significant digits)
+    assert np.allclose(vals, g["vm"], rtol=1e-6, atol=0)
