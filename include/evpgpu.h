@@ -96,10 +96,13 @@ typedef struct {
 
 const char *evpgpu_strerror(int status);
 
-/* Create a context for an nx*ny*nz grid (each a power of 2 in [8, 512]). */
+/* Create a context for an nx*ny*nz grid (each a power of 2 in [8, 256]). */
 int evpgpu_create(evpgpu_ctx **ctx, int nx, int ny, int nz, int precision);
 int evpgpu_destroy(evpgpu_ctx *ctx);
 int evpgpu_set_precision(evpgpu_ctx *ctx, int precision);
+/* Mixed precision: iterations between FP64 transforms of the stress anchor
+ * (default 16, at least 1).  No counterpart in the reference (FP64 only). */
+int evpgpu_set_anchor_interval(evpgpu_ctx *ctx, int iterations);
 
 int evpgpu_set_phases(evpgpu_ctx *ctx, const evpgpu_phase *phases, int nphases);
 int evpgpu_set_loading(evpgpu_ctx *ctx, const evpgpu_loading *ld);
@@ -144,12 +147,13 @@ int evpgpu_fft_c2r_host(evpgpu_ctx *ctx, const double *in_cplx, double *out);
 int evpgpu_green_host(evpgpu_ctx *ctx, const double *sig_hat6, double *e_hat9);
 /* Per-voxel stress update (src/evolution.c:44-310) on a host batch of n voxels
  * of the context's phases: inputs lambda (sig6), strain (6, symmetric part of the
- * displacement gradient), plastic strain eps6, Schmid schm60 (s*5+j, sample axes),
- * crss24, phase; sgr/orient as for the microstructure.  Outputs sig6 (in place),
- * edot6, gamdot12, err2 (err_s, err_e per voxel) and niter.                   */
+ * displacement gradient), plastic strain eps6, orientation tranmat9 (crystal ->
+ * sample, the Schmid tensors follow from it as update_schmid does), crss24,
+ * phase; sgr/orient as for the microstructure.  Outputs sig6 (in place), edot6,
+ * gamdot12, err2 (err_s, err_e per voxel) and niter.                          */
 int evpgpu_newton_host(evpgpu_ctx *ctx, int n, const int *phase, const int *orient,
                        const double *sgr, int n_orient, double *sig6, const double *strain6,
-                       const double *eps6, const double *schm60, const double *crss24,
+                       const double *eps6, const double *tranmat9, const double *crss24,
                        double *edot6, double *gamdot12, double *err2, int *niter);
 
 #ifdef __cplusplus

@@ -5,10 +5,20 @@
 //   R2C(sigma, 6 comps) -> Gamma_eff (in place, antisym part accumulated)
 //   -> C2R fused with  E += sym(dDisGradAvg) + re/N
 //   -> per-voxel Newton update + block partials -> one deterministic row sum
-// and the host reads 15 doubles back to finish get_smacro (src/evolution.c:
-// 473-547), the error normalisation and the loop test.  In mixed precision the
-// FFT/Green/C2R steps run in FP32 until the loop converges, then FP64 sweeps
-// run until the FP64 residual converges again (at least one sweep).
+// and the host reads 16 doubles back to finish get_smacro (src/evolution.c:
+// 473-547), the error normalisation and the loop test.
+//
+// Mixed precision splits sigma = sigma_A + delta.  Every anchor_every
+// iterations sigma_A = fp32(sigma) is transformed in FP64 and its Green
+// correction kept in FP32 (real space, cA, and the antisymmetric spectrum,
+// skA); in between only delta, which shrinks as the loop converges, passes
+// through the FP32 FFT/Green/C2R path, so the FP32 rounding scales with the
+// per-iteration update rather than with |sigma|.  Newton, E, the reductions
+// and the antisymmetric accumulator stay FP64.  When the mixed loop has
+// converged, the strain change of the increment is projected in FP64 onto
+// compatible fields (removing the rounding that Gamma can never correct),
+// then FP64 sweeps run until the FP64 residual converges again (at least
+// one sweep).
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -117,6 +127,8 @@ NewtonParams newton_params(const evpgpu_ctx *c)
             if (!(n == floor(n) && n >= 1 && n <= 64 && n == ph.nrs[0])) integral = 0;
         }
         P.int_n[p] = integral ? (int)ph.nrs[0] : 0;
+        memcpy(P.bn[p], ph.bn, sizeof(P.bn[p]));
+        memcpy(P.bb[p], ph.bb, sizeof(P.bb[p]));
     }
     P.crss_sym = c->crss_sym;
     return P;
@@ -180,15 +192,28 @@ void tcollect(evpgpu_ctx *c, const int *which, int n)
     }
 }
 
-// one Green iteration in precision prec; leaves errors and macros updated
-int green_iteration(evpgpu_ctx *c, int prec, double *newton_iters, double *ls_trials)
+// one Green iteration; leaves errors and macros updated.  prec FP64 runs the
+// FFT/Green/C2R path in FP64 on sigma; MIXED runs it in FP32 on
+// delta = sigma - sigma_A (anchor refreshed first when refresh != 0).
+int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, double *ls_trials)
 {
+    const double scale = 1.0 / (double)c->N;
+    int st = EVPGPU_OK;
+    if (prec == EVPGPU_MIXED && refresh) {
+        tmark(c, 1, 0);
+        st = to_f32_launch(c, c->sig, c->sigA, 6 * c->N);
+        if (!st) st = fft_forward_anchor(c, c->sigA);
+        if (!st) st = green_anchor(c);
+        if (!st) st = fft_inverse_anchor(c, c->cA, scale);
+        if (st) return st;
+        tmark(c, 1, 1);
+    }
     tmark(c, 1, 0);
-    int st = fft_forward(c, c->sig, 6, prec);
+    st = prec == EVPGPU_MIXED ? fft_forward_delta(c, c->sig, c->sigA) : fft_forward(c, c->sig, 6, prec);
     if (st) return st;
     tmark(c, 1, 1);
     tmark(c, 2, 0);
-    st = green_apply(c, prec, 1);
+    st = prec == EVPGPU_MIXED ? green_delta(c) : green_apply(c, prec, 1);
     if (st) return st;
     tmark(c, 2, 1);
     double add[6];
@@ -199,13 +224,14 @@ int green_iteration(evpgpu_ctx *c, int prec, double *newton_iters, double *ls_tr
     add[5] = (d[1] + d[3]) / 2.0;
     for (int i = 0; i < 9; i++) c->dD_applied[i] += d[i];
     tmark(c, 3, 0);
-    st = fft_inverse_update(c, 6, prec, c->E, add, 1.0 / (double)c->N);
+    st = prec == EVPGPU_MIXED ? fft_inverse_delta(c, c->E, add, c->cA, scale)
+                              : fft_inverse_update(c, 6, prec, c->E, add, scale);
     if (st) return st;
     tmark(c, 3, 1);
     NewtonParams P = newton_params(c);
     int nblk = 0;
     tmark(c, 0, 0);
-    st = newton_launch(c, P, c->sig, c->E, c->epsp, c->edot, c->schm, c->crss, c->gamdot, c->phase,
+    st = newton_launch(c, P, c->sig, c->E, c->epsp, c->edot, c->tran, c->crss, c->gamdot, c->phase,
                        c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr, nullptr);
     if (st) return st;
     tmark(c, 0, 1);
@@ -219,9 +245,8 @@ int green_iteration(evpgpu_ctx *c, int prec, double *newton_iters, double *ls_tr
         tcollect(c, w, 4);
     }
     if (*c->h_status) return *c->h_status;
-    const double wgt = 1.0 / (double)c->N;
-    c->m.err_e = c->h_red[1] * wgt;
-    c->m.err_s = c->h_red[0] * wgt;
+    c->m.err_e = c->h_red[1] * scale;
+    c->m.err_s = c->h_red[0] * scale;
     *newton_iters += c->h_red[2];
     *ls_trials += c->h_red[15];
     smacro(c, c->h_red);
@@ -229,6 +254,16 @@ int green_iteration(evpgpu_ctx *c, int prec, double *newton_iters, double *ls_tr
     else c->m.err_e /= c->m.e_vm;
     c->m.err_s /= c->m.s_vm;
     return EVPGPU_OK;
+}
+
+// E = Estart + P_compatible(E - Estart) in FP64 (uniform part unchanged)
+int project_increment(evpgpu_ctx *c)
+{
+    int st = sub_launch(c, c->E, c->Estart, 6 * c->N);
+    if (!st) st = fft_forward(c, c->E, 6, EVPGPU_FP64);
+    if (!st) st = green_project(c);
+    if (!st) st = fft_inverse_set(c, c->E, c->Estart, 1.0 / (double)c->N);
+    return st;
 }
 
 int ready(const evpgpu_ctx *c)
@@ -286,15 +321,20 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
     if (!st) st = dalloc(&c->crss, 24 * N);
     if (!st) st = dalloc(&c->gamacum, N);
     if (!st) st = dalloc(&c->gamdot, 12 * N);
-    if (!st) st = dalloc(&c->schm, 60 * N);
     if (!st) st = dalloc(&c->phase, N);
     if (!st) st = dalloc(&c->grain, N);
     if (!st) st = dalloc(&c->orient, N);
     if (!st) st = dalloc((double2 **)&c->spec, 6 * c->NK);
     if (!st) st = dalloc(&c->acc, 3 * c->NK);
     if (!st) st = dalloc(&c->rwork, 3 * N);
+    if (!st) st = dalloc(&c->sigA, 6 * N);
+    if (!st) st = dalloc(&c->cA, 6 * N);
+    if (!st) st = dalloc(&c->skA, 3 * c->NK);
+    if (!st) st = dalloc(&c->Estart, 6 * N);
+    c->anchor_every = 16;
     if (!st) st = dalloc(&c->partials, nblk * NRED);
     if (!st) st = dalloc(&c->red, NRED);
+    if (!st) st = dalloc(&c->red2, (long long)REDUCE_G * NRED);
     if (!st) st = dalloc(&c->d_status, 1);
     if (!st && hipHostMalloc((void **)&c->h_red, NRED * sizeof(double), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
     if (!st && hipHostMalloc((void **)&c->h_status, sizeof(int), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
@@ -302,7 +342,6 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
     if (!st) {
         // fields start at zero (gamdot of gas voxels is never written)
         st = fill_zero(c, c->gamdot, 12 * N);
-        if (!st) st = fill_zero(c, c->schm, 60 * N);
         if (!st && hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
         if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
     }
@@ -318,8 +357,8 @@ int evpgpu_destroy(evpgpu_ctx *c)
 {
     if (!c) return EVPGPU_OK;
     void *ptrs[] = {c->sig, c->E, c->W, c->epsp, c->edot, c->tran, c->crss, c->gamacum, c->gamdot,
-                    c->schm, c->phase, c->grain, c->orient, c->sgr, c->spec, c->acc, c->rwork,
-                    c->partials, c->red, c->d_status, c->tw64[0], c->tw64[1], c->tw64[2],
+                    c->phase, c->grain, c->orient, c->sgr, c->spec, c->acc, c->rwork,
+                    c->sigA, c->cA, c->skA, c->Estart, c->partials, c->red, c->red2, c->d_status, c->tw64[0], c->tw64[1], c->tw64[2],
                     c->tw32[0], c->tw32[1], c->tw32[2]};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
         if (ptrs[i]) (void)hipFree(ptrs[i]);
@@ -358,6 +397,13 @@ int evpgpu_get_timing(evpgpu_ctx *c, int which, double *ms, long long *launches)
     if (!c || which < 0 || which > 4) return EVPGPU_ERR_ARG;
     if (ms) *ms = c->t_ms[which];
     if (launches) *launches = c->t_n[which];
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_anchor_interval(evpgpu_ctx *c, int iterations)
+{
+    if (!c || iterations < 1) return EVPGPU_ERR_ARG;
+    c->anchor_every = iterations;
     return EVPGPU_OK;
 }
 
@@ -469,8 +515,8 @@ int evpgpu_upload_state(evpgpu_ctx *c, const double *sig6, const double *dg9, co
 #undef H2D
     free(E);
     if (st) return st;
-    // Schmid tensors from the current orientations (initialised at setup, Q12)
-    st = schmid_launch(c);
+    // (the Newton kernel derives the Schmid tensors from the current
+    // orientations, so they are initialised at setup as well, Q12)
     if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
     if (!st) c->have_state = 1;
     return st;
@@ -550,10 +596,8 @@ int evpgpu_increment(evpgpu_ctx *c, int forced, evpgpu_stats *stats)
     tmark(c, 4, 0);
     int st = prologue_launch(c, pro, c->creep_flag == 1, nullptr);
     if (st) return st;
-    if (m.istep == 1 || c->ld.update_flag == 1) {
-        st = schmid_launch(c);
-        if (st) return st;
-    }
+    // update_schmid (src/evolution.c:1334-1336) happens inside the Newton
+    // kernel, from the orientations the previous epilogue left
     st = fill_zero(c, (double *)c->acc, 6 * c->NK);
     if (st) return st;
     tmark(c, 4, 1);
@@ -574,35 +618,51 @@ int evpgpu_increment(evpgpu_ctx *c, int forced, evpgpu_stats *stats)
     auto more = [&](void) {
         return (iter < c->ld.iter_max) && (fmax(fabs(m.err_s), fabs(m.err_e)) > tol);
     };
+    const bool mixed = c->precision == EVPGPU_MIXED;
+    if (mixed) EVP_CHECK(hipMemcpyAsync(c->Estart, c->E, 6 * c->N * sizeof(double),
+                                        hipMemcpyDeviceToDevice, c->stream));
+    const int ae = c->anchor_every > 0 ? c->anchor_every : 1;
+    int imix = 0;
+    auto mixed_step = [&](void) {
+        iter++;
+        int r = green_iteration(c, EVPGPU_MIXED, imix % ae == 0, &nit, &ntr);
+        imix++;
+        return r;
+    };
+    auto fp64_step = [&](void) {
+        iter++;
+        it64++;
+        return green_iteration(c, EVPGPU_FP64, 0, &nit, &ntr);
+    };
     if (forced > 0) {
-        while (iter < forced) {
-            int prec = (c->precision == EVPGPU_MIXED && iter < forced - 1) ? EVPGPU_MIXED : EVPGPU_FP64;
-            iter++;
-            if (prec == EVPGPU_FP64) it64++;
-            st = green_iteration(c, prec, &nit, &ntr);
+        if (mixed) {
+            while (iter < forced - 1) {
+                st = mixed_step();
+                if (st) return st;
+            }
+            st = project_increment(c);
             if (st) return st;
         }
+        while (iter < forced) {
+            st = fp64_step();
+            if (st) return st;
+        }
+    } else if (mixed) {
+        while (more()) {
+            st = mixed_step();
+            if (st) return st;
+        }
+        st = project_increment(c);
+        if (st) return st;
+        // FP64 correction sweeps: at least one, until converged
+        do {
+            st = fp64_step();
+            if (st) return st;
+        } while (more());
     } else {
-        if (c->precision == EVPGPU_MIXED) {
-            while (more()) {
-                iter++;
-                st = green_iteration(c, EVPGPU_MIXED, &nit, &ntr);
-                if (st) return st;
-            }
-            // FP64 residual-correction sweeps: at least one, until converged
-            do {
-                iter++;
-                it64++;
-                st = green_iteration(c, EVPGPU_FP64, &nit, &ntr);
-                if (st) return st;
-            } while (more());
-        } else {
-            while (more()) {
-                iter++;
-                it64++;
-                st = green_iteration(c, EVPGPU_FP64, &nit, &ntr);
-                if (st) return st;
-            }
+        while (more()) {
+            st = fp64_step();
+            if (st) return st;
         }
     }
     S.seconds = now_s() - t0;
@@ -736,10 +796,10 @@ int evpgpu_green_host(evpgpu_ctx *c, const double *in6, double *out9)
 
 int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient, const double *sgr,
                        int n_orient, double *sig6, const double *strain6, const double *eps6,
-                       const double *schm60, const double *crss24, double *edot6, double *gamdot12,
+                       const double *tranmat9, const double *crss24, double *edot6, double *gamdot12,
                        double *err2, int *niter)
 {
-    if (!c || n < 1 || !phase || !orient || !sgr || !sig6 || !strain6 || !eps6 || !schm60 || !crss24 ||
+    if (!c || n < 1 || !phase || !orient || !sgr || !sig6 || !strain6 || !eps6 || !tranmat9 || !crss24 ||
         !edot6 || !gamdot12 || !err2 || !niter || !c->have_phases || !c->have_loading)
         return EVPGPU_ERR_ARG;
     for (int v = 0; v < n; v++)
@@ -756,7 +816,7 @@ int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient
     if (!st) st = dalloc(&dsig, 6LL * n);
     if (!st) st = dalloc(&dstr, 6LL * n);
     if (!st) st = dalloc(&deps, 6LL * n);
-    if (!st) st = dalloc(&dsc, 60LL * n);
+    if (!st) st = dalloc(&dsc, 9LL * n);
     if (!st) st = dalloc(&dcr, 24LL * n);
     if (!st) st = dalloc(&ded, 6LL * n);
     if (!st) st = dalloc(&dgd, 12LL * n);
@@ -770,7 +830,7 @@ int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient
     UP(dsig, sig6, 6LL * n, double);
     UP(dstr, strain6, 6LL * n, double);
     UP(deps, eps6, 6LL * n, double);
-    UP(dsc, schm60, 60LL * n, double);
+    UP(dsc, tranmat9, 9LL * n, double);
     UP(dcr, crss24, 24LL * n, double);
     UP(dgd, gamdot12, 12LL * n, double);
 #undef UP

@@ -31,6 +31,7 @@ struct NewtonParams {
     int is_gas[2];
     int int_n[2];    // integer rate exponent of the phase, 0 if not integral
     int crss_sym;    // 1: forward and backward CRSS are equal in every voxel
+    double bn[2][EVPGPU_NSYSMX][3], bb[2][EVPGPU_NSYSMX][3]; // crystal-axis normals, directions
 };
 
 struct GreenParams {
@@ -64,6 +65,8 @@ struct EpilogueParams {
 #define NRED 16
 // epilogue partials: sum eps*wgt (6), sum edot*wgt (6)
 #define NRED_EPI 12
+// first-stage workgroups of the partial-sum reduction
+#define REDUCE_G 512
 
 struct evpgpu_ctx {
     int device;
@@ -88,7 +91,6 @@ struct evpgpu_ctx {
     double *crss;     // 24N (s*2+d)
     double *gamacum;  // N
     double *gamdot;   // 12N
-    double *schm;     // 60N (s*5+j), sample axes, refreshed per increment
     int *phase, *grain, *orient;
     double *sgr;      // n_orient x 36
     int n_orient;
@@ -97,12 +99,20 @@ struct evpgpu_ctx {
     void *spec;       // 6*NK complex (double2 or float2)
     double2 *acc;     // 3*NK complex: antisymmetric e-hat summed over the increment
     double *rwork;    // 3N real scratch
+    // mixed-precision loop: FP32 anchor stress, its FP32 real-space Green
+    // correction and antisymmetric spectrum; FP64 strain at the loop start
+    float *sigA;      // 6N
+    float *cA;        // 6N
+    float2 *skA;      // 3*NK
+    double *Estart;   // 6N
+    int anchor_every; // iterations between FP64 anchor transforms
     double2 *tw64[3];
     float2 *tw32[3];
 
     // reductions
     double *partials; // max(nblk) * NRED
     double *red;      // NRED (device)
+    double *red2;     // REDUCE_G * NRED, first stage of the reduction
     double *h_red;    // NRED pinned host
     int *d_status;
     int *h_status;
@@ -128,19 +138,28 @@ int fft_inverse_update(evpgpu_ctx *c, int ncomp, int precision, double *target,
                        const double *add_const, double scale);
 int fft_inverse_real(evpgpu_ctx *c, const double2 *cin, int ncomp, double *out, double scale);
 int fft_twiddles(evpgpu_ctx *c);
+int fft_forward_delta(evpgpu_ctx *c, const double *sig, const float *anchor);
+int fft_forward_anchor(evpgpu_ctx *c, const float *anchor);
+int fft_inverse_delta(evpgpu_ctx *c, double *E, const double *add, const float *cA, double scale);
+int fft_inverse_anchor(evpgpu_ctx *c, float *cA, double scale);
+int fft_inverse_set(evpgpu_ctx *c, double *target, const double *base, double scale);
 // green.hip
 int green_apply(evpgpu_ctx *c, int precision, int accumulate);
 int green_full9(evpgpu_ctx *c, const double2 *in6, double2 *out9);
+int green_anchor(evpgpu_ctx *c);
+int green_delta(evpgpu_ctx *c);
+int green_project(evpgpu_ctx *c);
 // newton.hip
 int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const double *E,
-                  const double *epsp, double *edot, const double *schm, const double *crss,
+                  const double *epsp, double *edot, const double *tran, const double *crss,
                   double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status,
                   double *err_vox, int *niter_vox);
 int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out);
 // fields.hip
 int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *macro_sym6);
-int schmid_launch(evpgpu_ctx *c);
 int antisym_add_launch(evpgpu_ctx *c, const double *dW3);
 int epilogue_launch(evpgpu_ctx *c, const EpilogueParams &P, double *partials, int *nblk);
 int fill_zero(evpgpu_ctx *c, double *p, long long n);
+int to_f32_launch(evpgpu_ctx *c, const double *src, float *dst, long long n);
+int sub_launch(evpgpu_ctx *c, double *a, const double *b, long long n);

@@ -179,21 +179,27 @@ __global__ __launch_bounds__(NT) void k_fft_strided(typename CT<R>::T *data, int
     }
 }
 
-// ---- z pass, real -> half spectrum.  in: ncomp x N real (FP64), out: ncomp x NK.
-template <typename R, int L>
-__global__ __launch_bounds__(NT) void k_fft_z_r2c(const double *in, typename CT<R>::T *out,
-                                                  long long N, long long NK,
-                                                  const typename CT<R>::T *tw)
+// ---- z pass, real -> half spectrum (ncomp x N real -> ncomp x NK complex).
+// IN 0: FP64 field; IN 1: FP64 field minus an FP32 anchor field (the mixed
+// loop's delta); IN 2: an FP32 field.
+template <typename R, int L, int IN>
+__global__ __launch_bounds__(NT) void k_fft_z_r2c(const double *in, const float *inA,
+                                                  typename CT<R>::T *out, long long N,
+                                                  long long NK, const typename CT<R>::T *tw)
 {
     typedef typename CT<R>::T C;
     constexpr int P = L + 1, LH = L / 2 + 1;
     __shared__ C lds[TB * P];
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TB;
-    const double *src = in + comp * N + line0 * L;
+    const long long off = comp * N + line0 * L;
     for (int e = threadIdx.x; e < L * TB; e += NT) {
         int t = e / L, i = e % L;
-        lds[t * P + i] = C{(R)src[e], (R)0};
+        R x;
+        if (IN == 0) x = (R)in[off + e];
+        else if (IN == 1) x = (R)(in[off + e] - (double)inA[off + e]);
+        else x = (R)inA[off + e];
+        lds[t * P + i] = C{x, (R)0};
     }
     __syncthreads();
     stages<false, L, 0, 1>(lds, tw);
@@ -205,10 +211,15 @@ __global__ __launch_bounds__(NT) void k_fft_z_r2c(const double *in, typename CT<
 }
 
 // ---- z pass, half spectrum -> real, Re(IFFT) of the Hermitian extension.
-// MODE 0: target[c] += addc[c] + re*scale  (the strain update of
-//         src/evolution.c:1400-1401);  MODE 1: target[c] = re*scale.
+// MODE 0: target[c] += add[c] + re*scale     (the strain update of
+//         src/evolution.c:1400-1401)
+// MODE 1: target[c]  = re*scale
+// MODE 2: target[c] += add[c] + fa[c] + re*scale   (mixed loop: anchor + delta)
+// MODE 3: fo[c]      = (float)(re*scale)           (mixed loop: anchor update)
+// MODE 4: target[c]  = base[c] + re*scale          (projection of a strain change)
 template <typename R, int L, int MODE>
 __global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, double *target,
+                                                  const float *fa, float *fo, const double *base,
                                                   long long N, long long NK, double a0, double a1,
                                                   double a2, double a3, double a4, double a5,
                                                   double scale, const typename CT<R>::T *tw)
@@ -228,12 +239,15 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, d
     __syncthreads();
     stages<true, L, 0, 1>(lds, tw);
     double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
-    double *dst = target + comp * N + line0 * L;
+    const long long off = comp * N + line0 * L;
     for (int e = threadIdx.x; e < L * TB; e += NT) {
         int t = e / L, i = e % L;
         double re = (double)lds[t * P + i].x;
-        if (MODE == 0) dst[e] += add + re * scale;
-        else dst[e] = re * scale;
+        if (MODE == 0) target[off + e] += add + re * scale;
+        else if (MODE == 1) target[off + e] = re * scale;
+        else if (MODE == 2) target[off + e] += add + (double)fa[off + e] + re * scale;
+        else if (MODE == 3) fo[off + e] = (float)(re * scale);
+        else target[off + e] = base[off + e] + re * scale;
     }
 }
 
@@ -277,12 +291,12 @@ int pass_x(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 
-template <typename R>
-int fwd(evpgpu_ctx *c, const double *real_in, int ncomp, typename CT<R>::T *spec)
+template <typename R, int IN>
+int fwd(evpgpu_ctx *c, const double *real_in, const float *inA, int ncomp, typename CT<R>::T *spec)
 {
     dim3 g((unsigned)(c->nx * c->ny / TB), ncomp);
-#define CALLZ(L) hipLaunchKernelGGL((k_fft_z_r2c<R, L>), g, dim3(NT), 0, c->stream, real_in, spec, \
-                                    c->N, c->NK, twp<R>(c, 2))
+#define CALLZ(L) hipLaunchKernelGGL((k_fft_z_r2c<R, L, IN>), g, dim3(NT), 0, c->stream, real_in, inA, \
+                                    spec, c->N, c->NK, twp<R>(c, 2))
     FFT_DISPATCH(c->nz, CALLZ);
 #undef CALLZ
     int st = pass_y<R, false>(c, spec, ncomp);
@@ -292,7 +306,7 @@ int fwd(evpgpu_ctx *c, const double *real_in, int ncomp, typename CT<R>::T *spec
 
 template <typename R, int MODE>
 int inv(evpgpu_ctx *c, typename CT<R>::T *spec, int ncomp, double *target, const double *addc,
-        double scale)
+        double scale, const float *fa = nullptr, float *fo = nullptr, const double *base = nullptr)
 {
     int st = pass_x<R, true>(c, spec, ncomp);
     if (st) return st;
@@ -303,8 +317,8 @@ int inv(evpgpu_ctx *c, typename CT<R>::T *spec, int ncomp, double *target, const
         for (int i = 0; i < ncomp && i < 6; i++) a[i] = addc[i];
     dim3 g((unsigned)(c->nx * c->ny / TB), ncomp);
 #define CALLZI(L) hipLaunchKernelGGL((k_fft_z_c2r<R, L, MODE>), g, dim3(NT), 0, c->stream, spec, target, \
-                                     c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5], scale,        \
-                                     twp<R>(c, 2))
+                                     fa, fo, base, c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5],   \
+                                     scale, twp<R>(c, 2))
     FFT_DISPATCH(c->nz, CALLZI);
 #undef CALLZI
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
@@ -342,8 +356,38 @@ int fft_twiddles(evpgpu_ctx *c)
 // forward transform of ncomp real FP64 fields into c->spec (precision chosen)
 int fft_forward(evpgpu_ctx *c, const double *real_in, int ncomp, int precision)
 {
-    if (precision == EVPGPU_MIXED) return fwd<float>(c, real_in, ncomp, (float2 *)c->spec);
-    return fwd<double>(c, real_in, ncomp, (double2 *)c->spec);
+    if (precision == EVPGPU_MIXED) return fwd<float, 0>(c, real_in, nullptr, ncomp, (float2 *)c->spec);
+    return fwd<double, 0>(c, real_in, nullptr, ncomp, (double2 *)c->spec);
+}
+
+// mixed loop: FP32 transform of (sig - anchor), anchor stored in FP32
+int fft_forward_delta(evpgpu_ctx *c, const double *sig, const float *anchor)
+{
+    return fwd<float, 1>(c, sig, anchor, 6, (float2 *)c->spec);
+}
+
+// mixed loop: FP64 transform of the FP32 anchor field
+int fft_forward_anchor(evpgpu_ctx *c, const float *anchor)
+{
+    return fwd<double, 2>(c, nullptr, anchor, 6, (double2 *)c->spec);
+}
+
+// mixed loop: E += add + cA + re/N from the FP32 delta spectrum
+int fft_inverse_delta(evpgpu_ctx *c, double *E, const double *add, const float *cA, double scale)
+{
+    return inv<float, 2>(c, (float2 *)c->spec, 6, E, add, scale, cA, nullptr, nullptr);
+}
+
+// mixed loop: cA = re/N of the FP64 anchor spectrum
+int fft_inverse_anchor(evpgpu_ctx *c, float *cA, double scale)
+{
+    return inv<double, 3>(c, (double2 *)c->spec, 6, nullptr, nullptr, scale, nullptr, cA, nullptr);
+}
+
+// target = base + re*scale of the FP64 spectrum (6 comps)
+int fft_inverse_set(evpgpu_ctx *c, double *target, const double *base, double scale)
+{
+    return inv<double, 4>(c, (double2 *)c->spec, 6, target, nullptr, scale, nullptr, nullptr, base);
 }
 
 // inverse of c->spec (ncomp comps) accumulated into target: t += add[c] + re*scale

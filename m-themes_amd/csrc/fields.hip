@@ -1,7 +1,5 @@
 // fields.hip -- per-increment voxel kernels around the Green loop:
 //   k_prologue  : DisGrad += Udot*dt (src/evolution.c:1318-1332)
-//   k_schmid    : update_schmid, P_s = R P_xt R^T in the 5-D basis
-//                 (src/kinematics.c:430-463), once per increment
 //   k_epilogue  : Eps += Edot*dt, the Eps/Edot averages, update_orient and the
 //                 Voce hardening, fused in one pass over the voxels
 //                 (src/evolution.c:1469-1547, src/constitutive.c:142-326)
@@ -22,6 +20,18 @@ __global__ void k_fill(double *p, long long n)
     if (i < n) p[i] = 0.0;
 }
 
+__global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+__global__ void k_sub(double *__restrict__ a, const double *__restrict__ b, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] -= b[i];
+}
+
 // E (6 comps) += dE (sym of Udot*dt), W (3 comps) += dW; creep: E = m6, W = w3
 __global__ void k_prologue(double *E, double *W, long long N, double d0, double d1, double d2,
                            double d3, double d4, double d5, double w0, double w1, double w2,
@@ -34,48 +44,6 @@ __global__ void k_prologue(double *E, double *W, long long N, double d0, double 
     for (int c = 0; c < 6; c++) E[c * N + v] = creep ? d[c] : E[c * N + v] + d[c];
 #pragma unroll
     for (int c = 0; c < 3; c++) W[c * N + v] = creep ? w[c] : W[c * N + v] + w[c];
-}
-
-struct SchmidParams {
-    double a33[2][EVPGPU_NSYSMX][9]; // Schmid tensors of the phases, crystal axes, 3x3
-    int nsys[2];
-    int is_gas[2];
-};
-
-__global__ __launch_bounds__(256) void k_schmid(SchmidParams P, long long N,
-                                                const int *__restrict__ phase,
-                                                const double *__restrict__ tran,
-                                                double *__restrict__ schm)
-{
-    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
-    int p = phase[v] - 1;
-    p = p < 0 ? 0 : (p > 1 ? 1 : p);
-    if (P.is_gas[p]) return;
-    double T[3][3];
-#pragma unroll
-    for (int i = 0; i < 9; i++) T[i / 3][i % 3] = tran[i * N + v];
-    const double r2 = RSQ2, r6 = RSQ6;
-    for (int s = 0; s < P.nsys[p]; s++) {
-        const double *a = P.a33[p][s];
-        double r[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int pp = 0; pp < 3; pp++)
-#pragma unroll
-                    for (int q = 0; q < 3; q++) acc += T[i][pp] * T[j][q] * a[pp * 3 + q];
-                r[i][j] = acc;
-            }
-        schm[(s * 5 + 0) * N + v] = r[0][0] * (-1. * r2) + r[1][1] * r2;
-        schm[(s * 5 + 1) * N + v] = r[0][0] * (-1. * r6) + r[1][1] * (-1. * r6) + r[2][2] * (2. * r6);
-        schm[(s * 5 + 2) * N + v] = r[1][2] * r2 + r[2][1] * r2;
-        schm[(s * 5 + 3) * N + v] = r[0][2] * r2 + r[2][0] * r2;
-        schm[(s * 5 + 4) * N + v] = r[0][1] * r2 + r[1][0] * r2;
-    }
 }
 
 // dW (3 comps, fluctuation part from the inverse FFT, in rw) + uniform part m
@@ -252,6 +220,18 @@ int fill_zero(evpgpu_ctx *c, double *p, long long n)
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 
+int to_f32_launch(evpgpu_ctx *c, const double *src, float *dst, long long n)
+{
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, src, dst, n);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+int sub_launch(evpgpu_ctx *c, double *a, const double *b, long long n)
+{
+    hipLaunchKernelGGL(k_sub, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a, b, n);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
 int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *macro_sym6)
 {
     (void)macro_sym6;
@@ -259,32 +239,6 @@ int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *m
     hipLaunchKernelGGL(k_prologue, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, c->E,
                        c->W, c->N, dE6[0], dE6[1], dE6[2], dE6[3], dE6[4], dE6[5], w[0], w[1], w[2],
                        creep);
-    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
-}
-
-int schmid_launch(evpgpu_ctx *c)
-{
-    SchmidParams P;
-    memset(&P, 0, sizeof(P));
-    const double r2 = RSQ2, r3 = RSQ3, r6 = RSQ6;
-    (void)r3;
-    for (int p = 0; p < 2; p++) {
-        P.nsys[p] = p < c->nphases ? c->ph[p].nsys : 0;
-        P.is_gas[p] = p < c->nphases ? c->ph[p].is_gas : 1;
-        for (int s = 0; s < P.nsys[p]; s++) {
-            const double *v = c->ph[p].schm_xt[s];
-            double *a = P.a33[p][s];
-            // chg_basis5 opt 1 (V5 -> T2), nonzero terms in order
-            a[0] = v[0] * (-1. * r2) + v[1] * (-1. * r6);
-            a[4] = v[0] * r2 + v[1] * (-1. * r6);
-            a[8] = v[1] * (2. * r6);
-            a[5] = a[7] = v[2] * r2;
-            a[2] = a[6] = v[3] * r2;
-            a[1] = a[3] = v[4] * r2;
-        }
-    }
-    hipLaunchKernelGGL(k_schmid, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, P, c->N,
-                       c->phase, c->tran, c->schm);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 

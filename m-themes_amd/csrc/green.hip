@@ -104,10 +104,12 @@ __device__ __forceinline__ void gamma_apply(const R *c0, const double nd[3], con
 
 // MODE 0: in-place symmetric output + antisymmetric accumulation (acc != null)
 // MODE 1: write all 9 components to out9 (tests)
+// MODE 2: mixed loop, delta spectrum: in place symmetric, acc += antisym + skA
+// MODE 3: mixed loop, anchor spectrum: in place symmetric, skA = antisym
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 *acc,
-                                               double2 *out9, GreenParams P, int nx, int ny,
-                                               int nz, int nzh, long long NK)
+                                               double2 *out9, float2 *skA, GreenParams P, int nx,
+                                               int ny, int nz, int nzh, long long NK)
 {
     typedef typename CT<R>::T C;
     __shared__ R c0s[81];
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 
             gamma_apply<R>(c0s, n2, s, e, (R)0.5);
         }
     }
-    if (MODE == 0) {
+    if (MODE != 1) {
         const int ia[6] = {0, 1, 2, 1, 0, 0}, ib[6] = {0, 1, 2, 2, 2, 1};
 #pragma unroll
         for (int c = 0; c < 6; c++) {
@@ -170,13 +172,22 @@ __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 
             v.y = (e[ia[c]][ib[c]].y + e[ib[c]][ia[c]].y) * (R)0.5;
             spec[c * NK + id] = v;
         }
-        if (acc) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                int a = ia[c + 3], b = ib[c + 3];
+        for (int c = 0; c < 3; c++) {
+            int a = ia[c + 3], b = ib[c + 3];
+            double wx = (double)((e[a][b].x - e[b][a].x) * (R)0.5);
+            double wy = (double)((e[a][b].y - e[b][a].y) * (R)0.5);
+            if (MODE == 3) {
+                skA[c * NK + id] = make_float2((float)wx, (float)wy);
+            } else if (acc) {
                 double2 w = acc[c * NK + id];
-                w.x += (double)((e[a][b].x - e[b][a].x) * (R)0.5);
-                w.y += (double)((e[a][b].y - e[b][a].y) * (R)0.5);
+                if (MODE == 2) {
+                    float2 k = skA[c * NK + id];
+                    wx += (double)k.x;
+                    wy += (double)k.y;
+                }
+                w.x += wx;
+                w.y += wy;
                 acc[c * NK + id] = w;
             }
         }
@@ -189,7 +200,88 @@ __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 
     }
 }
 
+// Orthogonal projection of a symmetric strain spectrum onto compatible
+// fields sym(a x n): a = 2 e.n - (n.e.n) n (exact for e = sym(a x n)).  The
+// k = 0 mode (macroscopic strain) and the Nyquist wave vectors where
+// Gamma_eff mixes two directions are left unchanged.
+__global__ __launch_bounds__(256) void k_project(double2 *spec, int nx, int ny, int nz, int nzh,
+                                                 long long NK)
+{
+    long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= NK) return;
+    int kz = (int)(id % nzh);
+    int y = (int)((id / nzh) % ny);
+    int x = (int)(id / ((long long)nzh * ny));
+    if (x == 0 && y == 0 && kz == 0) return;
+    double n[3], n2[3];
+    unit_k(x, y, kz, nx, ny, nz, n);
+    if ((2 * x == nx) || (2 * y == ny) || (2 * kz == nz)) {
+        unit_k((nx - x) % nx, (ny - y) % ny, (nz - kz) % nz, nx, ny, nz, n2);
+        bool same = (n2[0] == n[0] && n2[1] == n[1] && n2[2] == n[2]) ||
+                    (n2[0] == -n[0] && n2[1] == -n[1] && n2[2] == -n[2]);
+        if (!same) return;
+    }
+    double2 v[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) v[c] = spec[c * NK + id];
+    double2 e[3][3] = {{v[0], v[5], v[4]}, {v[5], v[1], v[3]}, {v[4], v[3], v[2]}};
+    double2 en[3], a[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        en[i].x = e[i][0].x * n[0] + e[i][1].x * n[1] + e[i][2].x * n[2];
+        en[i].y = e[i][0].y * n[0] + e[i][1].y * n[1] + e[i][2].y * n[2];
+    }
+    double2 nen;
+    nen.x = en[0].x * n[0] + en[1].x * n[1] + en[2].x * n[2];
+    nen.y = en[0].y * n[0] + en[1].y * n[1] + en[2].y * n[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        a[i].x = 2.0 * en[i].x - nen.x * n[i];
+        a[i].y = 2.0 * en[i].y - nen.y * n[i];
+    }
+    const int ia[6] = {0, 1, 2, 1, 0, 0}, ib[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        int i = ia[c], j = ib[c];
+        spec[c * NK + id] = make_double2(0.5 * (a[i].x * n[j] + a[j].x * n[i]),
+                                         0.5 * (a[i].y * n[j] + a[j].y * n[i]));
+    }
+}
+
 } // namespace
+
+int green_project(evpgpu_ctx *c)
+{
+    unsigned nb = (unsigned)((c->NK + 255) / 256);
+    hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, c->stream, (double2 *)c->spec, c->nx, c->ny,
+                       c->nz, c->nzh, c->NK);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+// mixed loop: MODE 3 on the FP64 anchor spectrum, MODE 2 on the FP32 delta
+int green_anchor(evpgpu_ctx *c)
+{
+    GreenParams P;
+    memcpy(P.c0, c->ld.c0, sizeof(P.c0));
+    memcpy(P.s0, c->ld.s0, sizeof(P.s0));
+    P.elast_bc = c->ld.elast_bc;
+    unsigned nb = (unsigned)((c->NK + 255) / 256);
+    hipLaunchKernelGGL((k_green<double, 3>), dim3(nb), dim3(256), 0, c->stream, (double2 *)c->spec, nullptr,
+                       nullptr, c->skA, P, c->nx, c->ny, c->nz, c->nzh, c->NK);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+int green_delta(evpgpu_ctx *c)
+{
+    GreenParams P;
+    memcpy(P.c0, c->ld.c0, sizeof(P.c0));
+    memcpy(P.s0, c->ld.s0, sizeof(P.s0));
+    P.elast_bc = c->ld.elast_bc;
+    unsigned nb = (unsigned)((c->NK + 255) / 256);
+    hipLaunchKernelGGL((k_green<float, 2>), dim3(nb), dim3(256), 0, c->stream, (float2 *)c->spec, c->acc,
+                       nullptr, c->skA, P, c->nx, c->ny, c->nz, c->nzh, c->NK);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
 
 int green_apply(evpgpu_ctx *c, int precision, int accumulate)
 {
@@ -200,10 +292,12 @@ int green_apply(evpgpu_ctx *c, int precision, int accumulate)
     unsigned nb = (unsigned)((c->NK + 255) / 256);
     if (precision == EVPGPU_MIXED)
         hipLaunchKernelGGL((k_green<float, 0>), dim3(nb), dim3(256), 0, c->stream, (float2 *)c->spec,
-                           accumulate ? c->acc : nullptr, nullptr, P, c->nx, c->ny, c->nz, c->nzh, c->NK);
+                           accumulate ? c->acc : nullptr, nullptr, nullptr, P, c->nx, c->ny, c->nz, c->nzh,
+                           c->NK);
     else
         hipLaunchKernelGGL((k_green<double, 0>), dim3(nb), dim3(256), 0, c->stream, (double2 *)c->spec,
-                           accumulate ? c->acc : nullptr, nullptr, P, c->nx, c->ny, c->nz, c->nzh, c->NK);
+                           accumulate ? c->acc : nullptr, nullptr, nullptr, P, c->nx, c->ny, c->nz, c->nzh,
+                           c->NK);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 
@@ -215,6 +309,6 @@ int green_full9(evpgpu_ctx *c, const double2 *in6, double2 *out9)
     P.elast_bc = c->ld.elast_bc;
     unsigned nb = (unsigned)((c->NK + 255) / 256);
     hipLaunchKernelGGL((k_green<double, 1>), dim3(nb), dim3(256), 0, c->stream, (double2 *)in6, nullptr,
-                       out9, P, c->nx, c->ny, c->nz, c->nzh, c->NK);
+                       out9, nullptr, P, c->nx, c->ny, c->nz, c->nzh, c->NK);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }

@@ -12,11 +12,16 @@
 //   * NR_quality never uses d(edot)/d(sigma), so it is not formed there;
 //   * zero rows/columns of d(edot)/d(sigma) (6th component) are skipped;
 //   * LU_inv_66(C_gr) is a per-orientation constant, read from a table.
-// Differences in rounding: |x|^n uses repeated squaring for integral n, and
-// the compiler fuses multiply-adds.
+// Differences in rounding: |x|^n uses repeated squaring for integral n, the
+// rate law multiplies by 1/CRSS instead of dividing, the Schmid tensors are
+// sym((Rn) x (Rb)) rather than R P R^T through the 3x3 form, the step is an LU
+// solve of the residual (see lu_solve6), and the compiler fuses multiply-adds.
 //
-// Registers: all per-voxel arrays are indexed with compile-time indices only
-// (row swaps of the pivoting are selects), so nothing goes to scratch.
+// Data placement: the 12x5 Schmid tensors of a voxel are derived at kernel
+// start from its rotation (72 B read instead of 480 B) and kept in a
+// wave-private LDS slab laid out [component][lane] (conflict-free), which
+// keeps them out of the register file; all other per-voxel arrays live in
+// registers with compile-time indices only (row swaps are selects).
 #include "evpgpu_impl.h"
 
 namespace {
@@ -56,61 +61,67 @@ __device__ __forceinline__ void v6_to_sym(const double v[6], double t[6])
     t[5] = v[4] * r2;
 }
 
-struct RateIn {
-    double sc[EVPGPU_NSYSMX][5];
-    double tf[EVPGPU_NSYSMX], tb[EVPGPU_NSYSMX];
-};
+constexpr int NB_THREADS = 64; // one wave per block: the LDS slab is wave-private
 
+struct RateIn {
+    const double *sc;                              // LDS slab, sc[(s*5+j)*64]
+    double itf[EVPGPU_NSYSMX], itb[EVPGPU_NSYSMX]; // 1/CRSS forward, backward
+};
+#define SC(i, j) in.sc[((i)*5 + (j)) * NB_THREADS]
+
+// StrainRate_eval (src/constitutive.c:5-67) as one pass over the slip
+// systems: each system's Schmid vector is read from LDS once and feeds the
+// resolved shear, the rate, edot and d(edot)/d(sigma).  The sums run over the
+// systems in ascending order, as the reference's separate loops do.
 template <bool DD, bool INTN>
 __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn &in,
                                       const double s[6], double ed[5], double dd[5][5],
                                       double gd[EVPGPU_NSYSMX])
 {
-    double r1[EVPGPU_NSYSMX], r2[EVPGPU_NSYSMX];
     const int ns = P.nsys[p];
     const int intn = P.int_n[p];
 #pragma unroll
-    for (int i = 0; i < EVPGPU_NSYSMX; i++) {
-        r1[i] = 0.0;
-        r2[i] = 0.0;
-        if (i < ns) {
-            double r = in.sc[i][0] * s[0] + in.sc[i][1] * s[1] + in.sc[i][2] * s[2] +
-                       in.sc[i][3] * s[3] + in.sc[i][4] * s[4];
-            double tau = (r < 0) ? in.tb[i] : in.tf[i];
-            r = r / tau;
+    for (int i = 0; i < 5; i++) {
+        ed[i] = 0.;
+        if (DD)
+#pragma unroll
+            for (int j = i; j < 5; j++) dd[i][j] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < EVPGPU_NSYSMX; k++) {
+        if (k < ns) {
+            double sc[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) sc[j] = SC(k, j);
+            double r = sc[0] * s[0] + sc[1] * s[1] + sc[2] * s[2] + sc[3] * s[3] + sc[4] * s[4];
+            double itau = (r < 0) ? in.itb[k] : in.itf[k];
+            r = r * itau;
             double a = fabs(r), p1, p2;
             if (INTN) {
                 p1 = ipow(a, intn - 1);
                 p2 = p1 * a;
             } else {
-                p1 = pow(a, P.nrs[p][i] - 1.);
-                p2 = pow(a, P.nrs[p][i]);
+                p1 = pow(a, P.nrs[p][k] - 1.);
+                p2 = pow(a, P.nrs[p][k]);
             }
-            r1[i] = P.gam0[p][i] * P.nrs[p][i] * p1 / tau;
-            r2[i] = P.gam0[p][i] * p2 * (double)(2 * (r > 0) - 1);
-            gd[i] = r2[i];
+            double r2 = P.gam0[p][k] * p2 * (double)(2 * (r > 0) - 1);
+            gd[k] = r2;
+#pragma unroll
+            for (int i = 0; i < 5; i++) ed[i] += sc[i] * r2;
+            if (DD) {
+                double r1 = P.gam0[p][k] * P.nrs[p][k] * p1 * itau;
+#pragma unroll
+                for (int i = 0; i < 5; i++)
+#pragma unroll
+                    for (int j = i; j < 5; j++) dd[i][j] += sc[i] * sc[j] * r1;
+            }
         }
-    }
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-        double acc = 0.;
-#pragma unroll
-        for (int j = 0; j < EVPGPU_NSYSMX; j++)
-            if (j < ns) acc += in.sc[j][i] * r2[j];
-        ed[i] = acc;
     }
     if (DD) {
 #pragma unroll
         for (int i = 0; i < 5; i++)
 #pragma unroll
-            for (int j = i; j < 5; j++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < EVPGPU_NSYSMX; k++)
-                    if (k < ns) acc += in.sc[k][i] * in.sc[k][j] * r1[k];
-                dd[i][j] = acc;
-                dd[j][i] = acc;
-            }
+            for (int j = 0; j < i; j++) dd[i][j] = dd[j][i];
     }
 }
 
@@ -367,8 +378,6 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-constexpr int NB_THREADS = 64;
-
 // One lane per voxel.  Writes sig (in place), edot, gamdot; block partials
 // (NRED doubles per block) of err_s, err_e, Newton iterations, sum sig,
 // sum sig over phase 1.
@@ -376,10 +385,11 @@ template <bool INTN, bool CSYM>
 __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_newton(
     NewtonParams P, long long N, const int *__restrict__ phase, const int *__restrict__ orient,
     const double *__restrict__ sgr, double *__restrict__ sig, const double *__restrict__ E,
-    const double *__restrict__ epsp, double *__restrict__ edot, const double *__restrict__ schm,
+    const double *__restrict__ epsp, double *__restrict__ edot, const double *__restrict__ tran,
     const double *__restrict__ crss, double *__restrict__ gamdot, double *__restrict__ partials,
     int *__restrict__ status, double *__restrict__ err_vox, int *__restrict__ niter_vox)
 {
+    __shared__ double slab[EVPGPU_NSYSMX * 5 * NB_THREADS];
     long long v = (long long)blockIdx.x * NB_THREADS + threadIdx.x;
     double red[NRED];
 #pragma unroll
@@ -409,17 +419,37 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
 #pragma unroll
                 for (int j = 0; j < 6; j++) V.sg[i][j] = sgv[i * 6 + j];
             const int ns = P.nsys[p];
+            // Schmid tensors in sample axes: sym((T n) x (T b)) in the 5-D basis
+            // (update_schmid, src/kinematics.c:430-463), into this lane's LDS column
+            double T[3][3];
+#pragma unroll
+            for (int q = 0; q < 9; q++) T[q / 3][q % 3] = tran[q * N + v];
+            in.sc = slab + threadIdx.x;
+            for (int s = 0; s < ns; s++) {
+                double tn[3], tb[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    tn[a] = T[a][0] * P.bn[p][s][0] + T[a][1] * P.bn[p][s][1] + T[a][2] * P.bn[p][s][2];
+                    tb[a] = T[a][0] * P.bb[p][s][0] + T[a][1] * P.bb[p][s][1] + T[a][2] * P.bb[p][s][2];
+                }
+                double P00 = tn[0] * tb[0], P11 = tn[1] * tb[1], P22 = tn[2] * tb[2];
+                double P12 = (tn[1] * tb[2] + tn[2] * tb[1]) / 2.0;
+                double P02 = (tn[0] * tb[2] + tn[2] * tb[0]) / 2.0;
+                double P01 = (tn[0] * tb[1] + tn[1] * tb[0]) / 2.0;
+                double *col = slab + threadIdx.x + s * 5 * NB_THREADS;
+                col[0 * NB_THREADS] = P00 * (-1. * RSQ2) + P11 * RSQ2;
+                col[1 * NB_THREADS] = P00 * (-1. * RSQ6) + P11 * (-1. * RSQ6) + P22 * (2. * RSQ6);
+                col[2 * NB_THREADS] = P12 * RSQ2 + P12 * RSQ2;
+                col[3 * NB_THREADS] = P02 * RSQ2 + P02 * RSQ2;
+                col[4 * NB_THREADS] = P01 * RSQ2 + P01 * RSQ2;
+            }
 #pragma unroll
             for (int s = 0; s < EVPGPU_NSYSMX; s++) {
                 if (s < ns) {
-#pragma unroll
-                    for (int j = 0; j < 5; j++) in.sc[s][j] = schm[(s * 5 + j) * N + v];
-                    in.tf[s] = crss[(s * 2) * N + v];
-                    in.tb[s] = CSYM ? in.tf[s] : crss[(s * 2 + 1) * N + v];
+                    in.itf[s] = 1.0 / crss[(s * 2) * N + v];
+                    in.itb[s] = CSYM ? in.itf[s] : 1.0 / crss[(s * 2 + 1) * N + v];
                 } else {
-#pragma unroll
-                    for (int j = 0; j < 5; j++) in.sc[s][j] = 0.0;
-                    in.tf[s] = in.tb[s] = 1.0;
+                    in.itf[s] = in.itb[s] = 1.0;
                 }
             }
             double s6[6], e6[6], gd[EVPGPU_NSYSMX], cs, ce;
@@ -479,30 +509,36 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
     }
 }
 
-// Sum nblk rows of nval partials into out[nval], one thread per value,
-// rows in order (deterministic).
-__global__ void k_reduce_rows(const double *__restrict__ partials, int nblk, int nval,
-                              double *__restrict__ out)
+// Deterministic two-stage sum of nblk rows of nval (<= 16) partials.
+// Stage 1: workgroup g sums rows [g*per, (g+1)*per): thread t owns value
+// t % 16 and rows t/16, t/16 + 16, ... (coalesced row-major reads), then the
+// 16 row lanes of each value are summed in LDS in a fixed order.
+// Stage 2: one workgroup sums the stage-1 rows the same way.
+constexpr int RED_MAXV = 16;
+
+__global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ partials, int nblk,
+                                                     int nval, int per, double *__restrict__ out)
 {
     __shared__ double sh[256];
-    for (int val = 0; val < nval; val++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += 256) s += partials[(long long)b * nval + val];
-        sh[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[val] = sh[0];
-        __syncthreads();
+    const int v = threadIdx.x % RED_MAXV, r0 = threadIdx.x / RED_MAXV;
+    const long long b0 = (long long)blockIdx.x * per;
+    const long long b1 = min((long long)nblk, b0 + per);
+    double s = 0.0;
+    if (v < nval)
+        for (long long b = b0 + r0; b < b1; b += 256 / RED_MAXV) s += partials[b * nval + v];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < RED_MAXV) {
+        double t = 0.0;
+        for (int r = 0; r < 256 / RED_MAXV; r++) t += sh[r * RED_MAXV + threadIdx.x];
+        if (threadIdx.x < nval) out[(long long)blockIdx.x * nval + threadIdx.x] = t;
     }
 }
 
 } // namespace
 
 int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const double *E,
-                  const double *epsp, double *edot, const double *schm, const double *crss,
+                  const double *epsp, double *edot, const double *tran, const double *crss,
                   double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status, double *err_vox,
                   int *niter_vox)
@@ -513,7 +549,7 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const doubl
     for (int q = 0; q < 2; q++)
         if (!P.is_gas[q] && P.int_n[q] == 0) intn = false;
 #define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig, E, \
-                    epsp, edot, schm, crss, gamdot, partials, status, err_vox, niter_vox
+                    epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
     if (intn && P.crss_sym) hipLaunchKernelGGL((k_newton<true, true>), NEWTON_ARGS);
     else if (intn) hipLaunchKernelGGL((k_newton<true, false>), NEWTON_ARGS);
     else if (P.crss_sym) hipLaunchKernelGGL((k_newton<false, true>), NEWTON_ARGS);
@@ -523,8 +559,15 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const doubl
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 
+// out[nval] = column sums of partials[nblk][nval]; uses c->red2 (REDUCE_G rows)
 int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out)
 {
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->stream, partials, nblk, nval, out);
+    if (nval > RED_MAXV) return EVPGPU_ERR_ARG;
+    int g = nblk < REDUCE_G ? nblk : REDUCE_G;
+    int per = (nblk + g - 1) / g;
+    g = (nblk + per - 1) / per;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(g), dim3(256), 0, c->stream, partials, nblk, nval, per, c->red2);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->stream, (const double *)c->red2, g, nval, g,
+                       out);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }

@@ -64,6 +64,7 @@ class Stats(C.Structure):
 # every symbol each header declares (checked by the CPU tests)
 EVPGPU_SYMBOLS = [
     "evpgpu_strerror", "evpgpu_create", "evpgpu_destroy", "evpgpu_set_precision",
+    "evpgpu_set_anchor_interval",
     "evpgpu_set_phases", "evpgpu_set_loading", "evpgpu_upload_microstructure",
     "evpgpu_upload_state", "evpgpu_download_state", "evpgpu_set_macro", "evpgpu_get_macro",
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
@@ -107,6 +108,7 @@ def gpu():
         L.evpgpu_create.argtypes = [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_int]
         L.evpgpu_destroy.argtypes = [VP]
         L.evpgpu_set_precision.argtypes = [VP, C.c_int]
+        L.evpgpu_set_anchor_interval.argtypes = [VP, C.c_int]
         L.evpgpu_set_phases.argtypes = [VP, C.POINTER(Phase), C.c_int]
         L.evpgpu_set_loading.argtypes = [VP, C.POINTER(Loading)]
         L.evpgpu_upload_microstructure.argtypes = [VP, IP, IP, IP, C.c_int, DP]
@@ -215,6 +217,9 @@ class GpuJob:
     def set_precision(self, precision: int):
         _chk(gpu().evpgpu_set_precision(self.ctx, precision), "evpgpu_set_precision")
 
+    def set_anchor_interval(self, iterations: int):
+        _chk(gpu().evpgpu_set_anchor_interval(self.ctx, iterations), "evpgpu_set_anchor_interval")
+
     def macro(self) -> dict:
         m = Macro()
         _chk(evp().evp_job_macro(self.h, C.byref(m)), "evp_job_macro")
@@ -308,7 +313,7 @@ class Context:
              "green")
         return out
 
-    def newton(self, phase, orient, sgr, sig6, strain6, eps6, schm60, crss24):
+    def newton(self, phase, orient, sgr, sig6, strain6, eps6, tran9, crss24):
         """Per-voxel stress update on a batch (component-major arrays)."""
         n = len(phase)
         phase = np.ascontiguousarray(phase, dtype=np.int32)
@@ -317,7 +322,7 @@ class Context:
         sig = np.array(sig6, dtype=np.float64, order="C")
         ed, gd = np.zeros((6, n)), np.zeros((12, n))
         err, nit = np.zeros((n, 2)), np.zeros(n, dtype=np.int32)
-        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (strain6, eps6, schm60, crss24)]
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (strain6, eps6, tran9, crss24)]
         st = gpu().evpgpu_newton_host(self.h, n, _p(phase, IP), _p(orient, IP), _p(sgr),
                                       sgr.size // 36, _p(sig), _p(args[0]), _p(args[1]),
                                       _p(args[2]), _p(args[3]), _p(ed), _p(gd), _p(err),

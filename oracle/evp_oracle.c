@@ -1395,6 +1395,7 @@ int orc_threads(void)
 void orc_job_dims(const orc_job *J, int d[3]) { memcpy(d, J->CellDim, sizeof(J->CellDim)); }
 int *orc_job_grain(orc_job *J) { return J->grain; }
 int *orc_job_phase(orc_job *J) { return J->phase; }
+int *orc_job_newton_counts(orc_job *J) { return J->nr_v; }
 double orc_job_newton_iters(const orc_job *J) { return J->newton_iters; }
 
 double *orc_job_field(orc_job *J, int which)

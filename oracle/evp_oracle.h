@@ -88,6 +88,8 @@ void orc_job_dims(const orc_job *job, int dims[3]);
 double *orc_job_field(orc_job *job, int which);
 int *orc_job_grain(orc_job *job);
 int *orc_job_phase(orc_job *job);
+/* Newton iterations of each voxel in the last Green iteration */
+int *orc_job_newton_counts(orc_job *job);
 /* macro scalars: see orc_job_macro() in evp_oracle.c for the layout */
 void orc_job_macro(const orc_job *job, double *out /* 64 doubles */);
 double orc_job_newton_iters(const orc_job *job); /* total Newton iterations so far */

@@ -50,6 +50,8 @@ def lib():
         L.orc_job_grain.argtypes = [C.c_void_p]
         L.orc_job_phase.restype = IP
         L.orc_job_phase.argtypes = [C.c_void_p]
+        L.orc_job_newton_counts.restype = IP
+        L.orc_job_newton_counts.argtypes = [C.c_void_p]
         L.orc_job_macro.argtypes = [C.c_void_p, DP]
         L.orc_job_setup_info.argtypes = [C.c_void_p, DP, DP, DP, DP]
         L.orc_job_cgr.argtypes = [C.c_void_p, DP]
@@ -129,6 +131,9 @@ class Job:
 
     def grain(self):
         return np.ctypeslib.as_array(lib().orc_job_grain(self.h), shape=(self.n,)).copy()
+
+    def newton_counts(self):
+        return np.ctypeslib.as_array(lib().orc_job_newton_counts(self.h), shape=(self.n,)).copy()
 
     def phase(self):
         return np.ctypeslib.as_array(lib().orc_job_phase(self.h), shape=(self.n,)).copy()

@@ -149,7 +149,7 @@ def test_newton_vs_oracle(job_factory):
     st, s_out, ed, gd, err, nit = lib.Context.newton(
         gj_ctx(gj), np.ones(m, np.int32), np.arange(m, dtype=np.int32), sg.reshape(m, 36),
         sym6(sig[sel]), sym6(strain[sel]), sym6(eps[sel]),
-        schm.reshape(m, 60).T.copy(), crss[sel].reshape(m, 24).T.copy())
+        tran[sel].reshape(m, 9).T.copy(), crss[sel].reshape(m, 24).T.copy())
     assert st == 0
     s_ref = np.array([sym6(r[1][None])[:, 0] for r in ref]).T
     e_ref = np.array([sym6(r[2][None])[:, 0] for r in ref]).T
