@@ -70,6 +70,13 @@ int evp_job_setup_info(const evp_job *job, double *c066, double *s066, double *c
 int evp_job_reset(evp_job *job);
 /* vm.out row of the last increment (TIME EVM EVMP DVM DVMP SVM SVM_1) */
 int evp_job_vm_row(const evp_job *job, double row[7]);
+/* Calibration parameters of crystalline phase `phase` (0 or 1), in the order
+ * EVPGPU_P_TAU0, TAU1, THETA0, THETA1, NRS (evpgpu.h): set them on every
+ * slip system as the .sx reader would (tau0 forward = backward), then
+ * restore the post-setup state with the new initial CRSS.  No counterpart in
+ * the reference (parameters only come from the .sx file there).            */
+int evp_job_set_params(evp_job *job, int phase, const double params[5]);
+int evp_job_get_params(const evp_job *job, int phase, double params[5]);
 
 #ifdef __cplusplus
 }

@@ -35,7 +35,7 @@ enum {
     EVPGPU_ERR_LINESEARCH = 1203, /* slope >= 0, src/evolution.c:210-214          */
     EVPGPU_ERR_ARG = -1,          /* bad argument                                 */
     EVPGPU_ERR_HIP = -2,          /* HIP runtime failure                          */
-    EVPGPU_ERR_SIZE = -3,         /* grid size not supported (powers of 2, 8..512) */
+    EVPGPU_ERR_SIZE = -3,         /* grid size not supported (powers of 2, 8..256) */
     EVPGPU_ERR_NOMEM = -4,        /* device allocation failed                     */
     EVPGPU_ERR_STATE = -5,        /* call out of order                            */
     EVPGPU_ERR_NODEV = -6         /* no HIP device                                */
@@ -100,6 +100,13 @@ const char *evpgpu_strerror(int status);
 int evpgpu_create(evpgpu_ctx **ctx, int nx, int ny, int nz, int precision);
 int evpgpu_destroy(evpgpu_ctx *ctx);
 int evpgpu_set_precision(evpgpu_ctx *ctx, int precision);
+/* Newton tolerances of the per-voxel stress update: relative step (default
+ * 1e-10, src/evolution.c:105-110) and line-search TOLX (default 1e-7,
+ * src/evolution.c:166-170).  The defaults reproduce the reference; tighter
+ * values make the loop output smooth in the material parameters at the
+ * 1e-6 finite-difference scale (TOLX rejects Newton steps below 1e-7
+ * relative), for validating derivatives. */
+int evpgpu_set_newton_tolerances(evpgpu_ctx *ctx, double tol, double tolx);
 /* Mixed precision: iterations between FP64 transforms of the stress anchor
  * (default 16, at least 1).  No counterpart in the reference (FP64 only). */
 int evpgpu_set_anchor_interval(evpgpu_ctx *ctx, int iterations);
@@ -155,6 +162,49 @@ int evpgpu_newton_host(evpgpu_ctx *ctx, int n, const int *phase, const int *orie
                        const double *sgr, int n_orient, double *sig6, const double *strain6,
                        const double *eps6, const double *tranmat9, const double *crss24,
                        double *edot6, double *gamdot12, double *err2, int *niter);
+
+/* ------------------------------------------------------------ adjoint --
+ * Reverse-mode derivative of the macroscopic stress curve with respect to
+ * the hardening and rate parameters of each phase, for calibration.  No
+ * counterpart in the reference (it has no derivatives); it differentiates
+ * the converged equations of every increment (implicit differentiation of
+ * the Green fixed point, src/evolution.c:1343-1463, and of UpdateStress,
+ * src/evolution.c:44-310) and the increment-to-increment updates
+ * (Eps, update_orient, harden: src/evolution.c:1469-1547,
+ * src/constitutive.c:196-326).  Requirements: crystalline phases only (no
+ * gas voxels), ElastBC = 1, no creep stop.
+ *
+ * Parameters, per phase p (grad[p*EVPGPU_NPARAM + i]):
+ *   TAU0   initial CRSS, forward and backward (the .sx tau0xf = tau0xb)
+ *   TAU1, THETA0, THETA1   Voce constants of every system
+ *   NRS    rate exponent n of every system                                */
+enum { EVPGPU_P_TAU0 = 0, EVPGPU_P_TAU1, EVPGPU_P_THETA0, EVPGPU_P_THETA1, EVPGPU_P_NRS, EVPGPU_NPARAM };
+
+typedef struct {
+    int cg_iters;         /* conjugate-gradient iterations, all increments */
+    int cg_max_iters;     /* largest count of one increment */
+    double cg_rel_resid;  /* largest final relative residual */
+    double seconds;       /* wall time of the reverse pass, replays included */
+    double replay_seconds;/* of which forward replays from checkpoints */
+} evpgpu_adjoint_stats;
+
+/* Run n increments from the current state like evpgpu_increment (natural
+ * convergence), keeping a device snapshot of the state every
+ * checkpoint_every increments and after the last one.  sig_avg (n x 6,
+ * 11 22 33 23 13 12) receives the macroscopic stress after each increment,
+ * stats (n) the increment statistics; either may be NULL. */
+int evpgpu_adjoint_record(evpgpu_ctx *ctx, int n_increments, int checkpoint_every,
+                          double *sig_avg, evpgpu_stats *stats);
+
+/* Reverse pass over the recorded increments.  sig_avg_bar (n x 6): dL/d of
+ * each recorded macroscopic stress component (the 23 entry stands for the
+ * symmetric pair).  grad (2 x EVPGPU_NPARAM) receives dL/d(parameters) of
+ * the two phases.  Each increment's adjoint solve is a conjugate-gradient
+ * solve to relative residual cg_tol (at most cg_max iterations).  Segments
+ * are replayed from the checkpoints; the context is left in the state after
+ * the recorded increments. */
+int evpgpu_adjoint_run(evpgpu_ctx *ctx, const double *sig_avg_bar, double cg_tol, int cg_max,
+                       double *grad, evpgpu_adjoint_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -131,6 +131,8 @@ NewtonParams newton_params(const evpgpu_ctx *c)
         memcpy(P.bb[p], ph.bb, sizeof(P.bb[p]));
     }
     P.crss_sym = c->crss_sym;
+    P.nr_tol = c->nr_tol;
+    P.tolx = c->tolx;
     return P;
 }
 
@@ -332,6 +334,8 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
     if (!st) st = dalloc(&c->skA, 3 * c->NK);
     if (!st) st = dalloc(&c->Estart, 6 * N);
     c->anchor_every = 16;
+    c->nr_tol = 1E-10;
+    c->tolx = 1.E-7;
     if (!st) st = dalloc(&c->partials, nblk * NRED);
     if (!st) st = dalloc(&c->red, NRED);
     if (!st) st = dalloc(&c->red2, (long long)REDUCE_G * NRED);
@@ -356,6 +360,7 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
 int evpgpu_destroy(evpgpu_ctx *c)
 {
     if (!c) return EVPGPU_OK;
+    adjoint_free(c);
     void *ptrs[] = {c->sig, c->E, c->W, c->epsp, c->edot, c->tran, c->crss, c->gamacum, c->gamdot,
                     c->phase, c->grain, c->orient, c->sgr, c->spec, c->acc, c->rwork,
                     c->sigA, c->cA, c->skA, c->Estart, c->partials, c->red, c->red2, c->d_status, c->tw64[0], c->tw64[1], c->tw64[2],
@@ -397,6 +402,14 @@ int evpgpu_get_timing(evpgpu_ctx *c, int which, double *ms, long long *launches)
     if (!c || which < 0 || which > 4) return EVPGPU_ERR_ARG;
     if (ms) *ms = c->t_ms[which];
     if (launches) *launches = c->t_n[which];
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_newton_tolerances(evpgpu_ctx *c, double tol, double tolx)
+{
+    if (!c || !(tol > 0.0) || !(tolx > 0.0)) return EVPGPU_ERR_ARG;
+    c->nr_tol = tol;
+    c->tolx = tolx;
     return EVPGPU_OK;
 }
 
@@ -573,7 +586,11 @@ int evpgpu_get_macro(evpgpu_ctx *c, evpgpu_macro *m)
     return EVPGPU_OK;
 }
 
-int evpgpu_increment(evpgpu_ctx *c, int forced, evpgpu_stats *stats)
+// One loading increment.  forced > 0 runs exactly that many iterations;
+// rep64 >= 0 replays a recorded increment: rep_mix mixed iterations (mixed
+// precision only), the projection, then rep64 FP64 iterations -- the same
+// kernel sequence as the run that recorded it, so the same result bit for bit.
+int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_stats *stats)
 {
     if (!ready(c)) return EVPGPU_ERR_STATE;
     evpgpu_stats S;
@@ -634,7 +651,20 @@ int evpgpu_increment(evpgpu_ctx *c, int forced, evpgpu_stats *stats)
         it64++;
         return green_iteration(c, EVPGPU_FP64, 0, &nit, &ntr);
     };
-    if (forced > 0) {
+    if (rep64 >= 0) {
+        if (mixed) {
+            for (int i = 0; i < rep_mix; i++) {
+                st = mixed_step();
+                if (st) return st;
+            }
+            st = project_increment(c);
+            if (st) return st;
+        }
+        for (int i = 0; i < rep64; i++) {
+            st = fp64_step();
+            if (st) return st;
+        }
+    } else if (forced > 0) {
         if (mixed) {
             while (iter < forced - 1) {
                 st = mixed_step();
@@ -740,6 +770,11 @@ int evpgpu_increment(evpgpu_ctx *c, int forced, evpgpu_stats *stats)
     m.istep++;
     if (stats) *stats = S;
     return EVPGPU_OK;
+}
+
+int evpgpu_increment(evpgpu_ctx *c, int forced, evpgpu_stats *stats)
+{
+    return increment_core(c, forced, 0, -1, stats);
 }
 
 // ------------------------------------------------------ building blocks --

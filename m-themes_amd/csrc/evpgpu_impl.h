@@ -32,6 +32,8 @@ struct NewtonParams {
     int int_n[2];    // integer rate exponent of the phase, 0 if not integral
     int crss_sym;    // 1: forward and backward CRSS are equal in every voxel
     double bn[2][EVPGPU_NSYSMX][3], bb[2][EVPGPU_NSYSMX][3]; // crystal-axis normals, directions
+    double nr_tol;   // Newton tolerance on the relative step (reference 1e-10)
+    double tolx;     // line-search TOLX (reference 1e-7)
 };
 
 struct GreenParams {
@@ -106,6 +108,7 @@ struct evpgpu_ctx {
     float2 *skA;      // 3*NK
     double *Estart;   // 6N
     int anchor_every; // iterations between FP64 anchor transforms
+    double nr_tol, tolx; // Newton tolerances (evpgpu_set_newton_tolerances)
     double2 *tw64[3];
     float2 *tw32[3];
 
@@ -129,7 +132,15 @@ struct evpgpu_ctx {
     double udot[9], udot_s[9], udot_a[9], scauchy[9];
     int creep_flag;
     double disgrad_avg_actual[9];
+
+    // reverse pass (adjoint.hip), allocated by evpgpu_adjoint_record
+    struct evpgpu_adj *adj;
 };
+
+// evpgpu.hip (defined inside the ABI's extern "C" block)
+extern "C" int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_stats *stats);
+// adjoint.hip
+void adjoint_free(evpgpu_ctx *c);
 
 // ---- launchers (defined in the .hip files) ----
 // fft.hip
@@ -138,6 +149,7 @@ int fft_inverse_update(evpgpu_ctx *c, int ncomp, int precision, double *target,
                        const double *add_const, double scale);
 int fft_inverse_real(evpgpu_ctx *c, const double2 *cin, int ncomp, double *out, double scale);
 int fft_twiddles(evpgpu_ctx *c);
+int fft_forward_to(evpgpu_ctx *c, const double *real_in, int ncomp, double2 *dst);
 int fft_forward_delta(evpgpu_ctx *c, const double *sig, const float *anchor);
 int fft_forward_anchor(evpgpu_ctx *c, const float *anchor);
 int fft_inverse_delta(evpgpu_ctx *c, double *E, const double *add, const float *cA, double scale);

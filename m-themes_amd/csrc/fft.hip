@@ -361,6 +361,12 @@ int fft_forward(evpgpu_ctx *c, const double *real_in, int ncomp, int precision)
 }
 
 // mixed loop: FP32 transform of (sig - anchor), anchor stored in FP32
+// FP64 R2C of ncomp real fields into a caller-chosen spectrum buffer
+int fft_forward_to(evpgpu_ctx *c, const double *real_in, int ncomp, double2 *dst)
+{
+    return fwd<double, 0>(c, real_in, nullptr, ncomp, dst);
+}
+
 int fft_forward_delta(evpgpu_ctx *c, const double *sig, const float *anchor)
 {
     return fwd<float, 1>(c, sig, anchor, 6, (float2 *)c->spec);

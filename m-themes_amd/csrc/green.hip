@@ -21,6 +21,7 @@
 // accumulator that the increment epilogue inverts once (they only feed the
 // lattice spin of update_orient).
 #include "evpgpu_impl.h"
+#include "spectral.h"
 
 namespace {
 
@@ -28,26 +29,7 @@ template <typename R> struct CT;
 template <> struct CT<double> { typedef double2 T; };
 template <> struct CT<float> { typedef float2 T; };
 
-__device__ __forceinline__ void unit_k(int x, int y, int z, int nx, int ny, int nz, double n[3])
-{
-    const double TWO_PI_REF = 6.28318530718; // src/evp.h:78
-    double t[3] = {(double)x * (double)(1.0 / nx), (double)y * (double)(1.0 / ny),
-                   (double)z * (double)(1.0 / nz)};
-#pragma unroll
-    for (int m = 0; m < 3; m++) {
-        t[m] -= round(t[m]);
-        t[m] *= TWO_PI_REF;
-    }
-    double g2 = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-    if (x + y + z == 0) {
-        g2 = 1.0;
-        t[0] = t[1] = t[2] = 0.0;
-    }
-    double s = 1. / g2;
-    n[0] = t[0] * s;
-    n[1] = t[1] * s;
-    n[2] = t[2] * s;
-}
+using kspace::unit_k;
 
 // e = Gamma(n) s for complex s (3x3 full), e accumulated with weight w
 template <typename R, typename C>
@@ -202,8 +184,7 @@ __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 
 
 // Orthogonal projection of a symmetric strain spectrum onto compatible
 // fields sym(a x n): a = 2 e.n - (n.e.n) n (exact for e = sym(a x n)).  The
-// k = 0 mode (macroscopic strain) and the Nyquist wave vectors where
-// Gamma_eff mixes two directions are left unchanged.
+// k = 0 mode (macroscopic strain) is left unchanged.
 __global__ __launch_bounds__(256) void k_project(double2 *spec, int nx, int ny, int nz, int nzh,
                                                  long long NK)
 {
@@ -215,11 +196,25 @@ __global__ __launch_bounds__(256) void k_project(double2 *spec, int nx, int ny, 
     if (x == 0 && y == 0 && kz == 0) return;
     double n[3], n2[3];
     unit_k(x, y, kz, nx, ny, nz, n);
-    if ((2 * x == nx) || (2 * y == ny) || (2 * kz == nz)) {
-        unit_k((nx - x) % nx, (ny - y) % ny, (nz - kz) % nz, nx, ny, nz, n2);
-        bool same = (n2[0] == n[0] && n2[1] == n[1] && n2[2] == n[2]) ||
-                    (n2[0] == -n[0] && n2[1] == -n[1] && n2[2] == -n[2]);
-        if (!same) return;
+    if (kspace::mode_dirs(x, y, kz, nx, ny, nz, n, n2)) {
+        // Gamma_eff mixes n and n2 here: the compatible set is
+        // {sym(a x n + b x n2)}, whose orthogonal complement is m x m with
+        // m = n x n2, so remove that one component
+        double m[3] = {n[1] * n2[2] - n[2] * n2[1], n[2] * n2[0] - n[0] * n2[2],
+                       n[0] * n2[1] - n[1] * n2[0]};
+        double mm = 1.0 / sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        for (int i = 0; i < 3; i++) m[i] *= mm;
+        const double w[6] = {m[0] * m[0], m[1] * m[1], m[2] * m[2], m[1] * m[2], m[0] * m[2], m[0] * m[1]};
+        double2 v[6], s = make_double2(0.0, 0.0);
+        for (int c = 0; c < 6; c++) {
+            v[c] = spec[c * NK + id];
+            const double f = c < 3 ? 1.0 : 2.0; // m.e.m: off-diagonal entries count twice
+            s.x += f * w[c] * v[c].x;
+            s.y += f * w[c] * v[c].y;
+        }
+        for (int c = 0; c < 6; c++)
+            spec[c * NK + id] = make_double2(v[c].x - s.x * w[c], v[c].y - s.y * w[c]);
+        return;
     }
     double2 v[6];
 #pragma unroll

@@ -1,0 +1,156 @@
+// adjoint_check.cpp -- host build of the per-voxel adjoint functions
+// (csrc/adjoint_voxel.h, the same source the device kernels compile) behind
+// plain C entry points, so that each transposed linearisation can be checked
+// against finite differences of its forward function on the CPU.
+#include <string.h>
+
+#include "../adjoint_voxel.h"
+
+using namespace adj;
+
+namespace {
+
+void load_T(const double *t9, double T[3][3])
+{
+    for (int i = 0; i < 9; i++) T[i / 3][i % 3] = t9[i];
+}
+
+void store_T(const double T[3][3], double *t9)
+{
+    for (int i = 0; i < 9; i++) t9[i] = T[i / 3][i % 3];
+}
+
+HardParams hard_params(int ns, double dt, const double *hard, const double *tau, const double *thet)
+{
+    HardParams H;
+    memset(&H, 0, sizeof(H));
+    H.ns = ns;
+    H.dt = dt;
+    for (int i = 0; i < ns; i++) {
+        for (int j = 0; j < ns; j++) H.hard[i][j] = hard[i * ns + j];
+        for (int k = 0; k < 3; k++) H.tau[i][k] = tau[i * 3 + k];
+        for (int k = 0; k < 2; k++) H.thet[i][k] = thet[i * 2 + k];
+    }
+    return H;
+}
+
+OrientParams orient_params(int ns, double dt, const double *udot_a, const double *bn, const double *bb)
+{
+    OrientParams O;
+    memset(&O, 0, sizeof(O));
+    O.ns = ns;
+    O.dt = dt;
+    for (int i = 0; i < 9; i++) O.udot_a[i] = udot_a[i];
+    for (int s = 0; s < ns; s++)
+        for (int k = 0; k < 3; k++) {
+            O.bn[s][k] = bn[s * 3 + k];
+            O.bb[s][k] = bb[s * 3 + k];
+        }
+    return O;
+}
+
+} // namespace
+
+extern "C" {
+
+void adjc_schmid(const double *t9, const double *bn, const double *bb, double *c5)
+{
+    double T[3][3];
+    load_T(t9, T);
+    schmid(T, bn, bb, c5);
+}
+
+void adjc_schmid_bar(const double *t9, const double *bn, const double *bb, const double *cb5, double *tb9)
+{
+    double T[3][3], Tb[3][3];
+    load_T(t9, T);
+    load_T(tb9, Tb);
+    schmid_bar(T, bn, bb, cb5, Tb);
+    store_T(Tb, tb9);
+}
+
+void adjc_rates(int ns, const double *c, const double *crss, const double *nrs, const double *g0,
+                const double *sig, double *gd, double *ed, double *dd36)
+{
+    double C[ADJ_NS][5], D[6][6];
+    for (int s = 0; s < ns; s++)
+        for (int j = 0; j < 5; j++) C[s][j] = c[s * 5 + j];
+    rates(ns, C, crss, nrs, g0, sig, gd, ed, dd36 ? D : nullptr);
+    if (dd36)
+        for (int i = 0; i < 36; i++) dd36[i] = D[i / 6][i % 6];
+}
+
+void adjc_rates_bar(int ns, const double *c, const double *crss, const double *nrs, const double *g0,
+                    const double *sig, const double *gdb, const double *edb, double *sigb, double *cb,
+                    double *crssb, double *nb)
+{
+    double C[ADJ_NS][5], CB[ADJ_NS][5];
+    for (int s = 0; s < ns; s++)
+        for (int j = 0; j < 5; j++) {
+            C[s][j] = c[s * 5 + j];
+            CB[s][j] = cb[s * 5 + j];
+        }
+    rates_bar(ns, C, crss, nrs, g0, sig, gdb, edb, sigb, CB, crssb, nb);
+    for (int s = 0; s < ns; s++)
+        for (int j = 0; j < 5; j++) cb[s * 5 + j] = CB[s][j];
+}
+
+double adjc_voce(double G, double dg, double tau0, double tau1, double th0, double th1)
+{
+    return voce(G, dg, tau0, tau1, th0, th1);
+}
+
+void adjc_voce_grad(double G, double dg, double tau0, double tau1, double th0, double th1, double *d5)
+{
+    voce_grad(G, dg, tau0, tau1, th0, th1, d5);
+}
+
+void adjc_harden(int ns, double dt, const double *hard, const double *tau, const double *thet,
+                 const double *gd, double *crss, double *G)
+{
+    HardParams H = hard_params(ns, dt, hard, tau, thet);
+    harden(H, gd, crss, *G);
+}
+
+void adjc_harden_bar(int ns, double dt, const double *hard, const double *tau, const double *thet,
+                     const double *gd, double G, const double *crssb, double *Gb, double *gdb, double *pb3)
+{
+    HardParams H = hard_params(ns, dt, hard, tau, thet);
+    harden_bar(H, gd, G, crssb, *Gb, gdb, pb3);
+}
+
+void adjc_orient(int ns, double dt, const double *udot_a, const double *bn, const double *bb, double *t9,
+                 const double *dW, const double *gd)
+{
+    OrientParams O = orient_params(ns, dt, udot_a, bn, bb);
+    double T[3][3];
+    load_T(t9, T);
+    orient(O, T, dW, gd);
+    store_T(T, t9);
+}
+
+void adjc_orient_bar(int ns, double dt, const double *udot_a, const double *bn, const double *bb,
+                     const double *t9, const double *dW, const double *gd, double *tb9, double *gdb,
+                     double *dWb)
+{
+    OrientParams O = orient_params(ns, dt, udot_a, bn, bb);
+    double T[3][3], Tb[3][3];
+    load_T(t9, T);
+    load_T(tb9, Tb);
+    orient_bar(O, T, dW, gd, Tb, gdb, dWb);
+    store_T(Tb, tb9);
+}
+
+int adjc_tangent(const double *S36, const double *D36, double dt, double *Ct36)
+{
+    double S[6][6], D[6][6], C[6][6];
+    for (int i = 0; i < 36; i++) {
+        S[i / 6][i % 6] = S36[i];
+        D[i / 6][i % 6] = D36[i];
+    }
+    bool ok = tangent(S, D, dt, C);
+    for (int i = 0; i < 36; i++) Ct36[i] = C[i / 6][i % 6];
+    return ok ? 0 : 1;
+}
+
+} // extern "C"

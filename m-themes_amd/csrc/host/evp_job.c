@@ -349,6 +349,35 @@ int evp_job_reset(evp_job *J)
     return upload_initial(J);
 }
 
+int evp_job_set_params(evp_job *J, int phase, const double q[5])
+{
+    if (!J || !q || phase < 0 || phase >= J->nl.N_phases || J->ph[phase].is_gas) return EVPGPU_ERR_ARG;
+    evpgpu_phase *ph = &J->ph[phase];
+    for (int s = 0; s < ph->nsys; s++) {
+        ph->tau[s][0] = q[EVPGPU_P_TAU0];
+        ph->tau[s][1] = q[EVPGPU_P_TAU0];
+        ph->tau[s][2] = q[EVPGPU_P_TAU1];
+        ph->thet[s][0] = q[EVPGPU_P_THETA0];
+        ph->thet[s][1] = q[EVPGPU_P_THETA1];
+        ph->nrs[s] = q[EVPGPU_P_NRS];
+    }
+    int st = evpgpu_set_phases(J->ctx, J->ph, J->nl.N_phases);
+    if (st) return st;
+    return upload_initial(J);
+}
+
+int evp_job_get_params(const evp_job *J, int phase, double q[5])
+{
+    if (!J || !q || phase < 0 || phase >= J->nl.N_phases || J->ph[phase].is_gas) return EVPGPU_ERR_ARG;
+    const evpgpu_phase *ph = &J->ph[phase];
+    q[EVPGPU_P_TAU0] = ph->tau[0][0];
+    q[EVPGPU_P_TAU1] = ph->tau[0][2];
+    q[EVPGPU_P_THETA0] = ph->thet[0][0];
+    q[EVPGPU_P_THETA1] = ph->thet[0][1];
+    q[EVPGPU_P_NRS] = ph->nrs[0];
+    return 0;
+}
+
 int evp_job_run(evp_job *J, int n, int forced, int *iters, evpgpu_stats *stats)
 {
     if (!J) return EVPGPU_ERR_ARG;

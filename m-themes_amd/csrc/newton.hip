@@ -250,10 +250,10 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
     conv_s = 0.0;
     conv_e = 0.0;
     iterl = 0;
-    double erral = 10 * 1E-10;
+    double erral = 10 * P.nr_tol;
 #pragma unroll
     for (int i = 0; i < 6; i++) sig6[i] = V.xl[i];
-    while ((iterl < 500) && (fabs(erral) > fabs(1E-10))) {
+    while ((iterl < 500) && (fabs(erral) > fabs(P.nr_tol))) {
         iterl++;
 #pragma unroll
         for (int i = 0; i < 6; i++) old[i] = sig6[i];
@@ -303,7 +303,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
 #pragma unroll
         for (int i = 0; i < 6; i++) f_old += res[i] * res[i];
         f_old *= 0.5;
-        const double stpmax = 10.0, ALF = 1.E-4, TOLX = 1.E-7;
+        const double stpmax = 10.0, ALF = 1.E-4, TOLX = P.tolx;
         double sum = 0.0, slope = 0.0, test = 0.0;
 #pragma unroll
         for (int i = 0; i < 6; i++) sum += step[i] * step[i];

@@ -61,21 +61,30 @@ class Stats(C.Structure):
                 ("ls_trials", C.c_double), ("seconds", C.c_double), ("stopped", C.c_int)]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("cg_iters", C.c_int), ("cg_max_iters", C.c_int), ("cg_rel_resid", C.c_double),
+                ("seconds", C.c_double), ("replay_seconds", C.c_double)]
+
+
+PARAM_NAMES = ("tau0", "tau1", "theta0", "theta1", "n")
+NPARAM = 5
+
+
 # every symbol each header declares (checked by the CPU tests)
 EVPGPU_SYMBOLS = [
     "evpgpu_strerror", "evpgpu_create", "evpgpu_destroy", "evpgpu_set_precision",
-    "evpgpu_set_anchor_interval",
+    "evpgpu_set_anchor_interval", "evpgpu_set_newton_tolerances",
     "evpgpu_set_phases", "evpgpu_set_loading", "evpgpu_upload_microstructure",
     "evpgpu_upload_state", "evpgpu_download_state", "evpgpu_set_macro", "evpgpu_get_macro",
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
-    "evpgpu_newton_host",
+    "evpgpu_newton_host", "evpgpu_adjoint_record", "evpgpu_adjoint_run",
 ]
 EVP_SYMBOLS = [
     "GetNameList", "PrintNameList", "OpenFiles", "CloseFiles", "SetupJob", "Evolution",
     "DestroyJob", "evp_set_abort_on_error", "evp_status", "evp_set_precision", "evp_job_open",
     "evp_job_close", "evp_job_ctx", "evp_job_dims", "evp_job_nsteps", "evp_job_run",
     "evp_job_macro", "evp_job_ids", "evp_job_grain_list", "evp_job_setup_info", "evp_job_reset",
-    "evp_job_vm_row",
+    "evp_job_vm_row", "evp_job_set_params", "evp_job_get_params",
 ]
 
 _gpu = None
@@ -109,6 +118,7 @@ def gpu():
         L.evpgpu_destroy.argtypes = [VP]
         L.evpgpu_set_precision.argtypes = [VP, C.c_int]
         L.evpgpu_set_anchor_interval.argtypes = [VP, C.c_int]
+        L.evpgpu_set_newton_tolerances.argtypes = [VP, C.c_double, C.c_double]
         L.evpgpu_set_phases.argtypes = [VP, C.POINTER(Phase), C.c_int]
         L.evpgpu_set_loading.argtypes = [VP, C.POINTER(Loading)]
         L.evpgpu_upload_microstructure.argtypes = [VP, IP, IP, IP, C.c_int, DP]
@@ -125,6 +135,8 @@ def gpu():
         L.evpgpu_green_host.argtypes = [VP, DP, DP]
         L.evpgpu_newton_host.argtypes = [VP, C.c_int, IP, IP, DP, C.c_int, DP, DP, DP, DP, DP,
                                          DP, DP, DP, IP]
+        L.evpgpu_adjoint_record.argtypes = [VP, C.c_int, C.c_int, DP, C.POINTER(Stats)]
+        L.evpgpu_adjoint_run.argtypes = [VP, DP, C.c_double, C.c_int, DP, C.POINTER(AdjointStats)]
         _gpu = L
     return _gpu
 
@@ -150,6 +162,8 @@ def evp():
         L.evp_job_setup_info.argtypes = [VP, DP, DP, DP, DP]
         L.evp_job_reset.argtypes = [VP]
         L.evp_job_vm_row.argtypes = [VP, DP]
+        L.evp_job_set_params.argtypes = [VP, C.c_int, DP]
+        L.evp_job_get_params.argtypes = [VP, C.c_int, DP]
         L.GetNameList.argtypes = [C.POINTER(C.c_char_p)]
         L.evp_set_abort_on_error.argtypes = [C.c_int]
         L.evp_set_precision.argtypes = [C.c_int]
@@ -217,6 +231,9 @@ class GpuJob:
     def set_precision(self, precision: int):
         _chk(gpu().evpgpu_set_precision(self.ctx, precision), "evpgpu_set_precision")
 
+    def set_newton_tolerances(self, tol: float = 1e-10, tolx: float = 1e-7):
+        _chk(gpu().evpgpu_set_newton_tolerances(self.ctx, tol, tolx), "evpgpu_set_newton_tolerances")
+
     def set_anchor_interval(self, iterations: int):
         _chk(gpu().evpgpu_set_anchor_interval(self.ctx, iterations), "evpgpu_set_anchor_interval")
 
@@ -241,6 +258,34 @@ class GpuJob:
         c066, s066, c0, s0 = (np.zeros(k) for k in (36, 36, 81, 81))
         _chk(evp().evp_job_setup_info(self.h, _p(c066), _p(s066), _p(c0), _p(s0)), "setup_info")
         return c066.reshape(6, 6), s066.reshape(6, 6), c0.reshape(3, 3, 3, 3), s0.reshape(3, 3, 3, 3)
+
+    # ------------------------------------------------------ calibration --
+    def set_params(self, phase: int, params):
+        """(tau0, tau1, theta0, theta1, n) of crystalline phase `phase` (0-based);
+        resets the job to its post-setup state with the new initial CRSS."""
+        q = np.ascontiguousarray(params, dtype=np.float64)
+        assert q.shape == (NPARAM,)
+        _chk(evp().evp_job_set_params(self.h, phase, _p(q)), "evp_job_set_params")
+
+    def get_params(self, phase: int) -> np.ndarray:
+        q = np.zeros(NPARAM)
+        _chk(evp().evp_job_get_params(self.h, phase, _p(q)), "evp_job_get_params")
+        return q
+
+    def adjoint_record(self, n: int, checkpoint_every: int = 5):
+        """Run n increments keeping checkpoints; returns (sig_avg[n, 6], stats list)."""
+        sa = np.zeros((n, 6))
+        st = (Stats * n)()
+        _chk(gpu().evpgpu_adjoint_record(self.ctx, n, checkpoint_every, _p(sa), st), "adjoint_record")
+        return sa, [{f: getattr(x, f) for f, _ in Stats._fields_} for x in st]
+
+    def adjoint_run(self, sig_avg_bar, cg_tol: float = 1e-10, cg_max: int = 2000):
+        """dL/d(params) [2, 5] for the seeds dL/dsig_avg [n, 6] of the recorded run."""
+        b = np.ascontiguousarray(sig_avg_bar, dtype=np.float64)
+        g = np.zeros((2, NPARAM))
+        S = AdjointStats()
+        _chk(gpu().evpgpu_adjoint_run(self.ctx, _p(b), cg_tol, cg_max, _p(g), C.byref(S)), "adjoint_run")
+        return g, {f: getattr(S, f) for f, _ in AdjointStats._fields_}
 
     def state(self) -> dict:
         """Fields in the reference's AoS tensor layout ([N][3][3] etc.)."""

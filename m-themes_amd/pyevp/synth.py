@@ -261,11 +261,11 @@ def write_sx(path: str, m: Material):
         "# Labels of the modes to be used (iMode):", "1", "",
         "# Mode# 1 -- <110>(111) SLIP",
         "\tnsmx\tnrsx\tgamd0x\ttwshx\tisectwx",
-        f"\t12\t\t{m.nrs!r}\t\t{m.gam0!r}\t\t0.0\t\t0",
+        f"\t12\t\t{float(m.nrs)!r}\t\t{float(m.gam0)!r}\t\t0.0\t\t0",
         "\ttau0xf\ttau0xb\ttau1x\tthet0\tthet1",
-        f"\t{m.tau0f!r}\t\t{m.tau0b!r}\t\t{m.tau1!r}\t\t{m.thet0!r}\t{m.thet1!r}",
+        f"\t{float(m.tau0f)!r}\t\t{float(m.tau0b)!r}\t\t{float(m.tau1)!r}\t\t{float(m.thet0)!r}\t{float(m.thet1)!r}",
         "\thselfx\thlatex",
-        f"\t{m.hself!r}\t\t{m.hlat!r}",
+        f"\t{float(m.hself)!r}\t\t{float(m.hlat)!r}",
         "\tSlip (n-b)",
     ]
     for nv, bv in FCC_SLIP:

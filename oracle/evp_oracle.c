@@ -434,6 +434,16 @@ static double nr_quality(const nrctx *c, const double sig[6])
     return f / 2.0;
 }
 
+/* Newton tolerances: 1e-10 on the relative step and TOLX = 1e-7 on the line
+ * search (src/evolution.c:105-110, 166-170); adjustable for validation runs. */
+static double g_nr_tol = 1E-10, g_tolx = 1.E-7;
+
+void orc_set_newton_tol(double tol, double tolx)
+{
+    g_nr_tol = tol;
+    g_tolx = tolx;
+}
+
 int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6], double dt,
                             double sig[3][3], const double eps[3][3],
                             const double disgrad[3][3], int nsys, const double sc[][5],
@@ -460,7 +470,7 @@ int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6], do
     signorm = sqrt(signorm);
 
     nrctx c = {sg66, c066, xl6, eps6, str6, dt, nsys, sc, crss, nsr, gam0, gamdot};
-    const double erroral = 1E-10;
+    const double erroral = g_nr_tol;
     int iterl = 0;
     double erral = 10 * erroral;
     while ((iterl < 500) && (fabs(erral) > fabs(erroral))) {
@@ -495,7 +505,7 @@ int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6], do
             step[i] = 0.0;
             for (int j = 0; j < 6; j++) step[i] -= jinv[i][j] * res[j];
         }
-        const double stpmax = 10.0, ALF = 1.E-4, TOLX = 1.E-7;
+        const double stpmax = 10.0, ALF = 1.E-4, TOLX = g_tolx;
         double sum = 0.0, slope = 0.0, test = 0.0;
         for (int i = 0; i < 6; i++) sum += step[i] * step[i];
         sum = sqrt(sum);

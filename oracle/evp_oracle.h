@@ -68,6 +68,11 @@ int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6],
                             double edot_out[3][3], double gamdot[],
                             double *err_s, double *err_e, int *n_iter);
 
+/* Newton tolerances of orc_update_stress_voxel, process-wide: relative step
+ * (reference 1e-10) and line-search TOLX (reference 1e-7).  Validation runs
+ * tighten them so that the loop output is smooth in the material parameters. */
+void orc_set_newton_tol(double tol, double tolx);
+
 /* ---- FFT: unnormalised 3-D complex DFT, sign -1 forward / +1 backward ---- */
 void orc_fft3d(double *data /* interleaved re,im; [nx][ny][nz] */, int nx, int ny,
                int nz, int sign);

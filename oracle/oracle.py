@@ -59,6 +59,7 @@ def lib():
         L.orc_job_newton_iters.restype = C.c_double
         L.orc_job_newton_iters.argtypes = [C.c_void_p]
         L.orc_fft3d.argtypes = [DP, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_set_newton_tol.argtypes = [C.c_double, C.c_double]
         L.orc_lu_inv66.argtypes = [DP]
         L.orc_vonmises.restype = C.c_double
         L.orc_vonmises.argtypes = [DP]
@@ -278,6 +279,11 @@ def orientation(tran, rot):
     rot = np.array(rot, dtype=np.float64, order="C")
     lib().orc_orientation(ptr(tran), ptr(rot))
     return tran, rot
+
+
+def set_newton_tol(tol: float = 1e-10, tolx: float = 1e-7):
+    """Newton tolerances of every later oracle call (process-wide)."""
+    lib().orc_set_newton_tol(tol, tolx)
 
 
 def threads() -> int:

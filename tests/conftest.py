@@ -10,6 +10,7 @@ for p in (os.path.join(ROOT, "m-themes_amd"), os.path.join(ROOT, "oracle"), os.p
         sys.path.insert(0, p)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
+PKG = os.path.join(ROOT, "m-themes_amd")
 
 
 def pytest_configure(config):

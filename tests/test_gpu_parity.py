@@ -202,15 +202,30 @@ def test_loop_parity_16_pores(job_factory):
 
 
 def test_mixed_precision_stress_error(job_factory):
+    """FP32 FFT/Green with FP64 Newton, anchors, projection and FP64 sweeps
+    against the FP64 loop.  At equal iteration counts (the parity protocol
+    for convergence tests that can flip) the difference is the precision
+    error alone: max |sig_mix - sig_64| / max |sig_64| <= 1e-6.  Converging
+    on their own, the two runs may stop one iteration apart, which moves the
+    stress by about the loop tolerance (1e-6 here) itself."""
     inp = job_factory("mixed16", n=16, grains=8, seed=13, increments=3)
     a = lib.GpuJob(inp, lib.FP64)
+    it, _ = a.run(3)
     b = lib.GpuJob(inp, lib.MIXED)
-    a.run(3)
-    it, stats = b.run(3)
-    assert all(s["fp64_iters"] >= 1 for s in stats)
+    for k in range(3):
+        _, st = b.run(1, forced=int(it[k]))
+        assert st[0]["fp64_iters"] >= 1
     sa, sb = a.state(), b.state()
-    assert rel(sb["Sig"], sa["Sig"]) < 1e-6
-    assert rel(b.macro()["sig_avg"], a.macro()["sig_avg"]) < 1e-6
+    for f in ("Sig", "Edot", "DisGrad"):
+        err = np.abs(sb[f] - sa[f]).max() / np.abs(sa[f]).max()
+        print(f, err)
+        assert err < 1e-6, (f, err)
+    assert rel(b.macro()["sig_avg"], a.macro()["sig_avg"]) < 1e-8
+    c = lib.GpuJob(inp, lib.MIXED)
+    itc, stats = c.run(3)
+    assert all(s["fp64_iters"] >= 1 for s in stats)
+    assert np.abs(np.asarray(itc) - np.asarray(it)).max() <= 2
+    assert rel(c.macro()["sig_avg"], a.macro()["sig_avg"]) < 1e-6
 
 
 # ------------------------------------------------------ golden fixtures ----
