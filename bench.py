@@ -157,6 +157,12 @@ def main():
     ap.add_argument("--adj-n", type=int, default=64, help="grid edge of the adjoint workload")
     ap.add_argument("--adj-increments", type=int, default=20, help="0: skip the adjoint")
     ap.add_argument("--adj-checkpoint", type=int, default=5)
+    ap.add_argument("--profile-iters", type=int, default=0,
+                    help="profiling mode: one increment forced to this many Green iterations "
+                         "per precision, then exit (for rocprofv3 counter runs)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="profiling mode: also run one plain device copy of this size (the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs the whole grid "
@@ -173,6 +179,26 @@ def main():
     log(f"bench: wrote {cfg.n}^3 job ({cfg.grains} grains, {cfg.pore_fraction:.0%} pores) "
         f"in {time.perf_counter() - t:.1f} s")
     N = cfg.n ** 3
+    if args.profile_iters > 0:
+        for name, prec in (("fp64", lib.FP64), ("mixed", lib.MIXED)):
+            if args.precision not in ("both", name):
+                continue
+            job = lib.GpuJob(inp, prec)
+            t0 = time.perf_counter()
+            job.run(1, forced=args.profile_iters)
+            job.synchronize()
+            log(f"bench: profile {name}: {args.profile_iters} iterations in {time.perf_counter() - t0:.2f} s")
+            job.close()
+        if args.copy_probe_mib > 0:
+            import torch
+            n = args.copy_probe_mib * (1 << 20) // 8
+            a = torch.ones(n, dtype=torch.float64, device="cuda")
+            b = torch.empty_like(a)
+            b.copy_(a)
+            torch.cuda.synchronize()
+            log(f"bench: copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        print(json.dumps({"profile_iters": args.profile_iters, "grid": cfg.n}), flush=True)
+        return
     res = {}
     for name, prec in (("fp64", lib.FP64), ("mixed", lib.MIXED)):
         if args.precision not in ("both", name):

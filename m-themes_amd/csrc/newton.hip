@@ -208,31 +208,42 @@ __device__ __forceinline__ bool lu_solve6(double a[6][6], double b[6])
     return true;
 }
 
+// Per-voxel constants of the Newton solve.  The residual of UpdateStress,
+//   R(s) = s - lambda + C0 (eps_p + dt edot(s) + S s - E),
+// is evaluated as R(s) = A s + dt C0 edot(s) + b with A = I + C0 S (a
+// per-orientation table) and b = -lambda + C0 (eps_p - E) (once per voxel);
+// its Jacobian I + C0 (S + dt D) as A + dt C0 D.  Same mathematics as the
+// reference, fewer operations (the 6x6x6 product C0 (S + dt D) becomes the
+// 6x5x5 product C0 D, since D is zero outside its 5x5 block).
 struct Vox {
-    double sg[6][6];
-    double xl[6], eps[6], str[6];
+    double A[6][6];
+    double b[6], xl[6];
     double signorm; // Frobenius norm of lambda (src/evolution.c:92-96)
 };
+
+__device__ __forceinline__ void residual(const NewtonParams &P, const Vox &V, const double sig[6],
+                                         const double ed[5], double res[6])
+{
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double r = V.b[i];
+#pragma unroll
+        for (int j = 0; j < 6; j++) r += V.A[i][j] * sig[j];
+        double c = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) c += P.c066[i * 6 + k] * ed[k];
+        res[i] = r + P.dt * c;
+    }
+}
 
 template <bool INTN>
 __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const RateIn &in,
                                              const Vox &V, const double sig[6],
                                              double gd[EVPGPU_NSYSMX])
 {
-    double ed[5], dd[5][5], tot[6], res[6];
+    double ed[5], dd[5][5], res[6];
     srate<false, INTN>(P, p, in, sig, ed, dd, gd);
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        tot[i] = V.eps[i] + (i < 5 ? ed[i] : 0.0) * P.dt;
-#pragma unroll
-        for (int j = 0; j < 6; j++) tot[i] += V.sg[i][j] * sig[j];
-    }
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        res[i] = sig[i] - V.xl[i];
-#pragma unroll
-        for (int j = 0; j < 6; j++) res[i] += P.c066[i * 6 + j] * (tot[j] - V.str[j]);
-    }
+    residual(P, V, sig, ed, res);
     double f = 0.0;
 #pragma unroll
     for (int i = 0; i < 6; i++) f += res[i] * res[i];
@@ -246,7 +257,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
                             double &conv_s, double &conv_e, int &iterl, int &trials)
 {
     trials = 0;
-    double ed[5], dd[5][5], tot[6], res[6], old[6];
+    double ed[5], dd[5][5], res[6], old[6];
     conv_s = 0.0;
     conv_e = 0.0;
     iterl = 0;
@@ -260,18 +271,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
         srate<true, INTN>(P, p, in, sig6, ed, dd, gd);
 #pragma unroll
         for (int i = 0; i < 6; i++) edotp[i] = i < 5 ? ed[i] : 0.0;
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            tot[i] = V.eps[i] + edotp[i] * P.dt;
-#pragma unroll
-            for (int j = 0; j < 6; j++) tot[i] += V.sg[i][j] * sig6[j];
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            res[i] = sig6[i] - V.xl[i];
-#pragma unroll
-            for (int j = 0; j < 6; j++) res[i] += P.c066[i * 6 + j] * (tot[j] - V.str[j]);
-        }
+        residual(P, V, sig6, ed, res);
         double gradf[6], step[6];
         {
             double jac[6][6];
@@ -279,11 +279,12 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             for (int i = 0; i < 6; i++)
 #pragma unroll
                 for (int j = 0; j < 6; j++) {
-                    double acc = (double)(i == j);
+                    double acc = V.A[i][j];
+                    if (j < 5) {
+                        double c = 0.0;
 #pragma unroll
-                    for (int k = 0; k < 6; k++) {
-                        double m = (k < 5 && j < 5) ? (V.sg[k][j] + dd[k][j] * P.dt) : V.sg[k][j];
-                        acc += P.c066[i * 6 + k] * m;
+                        for (int k = 0; k < 5; k++) c += P.c066[i * 6 + k] * dd[k][j];
+                        acc += P.dt * c;
                     }
                     jac[i][j] = acc;
                 }
@@ -357,15 +358,28 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             f2 = f_new; // the backtrack value the cubic step needs (Q18)
             alam = tmplam > 0.1 * alam ? tmplam : 0.1 * alam;
         }
-        double conv_nr = 0.0, ce = 0.0, cs = 0.0;
+        double conv_nr = 0.0, cs = 0.0;
 #pragma unroll
         for (int i = 0; i < 6; i++) {
             conv_nr += (sig6[i] - old[i]) * (sig6[i] - old[i]);
-            ce += (tot[i] - V.str[i]) * (tot[i] - V.str[i]);
             cs += (sig6[i] - V.xl[i]) * (sig6[i] - V.xl[i]);
         }
         erral = sqrt(conv_nr) / V.signorm;
         conv_s = sqrt(cs);
+    }
+    // strain mismatch eps_p + dt edot + S s - E at the start of the last
+    // iteration: C0^-1 (R - (s - lambda)) with C0^-1 = S0
+    {
+        double d[6], ce = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; i++) d[i] = res[i] - (old[i] - V.xl[i]);
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double t = 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; j++) t += P.s066[i * 6 + j] * d[j];
+            ce += t * t;
+        }
         conv_e = sqrt(ce);
     }
     return 0;
@@ -384,7 +398,7 @@ __device__ __forceinline__ double wave_sum(double v)
 template <bool INTN, bool CSYM>
 __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_newton(
     NewtonParams P, long long N, const int *__restrict__ phase, const int *__restrict__ orient,
-    const double *__restrict__ sgr, double *__restrict__ sig, const double *__restrict__ E,
+    const double *__restrict__ agr, double *__restrict__ sig, const double *__restrict__ E,
     const double *__restrict__ epsp, double *__restrict__ edot, const double *__restrict__ tran,
     const double *__restrict__ crss, double *__restrict__ gamdot, double *__restrict__ partials,
     int *__restrict__ status, double *__restrict__ err_vox, int *__restrict__ niter_vox)
@@ -407,17 +421,27 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             V.signorm = sqrt(t[0] * t[0] + t[5] * t[5] + t[4] * t[4] + t[5] * t[5] + t[1] * t[1] +
                              t[3] * t[3] + t[4] * t[4] + t[3] * t[3] + t[2] * t[2]);
             sym_to_v6(t, V.xl);
+            {
+                double e6[6], s6[6];
 #pragma unroll
-            for (int c = 0; c < 6; c++) t[c] = epsp[c * N + v];
-            sym_to_v6(t, V.eps);
+                for (int c = 0; c < 6; c++) t[c] = epsp[c * N + v];
+                sym_to_v6(t, e6);
 #pragma unroll
-            for (int c = 0; c < 6; c++) t[c] = E[c * N + v];
-            sym_to_v6(t, V.str);
-            const double *sgv = sgr + 36LL * orient[v];
+                for (int c = 0; c < 6; c++) t[c] = E[c * N + v];
+                sym_to_v6(t, s6);
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 6; j++) a += P.c066[i * 6 + j] * (e6[j] - s6[j]);
+                    V.b[i] = a - V.xl[i];
+                }
+            }
+            const double *av = agr + 36LL * orient[v];
 #pragma unroll
             for (int i = 0; i < 6; i++)
 #pragma unroll
-                for (int j = 0; j < 6; j++) V.sg[i][j] = sgv[i * 6 + j];
+                for (int j = 0; j < 6; j++) V.A[i][j] = av[i * 6 + j];
             const int ns = P.nsys[p];
             // Schmid tensors in sample axes: sym((T n) x (T b)) in the 5-D basis
             // (update_schmid, src/kinematics.c:430-463), into this lane's LDS column
@@ -535,11 +559,29 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ 
     }
 }
 
+// A = I + C0 S per orientation, in the reference's summation order of
+// I + C0 (S + dt D) (src/evolution.c:134-146)
+__global__ void k_agr(const double *__restrict__ sgr, int n_orient, NewtonParams P, double *__restrict__ agr)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 36 * n_orient) return;
+    const int o = t / 36, i = (t % 36) / 6, j = t % 6;
+    double acc = (double)(i == j);
+    for (int k = 0; k < 6; k++) acc += P.c066[i * 6 + k] * sgr[36LL * o + k * 6 + j];
+    agr[t] = acc;
+}
+
 } // namespace
+
+int agr_build(evpgpu_ctx *c, const NewtonParams &P, const double *sgr, int n_orient, double *agr)
+{
+    hipLaunchKernelGGL(k_agr, dim3((36 * n_orient + 255) / 256), dim3(256), 0, c->stream, sgr, n_orient, P, agr);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
 
 int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const double *E,
                   const double *epsp, double *edot, const double *tran, const double *crss,
-                  double *gamdot, const int *phase, const int *orient, const double *sgr,
+                  double *gamdot, const int *phase, const int *orient, const double *agr,
                   long long n, double *partials, int *nblk_out, int *status, double *err_vox,
                   int *niter_vox)
 {
@@ -548,7 +590,7 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const doubl
     bool intn = true;
     for (int q = 0; q < 2; q++)
         if (!P.is_gas[q] && P.int_n[q] == 0) intn = false;
-#define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig, E, \
+#define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, agr, sig, E, \
                     epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
     if (intn && P.crss_sym) hipLaunchKernelGGL((k_newton<true, true>), NEWTON_ARGS);
     else if (intn) hipLaunchKernelGGL((k_newton<true, false>), NEWTON_ARGS);

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIBDIR = os.path.join(PKG, "lib")
+# the in-tree build; EVP_LIB_DIR points the loader at another build of the
+# same sources (for example one made with different compile options)
+LIBDIR = os.environ.get("EVP_LIB_DIR") or os.path.join(PKG, "lib")
 REPO = os.path.dirname(PKG)
 
 FP64, MIXED = 0, 1
