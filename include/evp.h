@@ -78,6 +78,26 @@ int evp_job_vm_row(const evp_job *job, double row[7]);
 int evp_job_set_params(evp_job *job, int phase, const double params[5]);
 int evp_job_get_params(const evp_job *job, int phase, double params[5]);
 
+/* ---- on-disk dumps (src/io.c:283-541) ----
+ * .iout fields: six files <prefix>_<c>_S<step:%06d>.iout (c = 1..6 for
+ * 11 22 33 23 13 12), each N native doubles in voxel order.  f6 is 6 x N SoA.
+ * Texture: <prefix>_S<step:%06d>.iout, N 48-byte records {double Bunge
+ * angles in degrees[3]; int coord[3] (1-based x, y, z); int grain, phase,
+ * buffer (0)}, the layout src/tex2out.c reads.                              */
+enum { EVP_FIELD_SIG = 0, EVP_FIELD_EPS, EVP_FIELD_EDOT, EVP_FIELD_ELS, EVP_FIELD_DISGRAD };
+int evp_write_iout6(const char *prefix, int step, long long n, const double *f6);
+int evp_write_texture(const char *prefix, int step, const int dims[3], const double *tran9,
+                      const int *grain, const int *phase);
+/* text listing of a texture file, one line per record (src/tex2out.c:56-64) */
+int evp_texture_to_text(const char *in_path, const char *out_path);
+/* EulerToTransMatrix option 1 (src/kinematics.c:187-217): row-major matrix ->
+ * Bunge angles in degrees (with the reference's PI = 3.14159265359)        */
+void evp_mat_to_euler(const double a9[9], double angles[3]);
+/* dump one field of the job's current state (WriteSigMPI ... WriteDisgradMPI) */
+int evp_job_write_field(evp_job *job, int which, const char *prefix, int step);
+/* dump the texture of the job's current state (WriteTextureMPI) */
+int evp_job_write_texture(evp_job *job, const char *prefix, int step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -378,6 +378,49 @@ int evp_job_get_params(const evp_job *J, int phase, double q[5])
     return 0;
 }
 
+int evp_job_write_field(evp_job *J, int which, const char *prefix, int step)
+{
+    if (!J || !prefix || which < EVP_FIELD_SIG || which > EVP_FIELD_DISGRAD) return EVPGPU_ERR_ARG;
+    const long long N = J->N;
+    double *sig = malloc(sizeof(double) * (6 + 9 + 6 + 6 + 6) * N);
+    if (!sig) return EVPGPU_ERR_NOMEM;
+    double *dg = sig + 6 * N, *eps = dg + 9 * N, *edot = eps + 6 * N, *out = edot + 6 * N;
+    int st = evpgpu_download_state(J->ctx, sig, dg, eps, edot, NULL, NULL, NULL, NULL);
+    if (!st) {
+        /* DisGrad components as the writers pick them: [i][i], [1][2], [0][2], [0][1] */
+        const int ij[6] = {0, 4, 8, 5, 2, 1}, ji[6] = {0, 4, 8, 7, 6, 3};
+        for (int c = 0; c < 6; c++)
+            for (long long v = 0; v < N; v++) {
+                double x;
+                switch (which) {
+                case EVP_FIELD_SIG: x = sig[c * N + v]; break;
+                case EVP_FIELD_EPS: x = eps[c * N + v]; break;
+                case EVP_FIELD_EDOT: x = edot[c * N + v]; break;
+                case EVP_FIELD_ELS:
+                    x = (dg[ij[c] * N + v] + dg[ji[c] * N + v]) / 2.0 - eps[c * N + v];
+                    if (c < 3) x = dg[ij[c] * N + v] - eps[c * N + v];
+                    break;
+                default: x = dg[ij[c] * N + v]; break;
+                }
+                out[c * N + v] = x;
+            }
+        st = evp_write_iout6(prefix, step, N, out);
+    }
+    free(sig);
+    return st;
+}
+
+int evp_job_write_texture(evp_job *J, const char *prefix, int step)
+{
+    if (!J || !prefix) return EVPGPU_ERR_ARG;
+    double *tran = malloc(sizeof(double) * 9 * J->N);
+    if (!tran) return EVPGPU_ERR_NOMEM;
+    int st = evpgpu_download_state(J->ctx, NULL, NULL, NULL, NULL, tran, NULL, NULL, NULL);
+    if (!st) st = evp_write_texture(prefix, step, J->nl.CellDim, tran, J->grain, J->phase);
+    free(tran);
+    return st;
+}
+
 int evp_job_run(evp_job *J, int n, int forced, int *iters, evpgpu_stats *stats)
 {
     if (!J) return EVPGPU_ERR_ARG;
@@ -543,6 +586,16 @@ void Evolution(void)
         }
         double row[7] = {m.time_tot, m.e_vm, m.evmp, m.d_vm, m.dvmp, m.s_vm, m.s_vm1};
         memcpy(G->vm_row, row, sizeof(row));
+        /* field dumps of the live loop (src/evolution.c:1633-1648): the
+         * elastic strain and the stress every 500 increments, under the
+         * file names the reference uses there */
+        if (G->nl.PrintControl[0] == 1 && istep % 500 == 0) {
+            if ((st = evp_job_write_field(G, EVP_FIELD_ELS, "els", istep)) ||
+                (st = evp_job_write_field(G, EVP_FIELD_SIG, "sig_afterDRX", istep))) {
+                fail_with(st, "field dump failed");
+                return;
+            }
+        }
     }
     printf("-------------Simulation ends----------------\n");
     printf("======================================\n\n");

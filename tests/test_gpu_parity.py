@@ -290,3 +290,65 @@ def test_forge_dropin_writes_vm_out(tmp_path):
     assert vals.shape == (2, 7)
     # vm.out carries %e (7 significant digits)
     assert np.allclose(vals, g["vm"], rtol=1e-6, atol=0)
+
+
+def test_forge_dumps_fields_on_the_reference_schedule(tmp_path):
+    """PrintControl 1 in test_run/input.in: after increment 0 (0 % 500 == 0)
+    the live loop dumps the elastic strain and the stress
+    (src/evolution.c:1633-1648) in the .iout layout of src/io.c:283-476."""
+    import os
+    import shutil
+    import subprocess
+    from conftest import GOLDEN
+    for f in ("input.in", "Cu_p.sx", "FFT_microstructure_pore_197.txt"):
+        shutil.copy(os.path.join(GOLDEN, "test_run", f), tmp_path / f)
+    r = subprocess.run([os.path.join(lib.LIBDIR, "forge"), "input.in"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    n = 32 ** 3
+    for pre in ("els", "sig_afterDRX"):
+        for c in range(1, 7):
+            f = tmp_path / f"{pre}_{c}_S000000.iout"
+            assert f.stat().st_size == 8 * n, f
+        assert not (tmp_path / f"{pre}_1_S000001.iout").exists()
+    # the stress dump after increment 0 equals the state then
+    job = lib.GpuJob(str(tmp_path / "input.in"))
+    job.run(1)
+    s = job.state()["Sig"]
+    ij = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    for c, (i, j) in enumerate(ij):
+        d = np.fromfile(tmp_path / f"sig_afterDRX_{c + 1}_S000000.iout", dtype="<f8")
+        assert np.array_equal(d, s[:, i, j])
+
+
+def test_job_field_and_texture_dumps(job_factory, tmp_path):
+    import ctypes as C
+    inp = job_factory("dump8", n=8, grains=4, seed=21, increments=3)
+    job = lib.GpuJob(inp)
+    job.run(3)
+    st = job.state()
+    L = lib.evp()
+    L.evp_job_write_field.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+    L.evp_job_write_texture.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    ij = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    dg = st["DisGrad"]
+    els = 0.5 * (dg + dg.transpose(0, 2, 1)) - st["Eps"]
+    expect = {0: st["Sig"], 1: st["Eps"], 2: st["Edot"], 3: els, 4: dg}
+    for which, ref in expect.items():
+        pre = str(tmp_path / f"f{which}").encode()
+        assert L.evp_job_write_field(job.h, which, pre, 12) == 0
+        for c, (i, j) in enumerate(ij):
+            d = np.fromfile(tmp_path / f"f{which}_{c + 1}_S000012.iout", dtype="<f8")
+            assert np.abs(d - ref[:, i, j]).max() <= 1e-15 * max(np.abs(ref).max(), 1.0), (which, c)
+    assert L.evp_job_write_texture(job.h, str(tmp_path / "tex").encode(), 3) == 0
+    tex = np.dtype([("angle", "<f8", (3,)), ("coord", "<i4", (3,)), ("grain", "<i4"),
+                    ("phase", "<i4"), ("buffer", "<i4")])
+    rec = np.fromfile(tmp_path / "tex_S000003.iout", dtype=tex)
+    g, p = job.ids()
+    assert np.array_equal(rec["grain"], g) and np.array_equal(rec["phase"], p)
+    # the angles rebuild each voxel's rotation (away from the poles)
+    import oracle as O
+    for v in range(0, job.n, 37):
+        a = O.euler_to_mat(*(rec["angle"][v] * 3.14159265359 / 180.0))
+        if abs(a[2, 2]) < 0.999:
+            assert np.abs(a.T - st["TranMat"][v]).max() < 1e-9
