@@ -190,13 +190,12 @@ def main():
             log(f"bench: profile {name}: {args.profile_iters} iterations in {time.perf_counter() - t0:.2f} s")
             job.close()
         if args.copy_probe_mib > 0:
-            import torch
+            job = lib.GpuJob(inp, lib.FP64)
             n = args.copy_probe_mib * (1 << 20) // 8
-            a = torch.ones(n, dtype=torch.float64, device="cuda")
-            b = torch.empty_like(a)
-            b.copy_(a)
-            torch.cuda.synchronize()
-            log(f"bench: copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+            gbs = job.copy_probe(n, 1)
+            log(f"bench: copy probe: k_copy of {args.copy_probe_mib} MiB read + "
+                f"{args.copy_probe_mib} MiB written, {gbs:.0f} GB/s")
+            job.close()
         print(json.dumps({"profile_iters": args.profile_iters, "grid": cfg.n}), flush=True)
         return
     res = {}

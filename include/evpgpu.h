@@ -163,6 +163,12 @@ int evpgpu_newton_host(evpgpu_ctx *ctx, int n, const int *phase, const int *orie
                        const double *eps6, const double *tranmat9, const double *crss24,
                        double *edot6, double *gamdot12, double *err2, int *niter);
 
+/* Diagnostic: copy n doubles (n even) device to device reps times with the
+ * library's streaming copy kernel (HBM traffic 16 n bytes per launch) and
+ * report the achieved GB/s.  No counterpart in the reference; used to
+ * measure achievable HBM bandwidth and to scale hardware traffic counters. */
+int evpgpu_copy_probe(evpgpu_ctx *ctx, long long n, int reps, double *gbs);
+
 /* ------------------------------------------------------------ adjoint --
  * Reverse-mode derivative of the macroscopic stress curve with respect to
  * the hardening and rate parameters of each phase, for calibration.  No

@@ -787,6 +787,30 @@ int evpgpu_increment(evpgpu_ctx *c, int forced, evpgpu_stats *stats)
 }
 
 // ------------------------------------------------------ building blocks --
+int evpgpu_copy_probe(evpgpu_ctx *c, long long n, int reps, double *gbs)
+{
+    if (!c || n < 2 || (n & 1) || reps < 1 || !gbs) return EVPGPU_ERR_ARG;
+    double *a = nullptr, *b = nullptr;
+    int st = dalloc(&a, n);
+    if (!st) st = dalloc(&b, n);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = EVPGPU_ERR_HIP;
+    if (!st) st = fill_zero(c, a, n);
+    if (!st) st = copy_launch(c, a, b, n); // warm
+    if (!st && hipEventRecord(e0, c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+    for (int r = 0; r < reps && !st; r++) st = copy_launch(c, a, b, n);
+    if (!st && hipEventRecord(e1, c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st && hipEventSynchronize(e1) != hipSuccess) st = EVPGPU_ERR_HIP;
+    float ms = 0.f;
+    if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st) *gbs = 16.0 * (double)n * reps / (1e-3 * ms) / 1e9;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return st;
+}
+
 int evpgpu_fft_r2c_host(evpgpu_ctx *c, const double *in, double *out)
 {
     if (!c || !in || !out) return EVPGPU_ERR_ARG;

@@ -178,3 +178,4 @@ int epilogue_launch(evpgpu_ctx *c, const EpilogueParams &P, double *partials, in
 int fill_zero(evpgpu_ctx *c, double *p, long long n);
 int to_f32_launch(evpgpu_ctx *c, const double *src, float *dst, long long n);
 int sub_launch(evpgpu_ctx *c, double *a, const double *b, long long n);
+int copy_launch(evpgpu_ctx *c, const double *src, double *dst, long long n);

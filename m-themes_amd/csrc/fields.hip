@@ -26,6 +26,16 @@ __global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst
     if (i < n) dst[i] = (float)src[i];
 }
 
+// plain streaming copy (16-byte accesses, grid-stride), the reference kernel
+// for achievable HBM bandwidth and for scaling the traffic counters
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                              long long n2)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_sub(double *__restrict__ a, const double *__restrict__ b, long long n)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -223,6 +233,13 @@ int fill_zero(evpgpu_ctx *c, double *p, long long n)
 int to_f32_launch(evpgpu_ctx *c, const double *src, float *dst, long long n)
 {
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, src, dst, n);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+int copy_launch(evpgpu_ctx *c, const double *src, double *dst, long long n)
+{
+    const long long n2 = n / 2;
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, c->stream, (const double2 *)src, (double2 *)dst, n2);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 

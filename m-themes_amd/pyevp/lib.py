@@ -79,14 +79,16 @@ EVPGPU_SYMBOLS = [
     "evpgpu_set_phases", "evpgpu_set_loading", "evpgpu_upload_microstructure",
     "evpgpu_upload_state", "evpgpu_download_state", "evpgpu_set_macro", "evpgpu_get_macro",
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
-    "evpgpu_newton_host", "evpgpu_adjoint_record", "evpgpu_adjoint_run",
+    "evpgpu_newton_host", "evpgpu_adjoint_record", "evpgpu_adjoint_run", "evpgpu_copy_probe",
 ]
 EVP_SYMBOLS = [
     "GetNameList", "PrintNameList", "OpenFiles", "CloseFiles", "SetupJob", "Evolution",
     "DestroyJob", "evp_set_abort_on_error", "evp_status", "evp_set_precision", "evp_job_open",
     "evp_job_close", "evp_job_ctx", "evp_job_dims", "evp_job_nsteps", "evp_job_run",
     "evp_job_macro", "evp_job_ids", "evp_job_grain_list", "evp_job_setup_info", "evp_job_reset",
-    "evp_job_vm_row", "evp_job_set_params", "evp_job_get_params",
+    "evp_job_vm_row", "evp_job_set_params", "evp_job_get_params", "evp_write_iout6",
+    "evp_write_texture", "evp_texture_to_text", "evp_mat_to_euler", "evp_job_write_field",
+    "evp_job_write_texture",
 ]
 
 _gpu = None
@@ -137,6 +139,7 @@ def gpu():
         L.evpgpu_green_host.argtypes = [VP, DP, DP]
         L.evpgpu_newton_host.argtypes = [VP, C.c_int, IP, IP, DP, C.c_int, DP, DP, DP, DP, DP,
                                          DP, DP, DP, IP]
+        L.evpgpu_copy_probe.argtypes = [VP, C.c_longlong, C.c_int, DP]
         L.evpgpu_adjoint_record.argtypes = [VP, C.c_int, C.c_int, DP, C.POINTER(Stats)]
         L.evpgpu_adjoint_run.argtypes = [VP, DP, C.c_double, C.c_int, DP, C.POINTER(AdjointStats)]
         _gpu = L
@@ -232,6 +235,12 @@ class GpuJob:
 
     def set_precision(self, precision: int):
         _chk(gpu().evpgpu_set_precision(self.ctx, precision), "evpgpu_set_precision")
+
+    def copy_probe(self, n: int, reps: int = 1) -> float:
+        """GB/s of the library's streaming copy of n doubles (16 n bytes per launch)."""
+        g = C.c_double(0)
+        _chk(gpu().evpgpu_copy_probe(self.ctx, n, reps, C.byref(g)), "evpgpu_copy_probe")
+        return g.value
 
     def set_newton_tolerances(self, tol: float = 1e-10, tolx: float = 1e-7):
         _chk(gpu().evpgpu_set_newton_tolerances(self.ctx, tol, tolx), "evpgpu_set_newton_tolerances")
