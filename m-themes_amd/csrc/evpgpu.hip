@@ -235,7 +235,7 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
     int nblk = 0;
     tmark(c, 0, 0);
     st = newton_launch(c, P, c->sig, c->E, c->epsp, c->edot, c->tran, c->crss, c->gamdot, c->phase,
-                       c->orient, c->agr, c->N, c->partials, &nblk, c->d_status, nullptr, nullptr);
+                       c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr, nullptr);
     if (st) return st;
     tmark(c, 0, 1);
     st = reduce_partials(c, c->partials, nblk, NRED, c->red);
@@ -363,7 +363,7 @@ int evpgpu_destroy(evpgpu_ctx *c)
     if (!c) return EVPGPU_OK;
     adjoint_free(c);
     void *ptrs[] = {c->sig, c->E, c->W, c->epsp, c->edot, c->tran, c->crss, c->gamacum, c->gamdot,
-                    c->phase, c->grain, c->orient, c->sgr, c->agr, c->spec, c->acc, c->rwork,
+                    c->phase, c->grain, c->orient, c->sgr, c->spec, c->acc, c->rwork,
                     c->sigA, c->cA, c->skA, c->Estart, c->partials, c->red, c->red2, c->d_status, c->tw64[0], c->tw64[1], c->tw64[2],
                     c->tw32[0], c->tw32[1], c->tw32[2]};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
@@ -474,10 +474,7 @@ int evpgpu_upload_microstructure(evpgpu_ctx *c, const int *grain, const int *pha
     int *o2 = (int *)malloc(sizeof(int) * c->N);
     for (long long v = 0; v < c->N; v++) o2[v] = orient[v] < 0 ? 0 : orient[v];
     if (c->sgr) (void)hipFree(c->sgr);
-    if (c->agr) (void)hipFree(c->agr);
-    c->agr = nullptr;
     int st = dalloc(&c->sgr, 36LL * n_orient);
-    if (!st) st = dalloc(&c->agr, 36LL * n_orient);
     if (!st && hipMemcpy(c->sgr, sgr, sizeof(double) * 36 * n_orient, hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
     if (!st && hipMemcpy(c->grain, grain, sizeof(int) * c->N, hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
     if (!st && hipMemcpy(c->phase, phase, sizeof(int) * c->N, hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
@@ -621,11 +618,6 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     // kernel, from the orientations the previous epilogue left
     st = fill_zero(c, (double *)c->acc, 6 * c->NK);
     if (st) return st;
-    {
-        NewtonParams P = newton_params(c);
-        st = agr_build(c, P, c->sgr, c->n_orient, c->agr);
-        if (st) return st;
-    }
     tmark(c, 4, 1);
     EVP_CHECK(hipStreamSynchronize(c->stream));
     {
@@ -873,7 +865,7 @@ int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient
     for (int v = 0; v < n; v++)
         if (orient[v] < 0 || orient[v] >= n_orient || phase[v] < 1 || phase[v] > c->nphases) return EVPGPU_ERR_ARG;
     int *dph = nullptr, *dor = nullptr, *dni = nullptr, *dst = nullptr;
-    double *dsg = nullptr, *dag = nullptr, *dsig = nullptr, *dstr = nullptr, *deps = nullptr, *dsc = nullptr,
+    double *dsg = nullptr, *dsig = nullptr, *dstr = nullptr, *deps = nullptr, *dsc = nullptr,
            *dcr = nullptr, *ded = nullptr, *dgd = nullptr, *der = nullptr, *dpart = nullptr;
     int st = EVPGPU_OK;
     if (!st) st = dalloc(&dph, n);
@@ -881,7 +873,6 @@ int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient
     if (!st) st = dalloc(&dni, n);
     if (!st) st = dalloc(&dst, 1);
     if (!st) st = dalloc(&dsg, 36LL * n_orient);
-    if (!st) st = dalloc(&dag, 36LL * n_orient);
     if (!st) st = dalloc(&dsig, 6LL * n);
     if (!st) st = dalloc(&dstr, 6LL * n);
     if (!st) st = dalloc(&deps, 6LL * n);
@@ -908,10 +899,8 @@ int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient
         NewtonParams P = newton_params(c);
         P.crss_sym = 0;
         int nb = 0;
-        st = agr_build(c, P, dsg, n_orient, dag);
-        if (!st)
-            st = newton_launch(c, P, dsig, dstr, deps, ded, dsc, dcr, dgd, dph, dor, dag, n, dpart, &nb, dst,
-                               der, dni);
+        st = newton_launch(c, P, dsig, dstr, deps, ded, dsc, dcr, dgd, dph, dor, dsg, n, dpart, &nb, dst,
+                           der, dni);
         if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
     }
     int kst = 0;
@@ -924,7 +913,7 @@ int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient
     DN(err2, der, 2LL * n, double);
     DN(niter, dni, n, int);
 #undef DN
-    void *ptrs[] = {dph, dor, dni, dst, dsg, dag, dsig, dstr, deps, dsc, dcr, ded, dgd, der, dpart};
+    void *ptrs[] = {dph, dor, dni, dst, dsg, dsig, dstr, deps, dsc, dcr, ded, dgd, der, dpart};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
         if (ptrs[i]) (void)hipFree(ptrs[i]);
     if (!st && kst) st = kst;

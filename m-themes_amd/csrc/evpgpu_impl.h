@@ -32,7 +32,7 @@ struct NewtonParams {
     int int_n[2];    // integer rate exponent of the phase, 0 if not integral
     int crss_sym;    // 1: forward and backward CRSS are equal in every voxel
     double bn[2][EVPGPU_NSYSMX][3], bb[2][EVPGPU_NSYSMX][3]; // crystal-axis normals, directions
-    double s066[36]; // C0^-1 (B-basis), for the strain-mismatch norm
+    double s066[36]; // C0^-1 (B-basis): the Newton step is solved in S0-scaled form
     double nr_tol;   // Newton tolerance on the relative step (reference 1e-10)
     double tolx;     // line-search TOLX (reference 1e-7)
 };
@@ -96,7 +96,6 @@ struct evpgpu_ctx {
     double *gamdot;   // 12N
     int *phase, *grain, *orient;
     double *sgr;      // n_orient x 36: LU_inv_66(C_gr)
-    double *agr;      // n_orient x 36: I + C066 LU_inv_66(C_gr), rebuilt per increment
     int n_orient;
 
     // spectral buffers
@@ -166,11 +165,10 @@ int green_project(evpgpu_ctx *c);
 // newton.hip
 int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const double *E,
                   const double *epsp, double *edot, const double *tran, const double *crss,
-                  double *gamdot, const int *phase, const int *orient, const double *agr,
+                  double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status,
                   double *err_vox, int *niter_vox);
 int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out);
-int agr_build(evpgpu_ctx *c, const NewtonParams &P, const double *sgr, int n_orient, double *agr);
 // fields.hip
 int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *macro_sym6);
 int antisym_add_launch(evpgpu_ctx *c, const double *dW3);

@@ -3,25 +3,27 @@
 // lane, plus the block partial sums of the convergence norms and the stress
 // averages that get_smacro needs (src/evolution.c:1415-1437, 473-497).
 //
-// The arithmetic follows the reference's operation order (basis changes with
-// their nonzero terms in the same order, Crout LU with implicit pivoting and
-// the same 1e-5 thresholds, explicit 6x6 inverse, NR_MODIFIED backtracking
-// line search).  Differences that are exact rewrites:
-//   * the 5x5 block of d(edot)/d(sigma) is symmetric -- (a*b)*c == (b*a)*c --
-//     so only its upper triangle is summed;
-//   * NR_quality never uses d(edot)/d(sigma), so it is not formed there;
-//   * zero rows/columns of d(edot)/d(sigma) (6th component) are skipped;
-//   * LU_inv_66(C_gr) is a per-orientation constant, read from a table.
-// Differences in rounding: |x|^n uses repeated squaring for integral n, the
-// rate law multiplies by 1/CRSS instead of dividing, the Schmid tensors are
-// sym((Rn) x (Rb)) rather than R P R^T through the 3x3 form, the step is an LU
-// solve of the residual (see lu_solve6), and the compiler fuses multiply-adds.
+// The arithmetic follows the reference's: the same residual and rate law in
+// the same basis, the NR_MODIFIED backtracking line search with its
+// constants, branches and the TOLX rejection of small steps.  Rewritten, with
+// the same mathematics:
+//   * the Newton step: J step = -R with J = I + C0 (S + dt D) is solved as
+//     K step = -S0 R with the symmetric positive definite K = S0 + S + dt D
+//     (S0 = C0^-1) by Cholesky, and the merit gradient J^T R as K (C0 R) --
+//     the reference forms LU_inv_66(J) with pivoting and multiplies;
+//   * S = LU_inv_66(C_gr) is a per-orientation constant (read from a table,
+//     upper triangle), and d(edot)/d(sigma) is symmetric, so only upper
+//     triangles are formed; NR_quality never uses d(edot)/d(sigma);
+//   * |x|^n by repeated squaring for integral n, 1/CRSS multiplied instead of
+//     divided, Schmid tensors as sym((Rn) x (Rb)), fused multiply-adds.
+// A non-positive-definite K returns status 35, the reference's singular-LU
+// exit.
 //
 // Data placement: the 12x5 Schmid tensors of a voxel are derived at kernel
 // start from its rotation (72 B read instead of 480 B) and kept in a
 // wave-private LDS slab laid out [component][lane] (conflict-free), which
 // keeps them out of the register file; all other per-voxel arrays live in
-// registers with compile-time indices only (row swaps are selects).
+// registers with compile-time indices only.
 #include "evpgpu_impl.h"
 
 namespace {
@@ -125,114 +127,77 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
     }
 }
 
-// Crout LU with implicit pivoting (src/kinematics.c:282-354: same pivot
-// choice, same 1e-5 singular-row and small-pivot rules) followed by one
-// forward/back substitution for x = a^-1 b.  The reference forms the explicit
-// inverse column by column (LU_inv_66) and multiplies it into the residual;
-// its forward-substitution shortcut (src/kinematics.c:377-388) only skips
-// exact zeros of the unit columns it inverts, so the plain substitution here
-// is the same mathematics with 1/6 of the work.
-// Returns false for a row of max |a| < 1e-5 (the reference exits with 35).
-__device__ __forceinline__ bool lu_solve6(double a[6][6], double b[6])
+// Packed upper triangle of a symmetric 6x6 matrix: PK(i, j), i <= j.
+__host__ __device__ constexpr int PK(int i, int j)
 {
-    double vv[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        double big = 0.0;
-#pragma unroll
-        for (int j = 0; j < 6; j++) big = fabs(a[i][j]) > big ? fabs(a[i][j]) : big;
-        if (fabs(big) < 1E-5) return false;
-        vv[i] = 1.0 / big;
-    }
+    return i <= j ? i * 6 - i * (i - 1) / 2 + (j - i) : j * 6 - j * (j - 1) / 2 + (i - j);
+}
+
+// Cholesky factorisation of the packed symmetric 6x6 k (in place: the lower
+// factor L, stored transposed in the same packing) and the solve
+// L L^T x = b (b overwritten).  Returns false if k is not positive definite.
+__device__ __forceinline__ bool chol6_solve(double k[21], double b[6])
+{
+    double inv[6];
 #pragma unroll
     for (int j = 0; j < 6; j++) {
+        double d = k[PK(j, j)];
 #pragma unroll
-        for (int i = 0; i < j; i++) {
-            double sum = a[i][j];
+        for (int q = 0; q < j; q++) d -= k[PK(q, j)] * k[PK(q, j)];
+        if (!(d > 0.0)) return false;
+        d = sqrt(d);
+        inv[j] = 1.0 / d;
 #pragma unroll
-            for (int k = 0; k < i; k++) sum -= a[i][k] * a[k][j];
-            a[i][j] = sum;
-        }
-        double big = 0.0;
-        int imax = j;
+        for (int i = j + 1; i < 6; i++) {
+            double s = k[PK(j, i)];
 #pragma unroll
-        for (int i = j; i < 6; i++) {
-            double sum = a[i][j];
-#pragma unroll
-            for (int k = 0; k < j; k++) sum -= a[i][k] * a[k][j];
-            a[i][j] = sum;
-            double dum = vv[i] * fabs(sum);
-            if (dum >= big) {
-                big = dum;
-                imax = i;
-            }
-        }
-        // row interchange j <-> imax, applied to the right-hand side as well
-        // (equivalent to the reference's permutation during substitution)
-#pragma unroll
-        for (int r = j + 1; r < 6; r++) {
-            if (r == imax) {
-#pragma unroll
-                for (int k = 0; k < 6; k++) {
-                    double d = a[r][k];
-                    a[r][k] = a[j][k];
-                    a[j][k] = d;
-                }
-                vv[r] = vv[j];
-                double d = b[r];
-                b[r] = b[j];
-                b[j] = d;
-            }
-        }
-        if (fabs(a[j][j]) < 1E-5) a[j][j] = 1.0E-20;
-        if (j != 5) {
-            double dum = 1.0 / (a[j][j]);
-#pragma unroll
-            for (int i = j + 1; i < 6; i++) a[i][j] *= dum;
+            for (int q = 0; q < j; q++) s -= k[PK(q, i)] * k[PK(q, j)];
+            k[PK(j, i)] = s * inv[j];
         }
     }
 #pragma unroll
-    for (int i = 1; i < 6; i++) {
-        double sum = b[i];
+    for (int i = 0; i < 6; i++) {
+        double s = b[i];
 #pragma unroll
-        for (int j = 0; j < i; j++) sum -= a[i][j] * b[j];
-        b[i] = sum;
+        for (int q = 0; q < i; q++) s -= k[PK(q, i)] * b[q];
+        b[i] = s * inv[i];
     }
 #pragma unroll
     for (int i = 5; i >= 0; i--) {
-        double sum = b[i];
+        double s = b[i];
 #pragma unroll
-        for (int j = i + 1; j < 6; j++) sum -= a[i][j] * b[j];
-        b[i] = sum / a[i][i];
+        for (int q = i + 1; q < 6; q++) s -= k[PK(i, q)] * b[q];
+        b[i] = s * inv[i];
     }
     return true;
 }
 
-// Per-voxel constants of the Newton solve.  The residual of UpdateStress,
-//   R(s) = s - lambda + C0 (eps_p + dt edot(s) + S s - E),
-// is evaluated as R(s) = A s + dt C0 edot(s) + b with A = I + C0 S (a
-// per-orientation table) and b = -lambda + C0 (eps_p - E) (once per voxel);
-// its Jacobian I + C0 (S + dt D) as A + dt C0 D.  Same mathematics as the
-// reference, fewer operations (the 6x6x6 product C0 (S + dt D) becomes the
-// 6x5x5 product C0 D, since D is zero outside its 5x5 block).
+// Per-voxel constants of the Newton solve: the compliance S = LU_inv_66(C_gr)
+// (symmetric; its upper triangle), lambda and eps_p - E in the B-basis.
 struct Vox {
-    double A[6][6];
-    double b[6], xl[6];
+    double S[21];
+    double xl[6], bp[6];
     double signorm; // Frobenius norm of lambda (src/evolution.c:92-96)
 };
 
+// R(s) = s - lambda + C0 t,  t = eps_p + dt edot(s) + S s - E
+// (src/evolution.c:126-140; t is the strain mismatch of the error norm)
 __device__ __forceinline__ void residual(const NewtonParams &P, const Vox &V, const double sig[6],
-                                         const double ed[5], double res[6])
+                                         const double ed[5], double t[6], double res[6])
 {
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        double r = V.b[i];
+        double a = V.bp[i] + (i < 5 ? ed[i] * P.dt : 0.0);
 #pragma unroll
-        for (int j = 0; j < 6; j++) r += V.A[i][j] * sig[j];
-        double c = 0.0;
+        for (int j = 0; j < 6; j++) a += V.S[PK(i, j)] * sig[j];
+        t[i] = a;
+    }
 #pragma unroll
-        for (int k = 0; k < 5; k++) c += P.c066[i * 6 + k] * ed[k];
-        res[i] = r + P.dt * c;
+    for (int i = 0; i < 6; i++) {
+        double r = sig[i] - V.xl[i];
+#pragma unroll
+        for (int j = 0; j < 6; j++) r += P.c066[i * 6 + j] * t[j];
+        res[i] = r;
     }
 }
 
@@ -241,9 +206,9 @@ __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const
                                              const Vox &V, const double sig[6],
                                              double gd[EVPGPU_NSYSMX])
 {
-    double ed[5], dd[5][5], res[6];
+    double ed[5], dd[5][5], t[6], res[6];
     srate<false, INTN>(P, p, in, sig, ed, dd, gd);
-    residual(P, V, sig, ed, res);
+    residual(P, V, sig, ed, t, res);
     double f = 0.0;
 #pragma unroll
     for (int i = 0; i < 6; i++) f += res[i] * res[i];
@@ -271,32 +236,48 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
         srate<true, INTN>(P, p, in, sig6, ed, dd, gd);
 #pragma unroll
         for (int i = 0; i < 6; i++) edotp[i] = i < 5 ? ed[i] : 0.0;
-        residual(P, V, sig6, ed, res);
         double gradf[6], step[6];
         {
-            double jac[6][6];
+            double t[6];
+            residual(P, V, sig6, ed, t, res);
+            double ce = 0.0;
+#pragma unroll
+            for (int i = 0; i < 6; i++) ce += t[i] * t[i];
+            conv_e = sqrt(ce);
+        }
+        {
+            // The Newton matrix J = I + C0 (S + dt D) times S0 = C0^-1 is the
+            // symmetric positive definite K = S0 + S + dt D: the step solves
+            // K step = -S0 R (Cholesky), and the merit-function gradient
+            // J^T R = R + (S + dt D) C0 R equals K (C0 R).
+            double k[21], u[6];
 #pragma unroll
             for (int i = 0; i < 6; i++)
 #pragma unroll
-                for (int j = 0; j < 6; j++) {
-                    double acc = V.A[i][j];
-                    if (j < 5) {
-                        double c = 0.0;
-#pragma unroll
-                        for (int k = 0; k < 5; k++) c += P.c066[i * 6 + k] * dd[k][j];
-                        acc += P.dt * c;
-                    }
-                    jac[i][j] = acc;
+                for (int j = i; j < 6; j++) {
+                    double kij = P.s066[i * 6 + j] + V.S[PK(i, j)];
+                    if (i < 5 && j < 5) kij += dd[i][j] * P.dt;
+                    k[PK(i, j)] = kij;
                 }
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                double a = 0.0, b = 0.0;
+#pragma unroll
+                for (int j = 0; j < 6; j++) {
+                    a += P.c066[i * 6 + j] * res[j];
+                    b += P.s066[i * 6 + j] * res[j];
+                }
+                u[i] = a;
+                step[i] = b;
+            }
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 double g = 0.0;
 #pragma unroll
-                for (int j = 0; j < 6; j++) g += res[j] * jac[j][i];
+                for (int j = 0; j < 6; j++) g += k[PK(i, j)] * u[j];
                 gradf[i] = g;
-                step[i] = res[i];
             }
-            if (!lu_solve6(jac, step)) return EVPGPU_ERR_SINGULAR;
+            if (!chol6_solve(k, step)) return EVPGPU_ERR_SINGULAR;
 #pragma unroll
             for (int i = 0; i < 6; i++) step[i] = -step[i];
         }
@@ -367,21 +348,6 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
         erral = sqrt(conv_nr) / V.signorm;
         conv_s = sqrt(cs);
     }
-    // strain mismatch eps_p + dt edot + S s - E at the start of the last
-    // iteration: C0^-1 (R - (s - lambda)) with C0^-1 = S0
-    {
-        double d[6], ce = 0.0;
-#pragma unroll
-        for (int i = 0; i < 6; i++) d[i] = res[i] - (old[i] - V.xl[i]);
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            double t = 0.0;
-#pragma unroll
-            for (int j = 0; j < 6; j++) t += P.s066[i * 6 + j] * d[j];
-            ce += t * t;
-        }
-        conv_e = sqrt(ce);
-    }
     return 0;
 }
 
@@ -398,7 +364,7 @@ __device__ __forceinline__ double wave_sum(double v)
 template <bool INTN, bool CSYM>
 __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_newton(
     NewtonParams P, long long N, const int *__restrict__ phase, const int *__restrict__ orient,
-    const double *__restrict__ agr, double *__restrict__ sig, const double *__restrict__ E,
+    const double *__restrict__ sgr, double *__restrict__ sig, const double *__restrict__ E,
     const double *__restrict__ epsp, double *__restrict__ edot, const double *__restrict__ tran,
     const double *__restrict__ crss, double *__restrict__ gamdot, double *__restrict__ partials,
     int *__restrict__ status, double *__restrict__ err_vox, int *__restrict__ niter_vox)
@@ -430,18 +396,13 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
                 for (int c = 0; c < 6; c++) t[c] = E[c * N + v];
                 sym_to_v6(t, s6);
 #pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    double a = 0.0;
-#pragma unroll
-                    for (int j = 0; j < 6; j++) a += P.c066[i * 6 + j] * (e6[j] - s6[j]);
-                    V.b[i] = a - V.xl[i];
-                }
+                for (int i = 0; i < 6; i++) V.bp[i] = e6[i] - s6[i];
             }
-            const double *av = agr + 36LL * orient[v];
+            const double *sv = sgr + 36LL * orient[v];
 #pragma unroll
             for (int i = 0; i < 6; i++)
 #pragma unroll
-                for (int j = 0; j < 6; j++) V.A[i][j] = av[i * 6 + j];
+                for (int j = i; j < 6; j++) V.S[PK(i, j)] = sv[i * 6 + j];
             const int ns = P.nsys[p];
             // Schmid tensors in sample axes: sym((T n) x (T b)) in the 5-D basis
             // (update_schmid, src/kinematics.c:430-463), into this lane's LDS column
@@ -559,29 +520,11 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ 
     }
 }
 
-// A = I + C0 S per orientation, in the reference's summation order of
-// I + C0 (S + dt D) (src/evolution.c:134-146)
-__global__ void k_agr(const double *__restrict__ sgr, int n_orient, NewtonParams P, double *__restrict__ agr)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 36 * n_orient) return;
-    const int o = t / 36, i = (t % 36) / 6, j = t % 6;
-    double acc = (double)(i == j);
-    for (int k = 0; k < 6; k++) acc += P.c066[i * 6 + k] * sgr[36LL * o + k * 6 + j];
-    agr[t] = acc;
-}
-
 } // namespace
-
-int agr_build(evpgpu_ctx *c, const NewtonParams &P, const double *sgr, int n_orient, double *agr)
-{
-    hipLaunchKernelGGL(k_agr, dim3((36 * n_orient + 255) / 256), dim3(256), 0, c->stream, sgr, n_orient, P, agr);
-    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
-}
 
 int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const double *E,
                   const double *epsp, double *edot, const double *tran, const double *crss,
-                  double *gamdot, const int *phase, const int *orient, const double *agr,
+                  double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status, double *err_vox,
                   int *niter_vox)
 {
@@ -590,7 +533,7 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const doubl
     bool intn = true;
     for (int q = 0; q < 2; q++)
         if (!P.is_gas[q] && P.int_n[q] == 0) intn = false;
-#define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, agr, sig, E, \
+#define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig, E, \
                     epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
     if (intn && P.crss_sym) hipLaunchKernelGGL((k_newton<true, true>), NEWTON_ARGS);
     else if (intn) hipLaunchKernelGGL((k_newton<true, false>), NEWTON_ARGS);

@@ -98,8 +98,10 @@ def test_jacobian_rows_vs_oracle_fd(rec, fx, k, c):
     b[k, c] = 1.0
     g, _ = job.adjoint_run(b, cg_tol=1e-12, cg_max=4000)
     ref = fx["jac_fd"][k, c]
-    scale = np.abs(fx["jac_fd"][:, c]).max(axis=0)  # per-parameter scale of this component
-    assert np.all(np.abs(g[0] - ref) <= TOL_GRAD * np.maximum(np.abs(ref), 1e-3 * scale)), (g[0], ref)
+    # the central differences carry ~1e-10 absolute noise; entries far below
+    # the row's largest are compared against 1e-3 of that largest one
+    floor = 1e-3 * np.abs(ref).max()
+    assert np.all(np.abs(g[0] - ref) <= TOL_GRAD * np.maximum(np.abs(ref), floor)), (g[0], ref)
 
 
 def test_stress_controlled_components_do_not_move(rec, fx):
