@@ -107,6 +107,12 @@ int evpgpu_set_precision(evpgpu_ctx *ctx, int precision);
  * 1e-6 finite-difference scale (TOLX rejects Newton steps below 1e-7
  * relative), for validating derivatives. */
 int evpgpu_set_newton_tolerances(evpgpu_ctx *ctx, double tol, double tolx);
+/* Newton iterations whose step the reference's line search certainly rejects
+ * (TOLX) end after the residual: |step| <= |S0 R| / lambda_min(S0 + S) is
+ * below TOLX.  Results are identical either way (the last Newton launch of
+ * each increment is repeated in full for the rejected trial's gamdot); off
+ * runs every iteration in full.  Default on. */
+int evpgpu_set_newton_shortcut(evpgpu_ctx *ctx, int on);
 /* Mixed precision: iterations between FP64 transforms of the stress anchor
  * (default 16, at least 1).  No counterpart in the reference (FP64 only). */
 int evpgpu_set_anchor_interval(evpgpu_ctx *ctx, int iterations);

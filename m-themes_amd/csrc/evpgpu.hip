@@ -105,6 +105,59 @@ template <typename T> int dalloc(T **p, size_t n)
     return EVPGPU_OK;
 }
 
+// smallest eigenvalue of a symmetric 6x6 matrix (cyclic Jacobi rotations)
+double sym6_min_eig(double a[6][6])
+{
+    for (int sweep = 0; sweep < 50; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < 6; i++) {
+            diag += a[i][i] * a[i][i];
+            for (int j = i + 1; j < 6; j++) off += a[i][j] * a[i][j];
+        }
+        if (off <= 1e-34 * diag) break;
+        for (int p = 0; p < 6; p++)
+            for (int q = p + 1; q < 6; q++) {
+                if (a[p][q] == 0.0) continue;
+                double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < 6; k++) {
+                    double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = cs * akp - sn * akq;
+                    a[k][q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < 6; k++) {
+                    double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = cs * apk - sn * aqk;
+                    a[q][k] = sn * apk + cs * aqk;
+                }
+            }
+    }
+    double m = a[0][0];
+    for (int i = 1; i < 6; i++) m = a[i][i] < m ? a[i][i] : m;
+    return m;
+}
+
+// min over orientations of lambda_min(S0 + S), both from their upper
+// triangles as the Newton kernel reads them (bounds its step, see newton.hip)
+void update_lam_min(evpgpu_ctx *c)
+{
+    c->lam_min = -1.0;
+    if (!c->have_loading || !c->h_sgr) return;
+    double m = 1e300;
+    for (int o = 0; o < c->n_orient; o++) {
+        double a[6][6];
+        for (int i = 0; i < 6; i++)
+            for (int j = i; j < 6; j++) {
+                a[i][j] = c->ld.s066[i * 6 + j] + c->h_sgr[36LL * o + i * 6 + j];
+                a[j][i] = a[i][j];
+            }
+        double e = sym6_min_eig(a);
+        m = e < m ? e : m;
+    }
+    c->lam_min = m;
+}
+
 NewtonParams newton_params(const evpgpu_ctx *c)
 {
     NewtonParams P;
@@ -134,6 +187,7 @@ NewtonParams newton_params(const evpgpu_ctx *c)
     P.nr_tol = c->nr_tol;
     P.tolx = c->tolx;
     memcpy(P.s066, c->ld.s066, sizeof(P.s066));
+    P.kmin_tolx = c->lam_min > 0.0 ? 0.99 * c->tolx * c->lam_min : 0.0;
     return P;
 }
 
@@ -234,9 +288,15 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
     NewtonParams P = newton_params(c);
     int nblk = 0;
     tmark(c, 0, 0);
-    st = newton_launch(c, P, c->sig, c->E, c->epsp, c->edot, c->tran, c->crss, c->gamdot, c->phase,
-                       c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr, nullptr);
+    st = newton_launch(c, P, c->sig, c->sig_alt, c->E, c->epsp, c->edot, c->tran, c->crss, c->gamdot,
+                       c->phase, c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr, nullptr,
+                       c->newton_shortcut);
     if (st) return st;
+    {
+        double *t = c->sig;
+        c->sig = c->sig_alt;
+        c->sig_alt = t;
+    }
     tmark(c, 0, 1);
     st = reduce_partials(c, c->partials, nblk, NRED, c->red);
     if (st) return st;
@@ -316,6 +376,7 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
     // the epilogue (256-lane blocks, NRED_EPI partials) needs less
     long long nblk = (N + 63) / 64;
     if (!st) st = dalloc(&c->sig, 6 * N);
+    if (!st) st = dalloc(&c->sig_alt, 6 * N);
     if (!st) st = dalloc(&c->E, 6 * N);
     if (!st) st = dalloc(&c->W, 3 * N);
     if (!st) st = dalloc(&c->epsp, 6 * N);
@@ -337,6 +398,7 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
     c->anchor_every = 16;
     c->nr_tol = 1E-10;
     c->tolx = 1.E-7;
+    c->newton_shortcut = 1;
     if (!st) st = dalloc(&c->partials, nblk * NRED);
     if (!st) st = dalloc(&c->red, NRED);
     if (!st) st = dalloc(&c->red2, (long long)REDUCE_G * NRED);
@@ -362,7 +424,7 @@ int evpgpu_destroy(evpgpu_ctx *c)
 {
     if (!c) return EVPGPU_OK;
     adjoint_free(c);
-    void *ptrs[] = {c->sig, c->E, c->W, c->epsp, c->edot, c->tran, c->crss, c->gamacum, c->gamdot,
+    void *ptrs[] = {c->sig, c->sig_alt, c->E, c->W, c->epsp, c->edot, c->tran, c->crss, c->gamacum, c->gamdot,
                     c->phase, c->grain, c->orient, c->sgr, c->spec, c->acc, c->rwork,
                     c->sigA, c->cA, c->skA, c->Estart, c->partials, c->red, c->red2, c->d_status, c->tw64[0], c->tw64[1], c->tw64[2],
                     c->tw32[0], c->tw32[1], c->tw32[2]};
@@ -371,6 +433,7 @@ int evpgpu_destroy(evpgpu_ctx *c)
     for (int k = 0; k < 5; k++)
         for (int e = 0; e < 2; e++)
             if (c->ev[k][e]) (void)hipEventDestroy(c->ev[k][e]);
+    free(c->h_sgr);
     if (c->h_red) (void)hipHostFree(c->h_red);
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -403,6 +466,13 @@ int evpgpu_get_timing(evpgpu_ctx *c, int which, double *ms, long long *launches)
     if (!c || which < 0 || which > 4) return EVPGPU_ERR_ARG;
     if (ms) *ms = c->t_ms[which];
     if (launches) *launches = c->t_n[which];
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_newton_shortcut(evpgpu_ctx *c, int on)
+{
+    if (!c) return EVPGPU_ERR_ARG;
+    c->newton_shortcut = on ? 1 : 0;
     return EVPGPU_OK;
 }
 
@@ -458,6 +528,7 @@ int evpgpu_set_loading(evpgpu_ctx *c, const evpgpu_loading *ld)
     for (int i = 0; i < 6; i++) c->creep_flag += ld->stress_bc_flag[i];
     c->creep_flag /= 6;
     c->have_loading = 1;
+    update_lam_min(c);
     return EVPGPU_OK;
 }
 
@@ -482,6 +553,10 @@ int evpgpu_upload_microstructure(evpgpu_ctx *c, const int *grain, const int *pha
     free(o2);
     if (st) return st;
     c->n_orient = n_orient;
+    free(c->h_sgr);
+    c->h_sgr = (double *)malloc(sizeof(double) * 36 * n_orient);
+    memcpy(c->h_sgr, sgr, sizeof(double) * 36 * n_orient);
+    update_lam_min(c);
     c->have_ms = 1;
     return EVPGPU_OK;
 }
@@ -696,6 +771,28 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
             if (st) return st;
         }
     }
+    // The loop's Newton launches may end an iteration whose step the
+    // reference rejects before evaluating its trial point; gamdot comes from
+    // that trial (Q5) and is read only by the epilogue, so the last launch is
+    // repeated with every iteration carried out (same lambda, same sigma,
+    // edot and norms; gamdot as the reference leaves it).
+    {
+        NewtonParams P = newton_params(c);
+        int nblk = 0;
+        tmark(c, 0, 0);
+        st = newton_launch(c, P, c->sig_alt, c->sig, c->E, c->epsp, c->edot, c->tran, c->crss, c->gamdot,
+                           c->phase, c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr,
+                           nullptr, 0);
+        if (st) return st;
+        tmark(c, 0, 1);
+        EVP_CHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        EVP_CHECK(hipStreamSynchronize(c->stream));
+        {
+            const int w[1] = {0};
+            tcollect(c, w, 1);
+        }
+        if (*c->h_status) return *c->h_status;
+    }
     S.seconds = now_s() - t0;
     S.iters = iter;
     S.fp64_iters = it64;
@@ -899,8 +996,8 @@ int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient
         NewtonParams P = newton_params(c);
         P.crss_sym = 0;
         int nb = 0;
-        st = newton_launch(c, P, dsig, dstr, deps, ded, dsc, dcr, dgd, dph, dor, dsg, n, dpart, &nb, dst,
-                           der, dni);
+        st = newton_launch(c, P, dsig, dsig, dstr, deps, ded, dsc, dcr, dgd, dph, dor, dsg, n, dpart, &nb, dst,
+                           der, dni, 0);
         if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
     }
     int kst = 0;

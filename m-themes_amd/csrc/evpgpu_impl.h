@@ -33,6 +33,7 @@ struct NewtonParams {
     int crss_sym;    // 1: forward and backward CRSS are equal in every voxel
     double bn[2][EVPGPU_NSYSMX][3], bb[2][EVPGPU_NSYSMX][3]; // crystal-axis normals, directions
     double s066[36]; // C0^-1 (B-basis): the Newton step is solved in S0-scaled form
+    double kmin_tolx; // 0.99 TOLX min over orientations of lambda_min(S0 + S)
     double nr_tol;   // Newton tolerance on the relative step (reference 1e-10)
     double tolx;     // line-search TOLX (reference 1e-7)
 };
@@ -85,7 +86,9 @@ struct evpgpu_ctx {
     int crss_sym;
 
     // per-voxel fields (FP64, SoA component-major)
-    double *sig;      // 6N
+    double *sig;      // 6N  current stress
+    double *sig_alt;  // 6N  the other stress buffer: the Newton reads lambda from
+                      //     one and writes sigma to the other
     double *E;        // 6N  symmetric part of DisGrad
     double *W;        // 3N  antisymmetric part of DisGrad: (23, 13, 12)
     double *epsp;     // 6N  plastic strain (Eps)
@@ -96,6 +99,8 @@ struct evpgpu_ctx {
     double *gamdot;   // 12N
     int *phase, *grain, *orient;
     double *sgr;      // n_orient x 36: LU_inv_66(C_gr)
+    double *h_sgr;    // host copy
+    double lam_min;   // min over orientations of lambda_min(S0 + S), < 0 when stale
     int n_orient;
 
     // spectral buffers
@@ -110,6 +115,7 @@ struct evpgpu_ctx {
     double *Estart;   // 6N
     int anchor_every; // iterations between FP64 anchor transforms
     double nr_tol, tolx; // Newton tolerances (evpgpu_set_newton_tolerances)
+    int newton_shortcut; // end certainly-rejected Newton iterations early (default 1)
     double2 *tw64[3];
     float2 *tw32[3];
 
@@ -163,11 +169,11 @@ int green_anchor(evpgpu_ctx *c);
 int green_delta(evpgpu_ctx *c);
 int green_project(evpgpu_ctx *c);
 // newton.hip
-int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const double *E,
+int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, double *sig, const double *E,
                   const double *epsp, double *edot, const double *tran, const double *crss,
                   double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status,
-                  double *err_vox, int *niter_vox);
+                  double *err_vox, int *niter_vox, int fast);
 int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out);
 // fields.hip
 int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *macro_sym6);

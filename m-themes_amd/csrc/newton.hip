@@ -216,7 +216,16 @@ __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const
 }
 
 // returns 0, EVPGPU_ERR_SINGULAR or EVPGPU_ERR_LINESEARCH
-template <bool INTN>
+// FAST: an iteration whose step the reference's line search is certain to
+// reject (TOLX) ends right after the residual.  Since D >= 0, K = S0 + S +
+// dt D >= S0 + S, so |step|_inf <= |S0 R|_2 / lambda_min(S0 + S); when that
+// bound is below TOLX (P.kmin_tolx holds 0.99 TOLX lambda_min over all
+// orientations) the reference's test max|step_i|/max(|s_i|,1) < TOLX holds
+// and its first trial is rejected, leaving s, edot, the error norms and the
+// counts exactly as computed here.  Only gamdot (from the rejected trial
+// point) differs; the loop recomputes the last Newton launch of an increment
+// with FAST off before gamdot is used.
+template <bool INTN, bool FAST>
 __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, const Vox &V,
                             double sig6[6], double edotp[6], double gd[EVPGPU_NSYSMX],
                             double &conv_s, double &conv_e, int &iterl, int &trials)
@@ -245,6 +254,21 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             for (int i = 0; i < 6; i++) ce += t[i] * t[i];
             conv_e = sqrt(ce);
         }
+        bool skip = false;
+        if (FAST) {
+            double y2 = 0.0;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < 6; j++) a += P.s066[i * 6 + j] * res[j];
+                y2 += a * a;
+            }
+            skip = (y2 > 0.0) && (sqrt(y2) < P.kmin_tolx);
+        }
+        if (skip) {
+            trials++; // the reference's one rejected trial
+        } else {
         {
             // The Newton matrix J = I + C0 (S + dt D) times S0 = C0^-1 is the
             // symmetric positive definite K = S0 + S + dt D: the step solves
@@ -339,6 +363,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             f2 = f_new; // the backtrack value the cubic step needs (Q18)
             alam = tmplam > 0.1 * alam ? tmplam : 0.1 * alam;
         }
+        } // !skip
         double conv_nr = 0.0, cs = 0.0;
 #pragma unroll
         for (int i = 0; i < 6; i++) {
@@ -361,10 +386,10 @@ __device__ __forceinline__ double wave_sum(double v)
 // One lane per voxel.  Writes sig (in place), edot, gamdot; block partials
 // (NRED doubles per block) of err_s, err_e, Newton iterations, sum sig,
 // sum sig over phase 1.
-template <bool INTN, bool CSYM>
+template <bool INTN, bool CSYM, bool FAST>
 __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_newton(
     NewtonParams P, long long N, const int *__restrict__ phase, const int *__restrict__ orient,
-    const double *__restrict__ sgr, double *__restrict__ sig, const double *__restrict__ E,
+    const double *__restrict__ sgr, const double *sig_in, double *sig, const double *__restrict__ E,
     const double *__restrict__ epsp, double *__restrict__ edot, const double *__restrict__ tran,
     const double *__restrict__ crss, double *__restrict__ gamdot, double *__restrict__ partials,
     int *__restrict__ status, double *__restrict__ err_vox, int *__restrict__ niter_vox)
@@ -382,7 +407,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             RateIn in;
             double t[6];
 #pragma unroll
-            for (int c = 0; c < 6; c++) t[c] = sig[c * N + v];
+            for (int c = 0; c < 6; c++) t[c] = sig_in[c * N + v];
             // |lambda|_F over the full 3x3, row-major order
             V.signorm = sqrt(t[0] * t[0] + t[5] * t[5] + t[4] * t[4] + t[5] * t[5] + t[1] * t[1] +
                              t[3] * t[3] + t[4] * t[4] + t[3] * t[3] + t[2] * t[2]);
@@ -439,7 +464,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             }
             double s6[6], e6[6], gd[EVPGPU_NSYSMX], cs, ce;
             int it, ntr;
-            int st = update_voxel<INTN>(P, p, in, V, s6, e6, gd, cs, ce, it, ntr);
+            int st = update_voxel<INTN, FAST>(P, p, in, V, s6, e6, gd, cs, ce, it, ntr);
             if (st) {
                 atomicCAS(status, 0, st);
             } else {
@@ -522,23 +547,32 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ 
 
 } // namespace
 
-int newton_launch(evpgpu_ctx *c, const NewtonParams &P, double *sig, const double *E,
+int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, double *sig, const double *E,
                   const double *epsp, double *edot, const double *tran, const double *crss,
                   double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status, double *err_vox,
-                  int *niter_vox)
+                  int *niter_vox, int fast)
 {
     int nblk = (int)((n + NB_THREADS - 1) / NB_THREADS);
     // the integral-exponent variant needs every crystal phase to have one
     bool intn = true;
     for (int q = 0; q < 2; q++)
         if (!P.is_gas[q] && P.int_n[q] == 0) intn = false;
-#define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig, E, \
+#define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig_in, sig, E, \
                     epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
-    if (intn && P.crss_sym) hipLaunchKernelGGL((k_newton<true, true>), NEWTON_ARGS);
-    else if (intn) hipLaunchKernelGGL((k_newton<true, false>), NEWTON_ARGS);
-    else if (P.crss_sym) hipLaunchKernelGGL((k_newton<false, true>), NEWTON_ARGS);
-    else hipLaunchKernelGGL((k_newton<false, false>), NEWTON_ARGS);
+#define LAUNCH(A, B, F) hipLaunchKernelGGL((k_newton<A, B, F>), NEWTON_ARGS)
+    if (fast) {
+        if (intn && P.crss_sym) LAUNCH(true, true, true);
+        else if (intn) LAUNCH(true, false, true);
+        else if (P.crss_sym) LAUNCH(false, true, true);
+        else LAUNCH(false, false, true);
+    } else {
+        if (intn && P.crss_sym) LAUNCH(true, true, false);
+        else if (intn) LAUNCH(true, false, false);
+        else if (P.crss_sym) LAUNCH(false, true, false);
+        else LAUNCH(false, false, false);
+    }
+#undef LAUNCH
 #undef NEWTON_ARGS
     if (nblk_out) *nblk_out = nblk;
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;

@@ -75,7 +75,7 @@ NPARAM = 5
 # every symbol each header declares (checked by the CPU tests)
 EVPGPU_SYMBOLS = [
     "evpgpu_strerror", "evpgpu_create", "evpgpu_destroy", "evpgpu_set_precision",
-    "evpgpu_set_anchor_interval", "evpgpu_set_newton_tolerances",
+    "evpgpu_set_anchor_interval", "evpgpu_set_newton_tolerances", "evpgpu_set_newton_shortcut",
     "evpgpu_set_phases", "evpgpu_set_loading", "evpgpu_upload_microstructure",
     "evpgpu_upload_state", "evpgpu_download_state", "evpgpu_set_macro", "evpgpu_get_macro",
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
@@ -123,6 +123,7 @@ def gpu():
         L.evpgpu_set_precision.argtypes = [VP, C.c_int]
         L.evpgpu_set_anchor_interval.argtypes = [VP, C.c_int]
         L.evpgpu_set_newton_tolerances.argtypes = [VP, C.c_double, C.c_double]
+        L.evpgpu_set_newton_shortcut.argtypes = [VP, C.c_int]
         L.evpgpu_set_phases.argtypes = [VP, C.POINTER(Phase), C.c_int]
         L.evpgpu_set_loading.argtypes = [VP, C.POINTER(Loading)]
         L.evpgpu_upload_microstructure.argtypes = [VP, IP, IP, IP, C.c_int, DP]
@@ -244,6 +245,9 @@ class GpuJob:
 
     def set_newton_tolerances(self, tol: float = 1e-10, tolx: float = 1e-7):
         _chk(gpu().evpgpu_set_newton_tolerances(self.ctx, tol, tolx), "evpgpu_set_newton_tolerances")
+
+    def set_newton_shortcut(self, on: bool):
+        _chk(gpu().evpgpu_set_newton_shortcut(self.ctx, int(bool(on))), "evpgpu_set_newton_shortcut")
 
     def set_anchor_interval(self, iterations: int):
         _chk(gpu().evpgpu_set_anchor_interval(self.ctx, iterations), "evpgpu_set_anchor_interval")

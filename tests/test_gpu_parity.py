@@ -201,6 +201,23 @@ def test_loop_parity_16_pores(job_factory):
     _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
 
 
+def test_newton_shortcut_is_exact(job_factory):
+    """Ending certainly-rejected Newton iterations early changes nothing: the
+    whole state after 3 increments is bit-identical with the shortcut off."""
+    inp = job_factory("short16", n=16, grains=8, seed=17, increments=3, pore_fraction=0.02)
+    a = lib.GpuJob(inp)
+    b = lib.GpuJob(inp)
+    b.set_newton_shortcut(False)
+    ia, sa = a.run(3)
+    ib, sb = b.run(3)
+    assert list(ia) == list(ib)
+    assert [x["newton_iters"] for x in sa] == [x["newton_iters"] for x in sb]
+    assert [x["ls_trials"] for x in sa] == [x["ls_trials"] for x in sb]
+    A, B = a.state(), b.state()
+    for k in A:
+        assert np.array_equal(A[k], B[k]), k
+
+
 def test_mixed_precision_stress_error(job_factory):
     """FP32 FFT/Green with FP64 Newton, anchors, projection and FP64 sweeps
     against the FP64 loop.  At equal iteration counts (the parity protocol
