@@ -74,7 +74,7 @@ def run_gpu(inp, precision, steps, warmup):
     job.synchronize()
     wall = time.perf_counter() - t0
     out = dict(wall=wall, iters=[int(i) for i in it], stats=stats,
-               timing={k: job.timing(k) for k in range(5)}, macro=job.macro(),
+               timing={k: job.timing(k) for k in range(6)}, macro=job.macro(),
                sig=job.state()["Sig"])
     job.close()
     return out
@@ -239,7 +239,8 @@ def main():
                     frac=gbs / PEAK_HBM_GBS, traffic=None, kernel="k_newton",
                     avg_launch_ms=ms / max(launches, 1), tflops=tf)
     kern = {}
-    names = ["newton", "fft_forward", "green", "fft_inverse_update", "increment_pro_epilogue"]
+    names = ["newton", "fft_forward", "green", "fft_inverse_update", "increment_pro_epilogue",
+             "anchor_refresh"]
     for k, nm in enumerate(names):
         kms, kn = R["timing"][k]
         kern[nm] = dict(ms_total=round(kms, 3), launches=kn, ms_avg=round(kms / max(kn, 1), 4))

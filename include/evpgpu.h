@@ -113,6 +113,11 @@ int evpgpu_set_newton_tolerances(evpgpu_ctx *ctx, double tol, double tolx);
  * each increment is repeated in full for the rejected trial's gamdot); off
  * runs every iteration in full.  Default on. */
 int evpgpu_set_newton_shortcut(evpgpu_ctx *ctx, int on);
+/* Where ny = nz <= 128, a Green iteration's transforms run as three fused
+ * passes (z+y plane R2C, x lines with the Green step, y+z plane C2R) instead
+ * of seven; the results agree with the seven-pass path to rounding.  Off
+ * forces the seven passes.  Default on. */
+int evpgpu_set_fused_transforms(evpgpu_ctx *ctx, int on);
 /* Mixed precision: iterations between FP64 transforms of the stress anchor
  * (default 16, at least 1).  No counterpart in the reference (FP64 only). */
 int evpgpu_set_anchor_interval(evpgpu_ctx *ctx, int iterations);
@@ -142,7 +147,8 @@ int evpgpu_increment(evpgpu_ctx *ctx, int forced_iters, evpgpu_stats *stats);
 
 /* Kernel timing with HIP events on the context's stream (off by default).
  * which: 0 per-voxel Newton update, 1 forward FFT, 2 Green step, 3 inverse FFT
- * with strain update, 4 increment prologue/epilogue.  Reports the summed
+ * with strain update, 4 increment prologue/epilogue, 5 anchor refresh of the
+ * mixed-precision loop (FP64 transform of sigma_A).  Reports the summed
  * elapsed milliseconds and the launch count since the last reset.            */
 int evpgpu_set_timing(evpgpu_ctx *ctx, int on);
 /* Wait for all work of the context's stream and of the device. */

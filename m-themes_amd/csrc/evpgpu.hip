@@ -28,6 +28,33 @@
 #include "evpgpu_impl.h"
 
 // ------------------------------------------------------------ host tensors --
+// ---- kernel timing ----
+void tmark(evpgpu_ctx *c, int k, int end)
+{
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev[k][end], c->stream);
+    if (end) c->tmask |= 1u << k;
+}
+
+// add the elapsed times of the event pairs recorded since the last collect
+// (after the stream has been synchronised); a pair never recorded is never
+// queried, so no error is left behind for the next launch check
+void tcollect(evpgpu_ctx *c)
+{
+    if (!c->timing) return;
+    for (int k = 0; k < EVP_NTIMERS; k++) {
+        if (!(c->tmask & (1u << k))) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[k][0], c->ev[k][1]) == hipSuccess) {
+            c->t_ms[k] += ms;
+            c->t_n[k]++;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    c->tmask = 0;
+}
+
 namespace {
 
 const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
@@ -231,24 +258,6 @@ void smacro(evpgpu_ctx *c, const double *red)
     c->m.s_vm1 = sqrt(3. / 2. * a);
 }
 
-inline void tmark(evpgpu_ctx *c, int k, int end)
-{
-    if (c->timing) (void)hipEventRecord(c->ev[k][end], c->stream);
-}
-
-// add the elapsed times of the events recorded since the last collect
-void tcollect(evpgpu_ctx *c, const int *which, int n)
-{
-    if (!c->timing) return;
-    for (int i = 0; i < n; i++) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev[which[i]][0], c->ev[which[i]][1]) == hipSuccess) {
-            c->t_ms[which[i]] += ms;
-            c->t_n[which[i]]++;
-        }
-    }
-}
-
 // one Green iteration; leaves errors and macros updated.  prec FP64 runs the
 // FFT/Green/C2R path in FP64 on sigma; MIXED runs it in FP32 on
 // delta = sigma - sigma_A (anchor refreshed first when refresh != 0).
@@ -256,15 +265,32 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
 {
     const double scale = 1.0 / (double)c->N;
     int st = EVPGPU_OK;
+    const bool fused = c->fused && fused_ok(c);
     if (prec == EVPGPU_MIXED && refresh) {
-        tmark(c, 1, 0);
+        tmark(c, 5, 0);
         st = to_f32_launch(c, c->sig, c->sigA, 6 * c->N);
-        if (!st) st = fft_forward_anchor(c, c->sigA);
-        if (!st) st = green_anchor(c);
-        if (!st) st = fft_inverse_anchor(c, c->cA, scale);
+        if (fused) {
+            if (!st) st = fused_anchor(c, c->sigA, c->cA, scale);
+        } else {
+            if (!st) st = fft_forward_anchor(c, c->sigA);
+            if (!st) st = green_anchor(c);
+            if (!st) st = fft_inverse_anchor(c, c->cA, scale);
+        }
         if (st) return st;
-        tmark(c, 1, 1);
+        tmark(c, 5, 1);
     }
+    double add[6];
+    const double *d = c->dD;
+    add[0] = d[0]; add[1] = d[4]; add[2] = d[8];
+    add[3] = (d[5] + d[7]) / 2.0;
+    add[4] = (d[2] + d[6]) / 2.0;
+    add[5] = (d[1] + d[3]) / 2.0;
+    if (fused) {
+        for (int i = 0; i < 9; i++) c->dD_applied[i] += d[i];
+        st = prec == EVPGPU_MIXED ? fused_iteration_delta(c, c->sig, c->sigA, c->E, add, c->cA, scale)
+                                  : fused_iteration_fp64(c, c->sig, c->E, add, scale);
+        if (st) return st;
+    } else {
     tmark(c, 1, 0);
     st = prec == EVPGPU_MIXED ? fft_forward_delta(c, c->sig, c->sigA) : fft_forward(c, c->sig, 6, prec);
     if (st) return st;
@@ -273,18 +299,13 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
     st = prec == EVPGPU_MIXED ? green_delta(c) : green_apply(c, prec, 1);
     if (st) return st;
     tmark(c, 2, 1);
-    double add[6];
-    const double *d = c->dD;
-    add[0] = d[0]; add[1] = d[4]; add[2] = d[8];
-    add[3] = (d[5] + d[7]) / 2.0;
-    add[4] = (d[2] + d[6]) / 2.0;
-    add[5] = (d[1] + d[3]) / 2.0;
     for (int i = 0; i < 9; i++) c->dD_applied[i] += d[i];
     tmark(c, 3, 0);
     st = prec == EVPGPU_MIXED ? fft_inverse_delta(c, c->E, add, c->cA, scale)
                               : fft_inverse_update(c, 6, prec, c->E, add, scale);
     if (st) return st;
     tmark(c, 3, 1);
+    }
     NewtonParams P = newton_params(c);
     int nblk = 0;
     tmark(c, 0, 0);
@@ -303,10 +324,7 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
     EVP_CHECK(hipMemcpyAsync(c->h_red, c->red, NRED * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EVP_CHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     EVP_CHECK(hipStreamSynchronize(c->stream));
-    {
-        const int w[4] = {0, 1, 2, 3};
-        tcollect(c, w, 4);
-    }
+    tcollect(c);
     if (*c->h_status) return *c->h_status;
     c->m.err_e = c->h_red[1] * scale;
     c->m.err_s = c->h_red[0] * scale;
@@ -399,6 +417,7 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
     c->nr_tol = 1E-10;
     c->tolx = 1.E-7;
     c->newton_shortcut = 1;
+    c->fused = 1;
     if (!st) st = dalloc(&c->partials, nblk * NRED);
     if (!st) st = dalloc(&c->red, NRED);
     if (!st) st = dalloc(&c->red2, (long long)REDUCE_G * NRED);
@@ -430,7 +449,7 @@ int evpgpu_destroy(evpgpu_ctx *c)
                     c->tw32[0], c->tw32[1], c->tw32[2]};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
         if (ptrs[i]) (void)hipFree(ptrs[i]);
-    for (int k = 0; k < 5; k++)
+    for (int k = 0; k < EVP_NTIMERS; k++)
         for (int e = 0; e < 2; e++)
             if (c->ev[k][e]) (void)hipEventDestroy(c->ev[k][e]);
     free(c->h_sgr);
@@ -445,9 +464,10 @@ int evpgpu_set_timing(evpgpu_ctx *c, int on)
 {
     if (!c) return EVPGPU_ERR_ARG;
     if (on && !c->ev[0][0])
-        for (int k = 0; k < 5; k++)
+        for (int k = 0; k < EVP_NTIMERS; k++)
             for (int e = 0; e < 2; e++) EVP_CHECK(hipEventCreate(&c->ev[k][e]));
     c->timing = on ? 1 : 0;
+    c->tmask = 0;
     memset(c->t_ms, 0, sizeof(c->t_ms));
     memset(c->t_n, 0, sizeof(c->t_n));
     return EVPGPU_OK;
@@ -463,9 +483,16 @@ int evpgpu_synchronize(evpgpu_ctx *c)
 
 int evpgpu_get_timing(evpgpu_ctx *c, int which, double *ms, long long *launches)
 {
-    if (!c || which < 0 || which > 4) return EVPGPU_ERR_ARG;
+    if (!c || which < 0 || which >= EVP_NTIMERS) return EVPGPU_ERR_ARG;
     if (ms) *ms = c->t_ms[which];
     if (launches) *launches = c->t_n[which];
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_fused_transforms(evpgpu_ctx *c, int on)
+{
+    if (!c) return EVPGPU_ERR_ARG;
+    c->fused = on ? 1 : 0;
     return EVPGPU_OK;
 }
 
@@ -695,10 +722,7 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     if (st) return st;
     tmark(c, 4, 1);
     EVP_CHECK(hipStreamSynchronize(c->stream));
-    {
-        const int w[1] = {4};
-        tcollect(c, w, 1);
-    }
+    tcollect(c);
     memset(c->dD, 0, sizeof(c->dD));
     memset(c->dD_acum, 0, sizeof(c->dD_acum));
     memset(c->dD_applied, 0, sizeof(c->dD_applied));
@@ -787,10 +811,7 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
         tmark(c, 0, 1);
         EVP_CHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         EVP_CHECK(hipStreamSynchronize(c->stream));
-        {
-            const int w[1] = {0};
-            tcollect(c, w, 1);
-        }
+        tcollect(c);
         if (*c->h_status) return *c->h_status;
     }
     S.seconds = now_s() - t0;
@@ -851,10 +872,7 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     tmark(c, 4, 1);
     EVP_CHECK(hipMemcpyAsync(c->h_red, c->red, NRED_EPI * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EVP_CHECK(hipStreamSynchronize(c->stream));
-    {
-        const int w[1] = {4};
-        tcollect(c, w, 1);
-    }
+    tcollect(c);
     sym6_to_full(c->h_red, m.eps_avg);
     sym6_to_full(c->h_red + 6, m.edot_avg);
     double ev = 0.0, dv = 0.0;

@@ -66,6 +66,10 @@ struct EpilogueParams {
 // number of doubles each Newton block writes to the partials array:
 // err_s, err_e, newton iterations, sum sig (6), sum sig over phase 1 (6),
 // line-search trials
+// timer slots: 0 Newton, 1 forward FFT, 2 Green, 3 inverse FFT + strain update,
+// 4 increment prologue/epilogue, 5 anchor refresh of the mixed-precision loop
+#define EVP_NTIMERS 6
+
 #define NRED 16
 // epilogue partials: sum eps*wgt (6), sum edot*wgt (6)
 #define NRED_EPI 12
@@ -116,6 +120,7 @@ struct evpgpu_ctx {
     int anchor_every; // iterations between FP64 anchor transforms
     double nr_tol, tolx; // Newton tolerances (evpgpu_set_newton_tolerances)
     int newton_shortcut; // end certainly-rejected Newton iterations early (default 1)
+    int fused;           // one-pass-per-axis-pair FFT/Green kernels where they fit (default 1)
     double2 *tw64[3];
     float2 *tw32[3];
 
@@ -129,9 +134,10 @@ struct evpgpu_ctx {
 
     // kernel timing (HIP events), see evpgpu_set_timing
     int timing;
-    hipEvent_t ev[5][2];
-    double t_ms[5];
-    long long t_n[5];
+    unsigned tmask;   // pairs recorded since the last collect
+    hipEvent_t ev[EVP_NTIMERS][2];
+    double t_ms[EVP_NTIMERS];
+    long long t_n[EVP_NTIMERS];
 
     // macro state (host)
     evpgpu_macro m;
@@ -166,6 +172,16 @@ int fft_inverse_set(evpgpu_ctx *c, double *target, const double *base, double sc
 int green_apply(evpgpu_ctx *c, int precision, int accumulate);
 int green_full9(evpgpu_ctx *c, const double2 *in6, double2 *out9);
 int green_anchor(evpgpu_ctx *c);
+// fft_fused.hip
+// HIP-event timing (no-ops unless evpgpu_set_timing is on)
+void tmark(evpgpu_ctx *c, int k, int end);
+void tcollect(evpgpu_ctx *c);
+
+bool fused_ok(const evpgpu_ctx *c);
+int fused_iteration_fp64(evpgpu_ctx *c, const double *sig, double *E, const double *add, double scale);
+int fused_iteration_delta(evpgpu_ctx *c, const double *sig, const float *sigA, double *E, const double *add,
+                          const float *cA, double scale);
+int fused_anchor(evpgpu_ctx *c, const float *sigA, float *cA, double scale);
 int green_delta(evpgpu_ctx *c);
 int green_project(evpgpu_ctx *c);
 // newton.hip

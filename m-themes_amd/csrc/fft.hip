@@ -21,81 +21,11 @@
 
 #include "evpgpu_impl.h"
 
+#include "fft_core.h"
+
 namespace {
 
-template <typename R> struct CT;
-template <> struct CT<double> { typedef double2 T; };
-template <> struct CT<float> { typedef float2 T; };
-
-template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
-template <typename C> __device__ __forceinline__ C csub(C a, C b) { return {a.x - b.x, a.y - b.y}; }
-template <typename C> __device__ __forceinline__ C cmul(C a, C b)
-{
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-// multiply by -i (forward) or +i (inverse)
-template <bool INV, typename C> __device__ __forceinline__ C cmi(C a)
-{
-    if (INV) return {-a.y, a.x};
-    return {a.y, -a.x};
-}
-
-template <bool INV, typename C> __device__ __forceinline__ void bf2(C &a, C &b)
-{
-    C t = a;
-    a = cadd(t, b);
-    b = csub(t, b);
-}
-
-template <bool INV, typename C> __device__ __forceinline__ void bf4(C v[4])
-{
-    C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    C s13 = cadd(v[1], v[3]), d13 = cmi<INV>(csub(v[1], v[3]));
-    v[0] = cadd(s02, s13);
-    v[2] = csub(s02, s13);
-    v[1] = cadd(d02, d13);
-    v[3] = csub(d02, d13);
-}
-
-template <bool INV, typename C> __device__ __forceinline__ void bf8(C v[8])
-{
-    typedef decltype(v[0].x) R;
-    const R h = (R)0.70710678118654752440;
-    C e[4] = {v[0], v[2], v[4], v[6]};
-    C o[4] = {v[1], v[3], v[5], v[7]};
-    bf4<INV>(e);
-    bf4<INV>(o);
-    // o[k] *= W8^k, W8 = exp(-+ 2 pi i / 8)
-    C w1 = INV ? C{h, h} : C{h, -h};
-    C w3 = INV ? C{-h, h} : C{-h, -h};
-    o[1] = cmul(o[1], w1);
-    o[2] = cmi<INV>(o[2]);
-    o[3] = cmul(o[3], w3);
-    for (int k = 0; k < 4; k++) {
-        v[k] = cadd(e[k], o[k]);
-        v[k + 4] = csub(e[k], o[k]);
-    }
-}
-
-template <bool INV, int RAD, typename C> __device__ __forceinline__ void bfly(C v[RAD])
-{
-    if constexpr (RAD == 2) bf2<INV>(v[0], v[1]);
-    else if constexpr (RAD == 4) bf4<INV>(v);
-    else bf8<INV>(v);
-}
-
-constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-// radix of stage s for length L: radix 8 first, then the 4 or 2 left over
-constexpr int radix_at(int L, int s)
-{
-    int m = ilog2(L), q = m / 3, r = m % 3;
-    return s < q ? 8 : (r == 2 ? 4 : 2);
-}
-constexpr int nstages(int L)
-{
-    int m = ilog2(L);
-    return m / 3 + (m % 3 ? 1 : 0);
-}
+using namespace fftc;
 
 constexpr int TB = 16;      // lines per workgroup
 constexpr int NT = 256;     // threads per workgroup

@@ -1,0 +1,140 @@
+// fft_core.h -- Stockham FFT building blocks shared by the FFT passes
+// (fft.hip) and the fused plane / line kernels (fft_fused.hip): complex
+// helpers, radix-2/4/8 butterflies, the radix plan of a power-of-2 length
+// and a generic LDS stage over any set of lines.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fftc {
+
+template <typename R> struct CT;
+template <> struct CT<double> { typedef double2 T; };
+template <> struct CT<float> { typedef float2 T; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return {a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename C> __device__ __forceinline__ C cmi(C a)
+{
+    if (INV) return {-a.y, a.x};
+    return {a.y, -a.x};
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void bf2(C &a, C &b)
+{
+    C t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void bf4(C v[4])
+{
+    C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    C s13 = cadd(v[1], v[3]), d13 = cmi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void bf8(C v[8])
+{
+    typedef decltype(v[0].x) R;
+    const R h = (R)0.70710678118654752440;
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
+    bf4<INV>(e);
+    bf4<INV>(o);
+    // o[k] *= W8^k, W8 = exp(-+ 2 pi i / 8)
+    C w1 = INV ? C{h, h} : C{h, -h};
+    C w3 = INV ? C{-h, h} : C{-h, -h};
+    o[1] = cmul(o[1], w1);
+    o[2] = cmi<INV>(o[2]);
+    o[3] = cmul(o[3], w3);
+    for (int k = 0; k < 4; k++) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+template <bool INV, int RAD, typename C> __device__ __forceinline__ void bfly(C v[RAD])
+{
+    if constexpr (RAD == 2) bf2<INV>(v[0], v[1]);
+    else if constexpr (RAD == 4) bf4<INV>(v);
+    else bf8<INV>(v);
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+// radix of stage s for length L: radix 8 first, then the 4 or 2 left over
+constexpr int radix_at(int L, int s)
+{
+    int m = ilog2(L), q = m / 3, r = m % 3;
+    return s < q ? 8 : (r == 2 ? 4 : 2);
+}
+constexpr int nstages(int L)
+{
+    int m = ilog2(L);
+    return m / 3 + (m % 3 ? 1 : 0);
+}
+
+// One Stockham stage of length-L FFTs on NL lines held in LDS, element e of
+// line t at lds[t*LS + e*ES].  MAP 0 gives consecutive threads consecutive
+// butterflies of one line (lines along the fast LDS axis), MAP 1 consecutive
+// lines (lines across it), so that LDS accesses stay bank-friendly either
+// way.  Twiddles: a table of exp(-2 pi i m / (L*TWS)), read at stride TWS.
+template <bool INV, int L, int RAD, int NS, int NL, int LS, int ES, int MAP, int NTH, int TWS, typename C>
+__device__ __forceinline__ void gstage(C *lds, const C *tw)
+{
+    constexpr int NB = L / RAD;
+    constexpr int TOT = NB * NL;
+    constexpr int PER = (TOT + NTH - 1) / NTH;
+    C v[PER][RAD];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int b = threadIdx.x + q * NTH;
+        if (b < TOT) {
+            int t = MAP ? b % NL : b / NB, j = MAP ? b / NL : b % NB;
+#pragma unroll
+            for (int r = 0; r < RAD; r++) v[q][r] = lds[t * LS + (j + r * NB) * ES];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int b = threadIdx.x + q * NTH;
+        if (b < TOT) {
+            int t = MAP ? b % NL : b / NB, j = MAP ? b / NL : b % NB;
+            int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; r++) {
+                    C w = tw[((r * k * (L / (NS * RAD))) % L) * TWS];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            bfly<INV, RAD>(v[q]);
+            int d = (j / NS) * NS * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; r++) lds[t * LS + (d + r * NS) * ES] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool INV, int L, int S, int NS, int NL, int LS, int ES, int MAP, int NTH, int TWS, typename C>
+__device__ __forceinline__ void gstages(C *lds, const C *tw)
+{
+    if constexpr (S < nstages(L)) {
+        constexpr int RAD = radix_at(L, S);
+        gstage<INV, L, RAD, NS, NL, LS, ES, MAP, NTH, TWS>(lds, tw);
+        gstages<INV, L, S + 1, NS * RAD, NL, LS, ES, MAP, NTH, TWS>(lds, tw);
+    }
+}
+
+} // namespace fftc

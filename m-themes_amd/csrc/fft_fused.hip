@@ -1,0 +1,379 @@
+// fft_fused.hip -- one Green iteration's transforms in three passes over the
+// spectrum instead of seven (fft.hip's z, y, x passes each way plus the Green
+// pass), for grids whose (y, z) half-spectrum plane fits in LDS.
+//
+//   k_zy_r2c   one workgroup per (component, x): the ny x nz real plane is
+//              read once, transformed along z as ny real FFTs (each as a
+//              length-nz/2 complex FFT of the even/odd samples plus the
+//              split step), then along y for every kz, and written once as
+//              the ny x (nz/2+1) half-spectrum plane;
+//   k_x_green  one workgroup per (y, run of TBX kz values): all 6 components
+//              of those lines are transformed along x, the Green operator is
+//              applied at each wave vector (green_point.h, with the
+//              antisymmetric accumulator update), and transformed back;
+//   k_yz_c2r   the mirror of k_zy_r2c, ending in the strain update.
+//
+// The results equal fft.hip's up to rounding: the same half-spectrum
+// convention, the real transforms built from half-length complex ones, and
+// the C2R taking the real part of the Hermitian extension (the imaginary
+// parts of the kz = 0 and kz = nz/2 entries are dropped as fft.hip drops
+// them).  LDS per workgroup: ny (nz/2+1) complex numbers for the plane
+// kernels (133 KB at 128^2 in FP64), 6 TBX (nx+1) for the line kernel.
+#include <math.h>
+
+#include "evpgpu_impl.h"
+#include "fft_core.h"
+#include "green_point.h"
+
+namespace {
+
+using namespace fftc;
+
+constexpr int NTH = 1024;        // threads per workgroup of the plane kernels
+constexpr int NTX = 512;         // of the x-line kernel (the Green step needs 256 VGPRs)
+constexpr int LDS_MAX = 160 * 1024;
+
+template <typename R, int NY, int NZ> struct Plane {
+    typedef typename CT<R>::T C;
+    static constexpr int M = NZ / 2;      // half length of a z line
+    static constexpr int P = M + 1;       // row pitch = nz/2 + 1 (half spectrum)
+    static constexpr int BYTES = NY * P * (int)sizeof(C);
+};
+
+// z line of M packed complex values Z -> the M+1 half-spectrum values X of
+// the 2M reals (in place; thread pairs k and M-k)
+template <typename R, int M, typename C>
+__device__ __forceinline__ void split_forward(C *row, int k, const C *twz)
+{
+    if (k == 0) {
+        C z = row[0];
+        row[0] = C{z.x + z.y, (R)0};
+        row[M] = C{z.x - z.y, (R)0};
+        return;
+    }
+    C a = row[k], b = row[M - k];
+    // Fe = (Z_k + conj(Z_{M-k}))/2, Fo = (Z_k - conj(Z_{M-k}))/(2i)
+    C fe = {(a.x + b.x) * (R)0.5, (a.y - b.y) * (R)0.5};
+    C fo = {(a.y + b.y) * (R)0.5, -(a.x - b.x) * (R)0.5};
+    C w = twz[k]; // exp(-2 pi i k / 2M)
+    C wf = cmul(w, fo);
+    row[k] = cadd(fe, wf);
+    if (2 * k != M) row[M - k] = C{fe.x - wf.x, -(fe.y - wf.y)}; // conj(Fe - W^k Fo)
+}
+
+// the inverse: half spectrum X_0..X_M -> M packed complex values Z whose
+// inverse FFT gives (x_2n, x_2n+1) of the C2R of the Hermitian extension
+template <typename R, int M, typename C>
+__device__ __forceinline__ void split_inverse(C *row, int k, const C *twz)
+{
+    if (k == 0) {
+        R a = row[0].x, b = row[M].x; // imaginary parts dropped
+        row[0] = C{a + b, a - b};
+        return;
+    }
+    C xk = row[k], xm = row[M - k];
+    C w = twz[k];
+    w.y = -w.y; // exp(+2 pi i k / 2M)
+    // Z_k = (X_k + conj(X_{M-k})) + i w^k (X_k - conj(X_{M-k}))
+    C s = {xk.x + xm.x, xk.y - xm.y}, d = {xk.x - xm.x, xk.y + xm.y};
+    C wd = cmul(w, d);
+    C zk = {s.x - wd.y, s.y + wd.x};
+    if (2 * k != M) {
+        // Z_{M-k} = (X_{M-k} + conj(X_k)) + i w^{M-k} (X_{M-k} - conj(X_k)), w^{M-k} = -conj(w^k)
+        C s2 = {xm.x + xk.x, xm.y - xk.y}, d2 = {xm.x - xk.x, xm.y + xk.y};
+        C w2 = {-w.x, w.y};
+        C wd2 = cmul(w2, d2);
+        row[M - k] = C{s2.x - wd2.y, s2.y + wd2.x};
+    }
+    row[k] = zk;
+}
+
+// IN 0: FP64 field; IN 1: FP64 field minus FP32 anchor; IN 2: FP32 field
+template <typename R, int NY, int NZ, int IN>
+__global__ __launch_bounds__(NTH) void k_zy_r2c(const double *__restrict__ in, const float *__restrict__ inA,
+                                                typename CT<R>::T *__restrict__ out, long long N, long long NK,
+                                                int nx, const typename CT<R>::T *__restrict__ twy,
+                                                const typename CT<R>::T *__restrict__ twz)
+{
+    typedef Plane<R, NY, NZ> PL;
+    typedef typename PL::C C;
+    constexpr int M = PL::M, P = PL::P;
+    __shared__ C lds[NY * P];
+    const int x = blockIdx.x, comp = blockIdx.y;
+    const long long base = comp * N + (long long)x * NY * NZ;
+    for (int i = threadIdx.x; i < NY * M; i += NTH) {
+        const int y = i / M, n = i % M;
+        R a, b;
+        if (IN == 0) {
+            double2 v = reinterpret_cast<const double2 *>(in + base)[i];
+            a = (R)v.x;
+            b = (R)v.y;
+        } else if (IN == 1) {
+            double2 v = reinterpret_cast<const double2 *>(in + base)[i];
+            float2 w = reinterpret_cast<const float2 *>(inA + base)[i];
+            a = (R)(v.x - (double)w.x);
+            b = (R)(v.y - (double)w.y);
+        } else {
+            float2 w = reinterpret_cast<const float2 *>(inA + base)[i];
+            a = (R)w.x;
+            b = (R)w.y;
+        }
+        lds[y * P + n] = C{a, b};
+    }
+    __syncthreads();
+    // z: NY rows of M packed values (twiddles of length M = the nz table at stride 2)
+    gstages<false, M, 0, 1, NY, P, 1, 0, NTH, 2>(lds, twz);
+    for (int i = threadIdx.x; i < NY * (M / 2 + 1); i += NTH) {
+        const int y = i / (M / 2 + 1), k = i % (M / 2 + 1);
+        split_forward<R, M>(lds + y * P, k, twz);
+    }
+    __syncthreads();
+    // y: P columns of length NY
+    gstages<false, NY, 0, 1, P, 1, P, 1, NTH, 1>(lds, twy);
+    C *dst = out + comp * NK + (long long)x * NY * P;
+    for (int i = threadIdx.x; i < NY * P; i += NTH) dst[i] = lds[i];
+}
+
+// MODE 0: target += add + re*scale; 1: target = re*scale; 2: target += add +
+// fa + re*scale; 3: fo = (float)(re*scale); 4: target = base + re*scale
+template <typename R, int NY, int NZ, int MODE>
+__global__ __launch_bounds__(NTH) void k_yz_c2r(const typename CT<R>::T *__restrict__ in, double *target,
+                                                const float *__restrict__ fa, float *__restrict__ fo,
+                                                const double *__restrict__ basef, long long N, long long NK,
+                                                double a0, double a1, double a2, double a3, double a4,
+                                                double a5, double scale,
+                                                const typename CT<R>::T *__restrict__ twy,
+                                                const typename CT<R>::T *__restrict__ twz)
+{
+    typedef Plane<R, NY, NZ> PL;
+    typedef typename PL::C C;
+    constexpr int M = PL::M, P = PL::P;
+    __shared__ C lds[NY * P];
+    const int x = blockIdx.x, comp = blockIdx.y;
+    const C *src = in + comp * NK + (long long)x * NY * P;
+    for (int i = threadIdx.x; i < NY * P; i += NTH) lds[i] = src[i];
+    __syncthreads();
+    gstages<true, NY, 0, 1, P, 1, P, 1, NTH, 1>(lds, twy);
+    for (int i = threadIdx.x; i < NY * (M / 2 + 1); i += NTH) {
+        const int y = i / (M / 2 + 1), k = i % (M / 2 + 1);
+        split_inverse<R, M>(lds + y * P, k, twz);
+    }
+    __syncthreads();
+    gstages<true, M, 0, 1, NY, P, 1, 0, NTH, 2>(lds, twz);
+    const double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
+    const long long base = comp * N + (long long)x * NY * NZ;
+    for (int i = threadIdx.x; i < NY * M; i += NTH) {
+        const int y = i / M, n = i % M;
+        C z = lds[y * P + n];
+        double r0 = (double)z.x * scale, r1 = (double)z.y * scale;
+        double2 *t2 = reinterpret_cast<double2 *>(target + base);
+        if (MODE == 0) {
+            double2 t = t2[i];
+            t2[i] = make_double2(t.x + (add + r0), t.y + (add + r1));
+        } else if (MODE == 1) {
+            t2[i] = make_double2(r0, r1);
+        } else if (MODE == 2) {
+            double2 t = t2[i];
+            float2 f = reinterpret_cast<const float2 *>(fa + base)[i];
+            t2[i] = make_double2(t.x + (add + (double)f.x + r0), t.y + (add + (double)f.y + r1));
+        } else if (MODE == 3) {
+            reinterpret_cast<float2 *>(fo + base)[i] = make_float2((float)r0, (float)r1);
+        } else {
+            double2 b = reinterpret_cast<const double2 *>(basef + base)[i];
+            t2[i] = make_double2(b.x + r0, b.y + r1);
+        }
+    }
+}
+
+// GMODE 0: acc += antisym; 2: acc += antisym + skA; 3: skA = antisym
+template <typename R, int NX, int TBX, int GMODE>
+__global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__ spec, double2 *__restrict__ acc,
+                                                 float2 *__restrict__ skA, GreenParams G, int ny, int nz,
+                                                 int nzh, long long NK, const typename CT<R>::T *__restrict__ twx)
+{
+    typedef typename CT<R>::T C;
+    constexpr int PX = NX + 1;
+    constexpr int NL = 6 * TBX;
+    __shared__ C lds[NL * PX];
+    __shared__ R c0s[81];
+    if (threadIdx.x < 81) c0s[threadIdx.x] = (R)G.c0[threadIdx.x];
+    const int kz0 = blockIdx.x * TBX, y = blockIdx.y;
+    const int nk = min(TBX, nzh - kz0);
+    // line (c, j) = component c, kz = kz0 + j, along x; element x at spec[c*NK + (x*ny + y)*nzh + kz]
+    for (int i = threadIdx.x; i < NL * NX; i += NTX) {
+        const int c = i / (TBX * NX), r = i % (TBX * NX), x = r / TBX, j = r % TBX;
+        C v = C{(R)0, (R)0};
+        if (j < nk) v = spec[c * NK + ((long long)x * ny + y) * nzh + kz0 + j];
+        lds[(c * TBX + j) * PX + x] = v;
+    }
+    __syncthreads();
+    gstages<false, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
+#pragma unroll 1
+    for (int i = threadIdx.x; i < TBX * NX; i += NTX) {
+        // keep the 81 C0 values in LDS: hoisted out of this loop they would
+        // occupy 162 registers for the whole loop
+        __asm__ volatile("" ::: "memory");
+        const int x = i / TBX, j = i % TBX;
+        if (j >= nk) continue;
+        C s6[6], e6[6];
+        double wr[3], wi[3];
+#pragma unroll
+        for (int c = 0; c < 6; c++) s6[c] = lds[(c * TBX + j) * PX + x];
+        gop::green_point<R>(c0s, G.s0, G.elast_bc, x, y, kz0 + j, NX, ny, nz, s6, e6, wr, wi);
+#pragma unroll
+        for (int c = 0; c < 6; c++) lds[(c * TBX + j) * PX + x] = e6[c];
+        const long long id = ((long long)x * ny + y) * nzh + kz0 + j;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (GMODE == 3) {
+                skA[c * NK + id] = make_float2((float)wr[c], (float)wi[c]);
+            } else {
+                double2 w = acc[c * NK + id];
+                double ax = wr[c], ay = wi[c];
+                if (GMODE == 2) {
+                    float2 k = skA[c * NK + id];
+                    ax += (double)k.x;
+                    ay += (double)k.y;
+                }
+                acc[c * NK + id] = make_double2(w.x + ax, w.y + ay);
+            }
+        }
+    }
+    __syncthreads();
+    gstages<true, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
+    for (int i = threadIdx.x; i < NL * NX; i += NTX) {
+        const int c = i / (TBX * NX), r = i % (TBX * NX), x = r / TBX, j = r % TBX;
+        if (j < nk) spec[c * NK + ((long long)x * ny + y) * nzh + kz0 + j] = lds[(c * TBX + j) * PX + x];
+    }
+}
+
+template <typename R> const typename CT<R>::T *twp(evpgpu_ctx *c, int axis);
+template <> const double2 *twp<double>(evpgpu_ctx *c, int axis) { return c->tw64[axis]; }
+template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[axis]; }
+
+// lines per x-green workgroup: 6 components x TBX lines x (nx+1) in LDS
+template <typename R, int NX> constexpr int tbx()
+{
+    return (6 * 16 * (NX + 1) * (int)sizeof(typename CT<R>::T) <= LDS_MAX - 1024) ? 16
+           : (6 * 8 * (NX + 1) * (int)sizeof(typename CT<R>::T) <= LDS_MAX - 1024) ? 8
+                                                                                    : 4;
+}
+
+#define PLANE_DISPATCH(CALL)                                   \
+    if (c->ny == 128 && c->nz == 128) CALL(128, 128);          \
+    else if (c->ny == 64 && c->nz == 64) CALL(64, 64);         \
+    else if (c->ny == 32 && c->nz == 32) CALL(32, 32);         \
+    else if (c->ny == 16 && c->nz == 16) CALL(16, 16);         \
+    else if (c->ny == 8 && c->nz == 8) CALL(8, 8);             \
+    else return EVPGPU_ERR_SIZE;
+
+#define LINE_DISPATCH(CALL)             \
+    switch (c->nx) {                    \
+    case 8: CALL(8); break;             \
+    case 16: CALL(16); break;           \
+    case 32: CALL(32); break;           \
+    case 64: CALL(64); break;           \
+    case 128: CALL(128); break;         \
+    case 256: CALL(256); break;         \
+    default: return EVPGPU_ERR_SIZE;    \
+    }
+
+template <typename R, int IN>
+int zy(evpgpu_ctx *c, const double *in, const float *inA)
+{
+    typedef typename CT<R>::T C;
+    C *spec = (C *)c->spec;
+#define CALLP(NY_, NZ_)                                                                                   \
+    hipLaunchKernelGGL((k_zy_r2c<R, NY_, NZ_, IN>), dim3(c->nx, 6), dim3(NTH), 0, c->stream, in, inA, spec, \
+                       c->N, c->NK, c->nx, twp<R>(c, 1), twp<R>(c, 2))
+    PLANE_DISPATCH(CALLP);
+#undef CALLP
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+template <typename R, int MODE>
+int yz(evpgpu_ctx *c, double *target, const double *addc, double scale, const float *fa, float *fo,
+       const double *basef)
+{
+    typedef typename CT<R>::T C;
+    const C *spec = (const C *)c->spec;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    if (addc)
+        for (int i = 0; i < 6; i++) a[i] = addc[i];
+#define CALLP(NY_, NZ_)                                                                                    \
+    hipLaunchKernelGGL((k_yz_c2r<R, NY_, NZ_, MODE>), dim3(c->nx, 6), dim3(NTH), 0, c->stream, spec, target, \
+                       fa, fo, basef, c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5], scale, twp<R>(c, 1),  \
+                       twp<R>(c, 2))
+    PLANE_DISPATCH(CALLP);
+#undef CALLP
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+template <typename R, int GMODE>
+int xg(evpgpu_ctx *c)
+{
+    typedef typename CT<R>::T C;
+    GreenParams G;
+    memcpy(G.c0, c->ld.c0, sizeof(G.c0));
+    memcpy(G.s0, c->ld.s0, sizeof(G.s0));
+    G.elast_bc = c->ld.elast_bc;
+#define CALLL(NX_)                                                                                        \
+    {                                                                                                    \
+        constexpr int T = tbx<R, NX_>();                                                                 \
+        hipLaunchKernelGGL((k_x_green<R, NX_, T, GMODE>), dim3((c->nzh + T - 1) / T, c->ny), dim3(NTX), 0, \
+                           c->stream, (C *)c->spec, c->acc, c->skA, G, c->ny, c->nz, c->nzh, c->NK,      \
+                           twp<R>(c, 0));                                                                \
+    }
+    LINE_DISPATCH(CALLL);
+#undef CALLL
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+} // namespace
+
+// the fused path covers ny = nz in {8..128} (the plane fits in LDS in FP64)
+bool fused_ok(const evpgpu_ctx *c)
+{
+    return c->ny == c->nz && c->ny >= 8 && c->ny <= 128 && c->nx >= 8 && c->nx <= 256;
+}
+
+// FP64 iteration: spec = R2C(sigma) ; Green (acc += antisym) ; E += add + C2R/N
+int fused_iteration_fp64(evpgpu_ctx *c, const double *sig, double *E, const double *add, double scale)
+{
+    tmark(c, 1, 0);
+    int st = zy<double, 0>(c, sig, nullptr);
+    tmark(c, 1, 1);
+    tmark(c, 2, 0);
+    if (!st) st = xg<double, 0>(c);
+    tmark(c, 2, 1);
+    tmark(c, 3, 0);
+    if (!st) st = yz<double, 0>(c, E, add, scale, nullptr, nullptr, nullptr);
+    tmark(c, 3, 1);
+    return st;
+}
+
+// mixed loop, delta step (FP32): spec = R2C(sigma - sigma_A) ; Green (acc += antisym + skA) ;
+// E += add + c_A + C2R/N
+int fused_iteration_delta(evpgpu_ctx *c, const double *sig, const float *sigA, double *E, const double *add,
+                          const float *cA, double scale)
+{
+    tmark(c, 1, 0);
+    int st = zy<float, 1>(c, sig, sigA);
+    tmark(c, 1, 1);
+    tmark(c, 2, 0);
+    if (!st) st = xg<float, 2>(c);
+    tmark(c, 2, 1);
+    tmark(c, 3, 0);
+    if (!st) st = yz<float, 2>(c, E, add, scale, cA, nullptr, nullptr);
+    tmark(c, 3, 1);
+    return st;
+}
+
+// mixed loop, anchor (FP64): skA = antisym(Gamma sigma_A) ; c_A = C2R(Gamma sigma_A)/N
+int fused_anchor(evpgpu_ctx *c, const float *sigA, float *cA, double scale)
+{
+    int st = zy<double, 2>(c, nullptr, sigA);
+    if (!st) st = xg<double, 3>(c);
+    if (!st) st = yz<double, 3>(c, nullptr, nullptr, scale, nullptr, cA, nullptr);
+    return st;
+}

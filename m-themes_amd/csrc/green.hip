@@ -21,68 +21,14 @@
 // accumulator that the increment epilogue inverts once (they only feed the
 // lattice spin of update_orient).
 #include "evpgpu_impl.h"
+#include "green_point.h"
 #include "spectral.h"
 
 namespace {
 
-template <typename R> struct CT;
-template <> struct CT<double> { typedef double2 T; };
-template <> struct CT<float> { typedef float2 T; };
-
+using fftc::CT;
 using kspace::unit_k;
-
-// e = Gamma(n) s for complex s (3x3 full), e accumulated with weight w
-template <typename R, typename C>
-__device__ __forceinline__ void gamma_apply(const R *c0, const double nd[3], const C s[3][3],
-                                            C e[3][3], R w)
-{
-    R n[3] = {(R)nd[0], (R)nd[1], (R)nd[2]};
-    R A[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            R acc = 0;
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-#pragma unroll
-                for (int l = 0; l < 3; l++) acc += c0[((i * 3 + j) * 3 + k) * 3 + l] * n[j] * n[l];
-            A[i][k] = acc;
-        }
-    // M3Inverse (src/init.c:517-539)
-    R det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
-            A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
-            A[0][2] * (A[1][0] * A[2][1] - A[2][0] * A[1][1]);
-    R O[3][3];
-    O[0][0] = A[1][1] * A[2][2] - A[1][2] * A[2][1];
-    O[0][1] = A[0][2] * A[2][1] - A[0][1] * A[2][2];
-    O[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
-    O[1][0] = A[1][2] * A[2][0] - A[1][0] * A[2][2];
-    O[1][1] = A[0][0] * A[2][2] - A[0][2] * A[2][0];
-    O[1][2] = A[0][2] * A[1][0] - A[0][0] * A[1][2];
-    O[2][0] = A[1][0] * A[2][1] - A[1][1] * A[2][0];
-    O[2][1] = A[0][1] * A[2][0] - A[0][0] * A[2][1];
-    O[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
-    R id = (R)1 / det;
-    C t[3], u[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        t[k].x = s[k][0].x * n[0] + s[k][1].x * n[1] + s[k][2].x * n[2];
-        t[k].y = s[k][0].y * n[0] + s[k][1].y * n[1] + s[k][2].y * n[2];
-    }
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        u[i].x = (O[i][0] * t[0].x + O[i][1] * t[1].x + O[i][2] * t[2].x) * id * w;
-        u[i].y = (O[i][0] * t[0].y + O[i][1] * t[1].y + O[i][2] * t[2].y) * id * w;
-    }
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            e[i][j].x -= u[i].x * n[j];
-            e[i][j].y -= u[i].y * n[j];
-        }
-}
+using gop::gamma_apply;
 
 // MODE 0: in-place symmetric output + antisymmetric accumulation (acc != null)
 // MODE 1: write all 9 components to out9 (tests)
@@ -105,80 +51,37 @@ __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 
     C s6[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) s6[c] = spec[c * NK + id];
-    C s[3][3] = {{s6[0], s6[5], s6[4]}, {s6[5], s6[1], s6[3]}, {s6[4], s6[3], s6[2]}};
-    C e[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) e[i][j] = C{(R)0, (R)0};
-    if (x == 0 && y == 0 && kz == 0) {
-        if (P.elast_bc == 0) {
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++)
-#pragma unroll
-                    for (int k = 0; k < 3; k++)
-#pragma unroll
-                        for (int l = 0; l < 3; l++) {
-                            R g = -(R)P.s0[((i * 3 + j) * 3 + k) * 3 + l];
-                            e[i][j].x += g * s[k][l].x;
-                            e[i][j].y += g * s[k][l].y;
-                        }
-        }
-    } else {
-        double n1[3];
-        unit_k(x, y, kz, nx, ny, nz, n1);
-        bool nyq = (2 * x == nx) || (2 * y == ny) || (2 * kz == nz);
-        bool split = false;
-        double n2[3];
-        if (nyq) {
-            unit_k((nx - x) % nx, (ny - y) % ny, (nz - kz) % nz, nx, ny, nz, n2);
-            bool same = (n2[0] == n1[0] && n2[1] == n1[1] && n2[2] == n1[2]) ||
-                        (n2[0] == -n1[0] && n2[1] == -n1[1] && n2[2] == -n1[2]);
-            split = !same;
-        }
-        if (!split) {
-            gamma_apply<R>(c0s, n1, s, e, (R)1);
-        } else {
-            gamma_apply<R>(c0s, n1, s, e, (R)0.5);
-            gamma_apply<R>(c0s, n2, s, e, (R)0.5);
-        }
-    }
-    if (MODE != 1) {
-        const int ia[6] = {0, 1, 2, 1, 0, 0}, ib[6] = {0, 1, 2, 2, 2, 1};
-#pragma unroll
-        for (int c = 0; c < 6; c++) {
-            C v;
-            v.x = (e[ia[c]][ib[c]].x + e[ib[c]][ia[c]].x) * (R)0.5;
-            v.y = (e[ia[c]][ib[c]].y + e[ib[c]][ia[c]].y) * (R)0.5;
-            spec[c * NK + id] = v;
-        }
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            int a = ia[c + 3], b = ib[c + 3];
-            double wx = (double)((e[a][b].x - e[b][a].x) * (R)0.5);
-            double wy = (double)((e[a][b].y - e[b][a].y) * (R)0.5);
-            if (MODE == 3) {
-                skA[c * NK + id] = make_float2((float)wx, (float)wy);
-            } else if (acc) {
-                double2 w = acc[c * NK + id];
-                if (MODE == 2) {
-                    float2 k = skA[c * NK + id];
-                    wx += (double)k.x;
-                    wy += (double)k.y;
-                }
-                w.x += wx;
-                w.y += wy;
-                acc[c * NK + id] = w;
-            }
-        }
-    } else {
+    if (MODE == 1) {
+        C e[3][3];
+        gop::green_tensor<R>(c0s, P.s0, P.elast_bc, x, y, kz, nx, ny, nz, s6, e);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++)
                 out9[(i * 3 + j) * NK + id] = make_double2((double)e[i][j].x, (double)e[i][j].y);
+        return;
+    }
+    C e6[6];
+    double wr[3], wi[3];
+    gop::green_point<R>(c0s, P.s0, P.elast_bc, x, y, kz, nx, ny, nz, s6, e6, wr, wi);
+#pragma unroll
+    for (int c = 0; c < 6; c++) spec[c * NK + id] = e6[c];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (MODE == 3) {
+            skA[c * NK + id] = make_float2((float)wr[c], (float)wi[c]);
+        } else if (acc) {
+            double2 w = acc[c * NK + id];
+            double ax = wr[c], ay = wi[c];
+            if (MODE == 2) {
+                float2 k = skA[c * NK + id];
+                ax += (double)k.x;
+                ay += (double)k.y;
+            }
+            w.x += ax;
+            w.y += ay;
+            acc[c * NK + id] = w;
+        }
     }
 }
 

@@ -1,0 +1,133 @@
+// green_point.h -- the Green operator at one wave vector of the half
+// spectrum: e-hat = Gamma_eff(k) : sigma-hat (src/evolution.c:1373-1387 with
+// Gamma from src/init.c:541-591), shared by the Green pass (green.hip) and
+// the fused x-line kernel (fft_fused.hip).
+//
+// Gamma is never stored: Gamma_ijkl = -Omega_ik n_j n_l with
+// Omega = (C0_ijkl n_j n_l)^-1, so t = sigma-hat . n, u = Omega t,
+// e-hat_ij = -u_i n_j.  Where a component of k sits at Nyquist, n(-k) is not
+// -n(k) and Gamma_eff = (Gamma(n(k)) + Gamma(n(-k)))/2, which makes the C2R
+// of the half spectrum equal the reference's Re(IFFT) of the full one.  At
+// k = 0, -S0 : sigma-hat when ElastBC == 0, else zero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_core.h"
+#include "spectral.h"
+
+namespace gop {
+
+// e = Gamma(n) s for complex s (3x3 full), e accumulated with weight w
+template <typename R, typename C>
+__device__ __forceinline__ void gamma_apply(const R *c0, const double nd[3], const C s[3][3],
+                                            C e[3][3], R w)
+{
+    R n[3] = {(R)nd[0], (R)nd[1], (R)nd[2]};
+    R A[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            R acc = 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) acc += c0[((i * 3 + j) * 3 + k) * 3 + l] * n[j] * n[l];
+            A[i][k] = acc;
+        }
+    // M3Inverse (src/init.c:517-539)
+    R det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+            A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+            A[0][2] * (A[1][0] * A[2][1] - A[2][0] * A[1][1]);
+    R O[3][3];
+    O[0][0] = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    O[0][1] = A[0][2] * A[2][1] - A[0][1] * A[2][2];
+    O[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+    O[1][0] = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    O[1][1] = A[0][0] * A[2][2] - A[0][2] * A[2][0];
+    O[1][2] = A[0][2] * A[1][0] - A[0][0] * A[1][2];
+    O[2][0] = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    O[2][1] = A[0][1] * A[2][0] - A[0][0] * A[2][1];
+    O[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    R id = (R)1 / det;
+    C t[3], u[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        t[k].x = s[k][0].x * n[0] + s[k][1].x * n[1] + s[k][2].x * n[2];
+        t[k].y = s[k][0].y * n[0] + s[k][1].y * n[1] + s[k][2].y * n[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        u[i].x = (O[i][0] * t[0].x + O[i][1] * t[1].x + O[i][2] * t[2].x) * id * w;
+        u[i].y = (O[i][0] * t[0].y + O[i][1] * t[1].y + O[i][2] * t[2].y) * id * w;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            e[i][j].x -= u[i].x * n[j];
+            e[i][j].y -= u[i].y * n[j];
+        }
+}
+
+// the full (non-symmetric) e-hat of one wave vector
+template <typename R, typename C>
+__device__ __forceinline__ void green_tensor(const R *c0s, const double *s0, int elast_bc, int x, int y,
+                                             int kz, int nx, int ny, int nz, const C s6[6], C e[3][3])
+{
+    C s[3][3] = {{s6[0], s6[5], s6[4]}, {s6[5], s6[1], s6[3]}, {s6[4], s6[3], s6[2]}};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) e[i][j] = C{(R)0, (R)0};
+    if (x == 0 && y == 0 && kz == 0) {
+        if (elast_bc == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+#pragma unroll
+                        for (int l = 0; l < 3; l++) {
+                            R g = -(R)s0[((i * 3 + j) * 3 + k) * 3 + l];
+                            e[i][j].x += g * s[k][l].x;
+                            e[i][j].y += g * s[k][l].y;
+                        }
+        }
+        return;
+    }
+    double n1[3], n2[3];
+    if (!kspace::mode_dirs(x, y, kz, nx, ny, nz, n1, n2)) {
+        gamma_apply<R>(c0s, n1, s, e, (R)1);
+    } else {
+        gamma_apply<R>(c0s, n1, s, e, (R)0.5);
+        gamma_apply<R>(c0s, n2, s, e, (R)0.5);
+    }
+}
+
+// symmetric part (11 22 33 23 13 12) and antisymmetric part (23 13 12, as
+// FP64 real and imaginary parts) of e-hat at one wave vector
+template <typename R, typename C>
+__device__ __forceinline__ void green_point(const R *c0s, const double *s0, int elast_bc, int x, int y,
+                                            int kz, int nx, int ny, int nz, const C s6[6], C e6[6],
+                                            double wr[3], double wi[3])
+{
+    C e[3][3];
+    green_tensor<R>(c0s, s0, elast_bc, x, y, kz, nx, ny, nz, s6, e);
+    const int ia[6] = {0, 1, 2, 1, 0, 0}, ib[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        e6[c].x = (e[ia[c]][ib[c]].x + e[ib[c]][ia[c]].x) * (R)0.5;
+        e6[c].y = (e[ia[c]][ib[c]].y + e[ib[c]][ia[c]].y) * (R)0.5;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        int a = ia[c + 3], b = ib[c + 3];
+        wr[c] = (double)((e[a][b].x - e[b][a].x) * (R)0.5);
+        wi[c] = (double)((e[a][b].y - e[b][a].y) * (R)0.5);
+    }
+}
+
+} // namespace gop

@@ -76,6 +76,7 @@ NPARAM = 5
 EVPGPU_SYMBOLS = [
     "evpgpu_strerror", "evpgpu_create", "evpgpu_destroy", "evpgpu_set_precision",
     "evpgpu_set_anchor_interval", "evpgpu_set_newton_tolerances", "evpgpu_set_newton_shortcut",
+    "evpgpu_set_fused_transforms",
     "evpgpu_set_phases", "evpgpu_set_loading", "evpgpu_upload_microstructure",
     "evpgpu_upload_state", "evpgpu_download_state", "evpgpu_set_macro", "evpgpu_get_macro",
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
@@ -124,6 +125,7 @@ def gpu():
         L.evpgpu_set_anchor_interval.argtypes = [VP, C.c_int]
         L.evpgpu_set_newton_tolerances.argtypes = [VP, C.c_double, C.c_double]
         L.evpgpu_set_newton_shortcut.argtypes = [VP, C.c_int]
+        L.evpgpu_set_fused_transforms.argtypes = [VP, C.c_int]
         L.evpgpu_set_phases.argtypes = [VP, C.POINTER(Phase), C.c_int]
         L.evpgpu_set_loading.argtypes = [VP, C.POINTER(Loading)]
         L.evpgpu_upload_microstructure.argtypes = [VP, IP, IP, IP, C.c_int, DP]
@@ -228,7 +230,8 @@ class GpuJob:
 
     def timing(self, which: int):
         """(summed ms, launches) of kernel group `which` (0 Newton, 1 FFT fwd, 2 Green,
-        3 FFT inverse + strain update, 4 prologue/epilogue)."""
+        3 FFT inverse + strain update, 4 prologue/epilogue, 5 mixed-precision anchor
+        refresh)."""
         ms = C.c_double(0)
         n = C.c_longlong(0)
         _chk(gpu().evpgpu_get_timing(self.ctx, which, C.byref(ms), C.byref(n)), "get_timing")
@@ -248,6 +251,9 @@ class GpuJob:
 
     def set_newton_shortcut(self, on: bool):
         _chk(gpu().evpgpu_set_newton_shortcut(self.ctx, int(bool(on))), "evpgpu_set_newton_shortcut")
+
+    def set_fused_transforms(self, on: bool):
+        _chk(gpu().evpgpu_set_fused_transforms(self.ctx, int(bool(on))), "evpgpu_set_fused_transforms")
 
     def set_anchor_interval(self, iterations: int):
         _chk(gpu().evpgpu_set_anchor_interval(self.ctx, iterations), "evpgpu_set_anchor_interval")

@@ -201,6 +201,48 @@ def test_loop_parity_16_pores(job_factory):
     _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
 
 
+@pytest.mark.parametrize("n,prec", [(16, lib.FP64), (32, lib.FP64), (32, lib.MIXED)])
+def test_fused_transforms_match_seven_passes(job_factory, n, prec):
+    """The three fused FFT/Green passes against the seven-pass path (the one
+    the FFT tests pin to numpy): same Green iteration counts, states equal to
+    rounding."""
+    inp = job_factory(f"fused{n}", n=n, grains=8, seed=19, increments=2, pore_fraction=0.02 if n == 32 else 0.0)
+    a = lib.GpuJob(inp, prec)
+    b = lib.GpuJob(inp, prec)
+    b.set_fused_transforms(False)
+    ia, _ = a.run(2)
+    ib, _ = b.run(2)
+    assert list(ia) == list(ib)
+    A, B = a.state(), b.state()
+    tol = 1e-10 if prec == lib.FP64 else 1e-6
+    for k in ("Sig", "Edot", "DisGrad", "TranMat", "crss"):
+        assert rel(A[k], B[k]) < tol, (k, rel(A[k], B[k]))
+    assert rel(a.macro()["sig_avg"], b.macro()["sig_avg"]) < tol
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("prec", [lib.FP64, lib.MIXED])
+def test_kernel_timing_groups(job_factory, fused, prec):
+    """With HIP-event timing on, every kernel group of the path in use is
+    counted once per Green iteration and no timer pair left unrecorded is
+    queried (that would leave a HIP error for the next launch check)."""
+    inp = job_factory("timing16", n=16, grains=8, seed=23, increments=1)
+    job = lib.GpuJob(inp, prec)
+    job.set_fused_transforms(fused)
+    job.set_timing(True)
+    it, _ = job.run(1)
+    t = [job.timing(k) for k in range(6)]
+    for k in (1, 2, 3):
+        assert t[k][1] == t[0][1] - 1, (k, t)   # Newton also replays its last launch
+        assert t[k][0] > 0.0
+    assert t[0][1] == it[0] + 1
+    assert t[4][1] == 2
+    if prec == lib.MIXED:
+        assert t[5][1] >= 1
+    else:
+        assert t[5][1] == 0
+
+
 def test_newton_shortcut_is_exact(job_factory):
     """Ending certainly-rejected Newton iterations early changes nothing: the
     whole state after 3 increments is bit-identical with the shortcut off."""
