@@ -39,6 +39,10 @@ __device__ __forceinline__ double ipow(double x, int m)
     return r;
 }
 
+// Rate-law evaluation: RM_POW pow() for a non-integral exponent, RM_LOOP
+// repeated squaring for an integral one
+enum { RM_POW = 0, RM_LOOP = 1 };
+
 // chg_basis opt 2 for a symmetric tensor given as (11 22 33 23 13 12)
 __device__ __forceinline__ void sym_to_v6(const double t[6], double v[6])
 {
@@ -75,7 +79,7 @@ struct RateIn {
 // systems: each system's Schmid vector is read from LDS once and feeds the
 // resolved shear, the rate, edot and d(edot)/d(sigma).  The sums run over the
 // systems in ascending order, as the reference's separate loops do.
-template <bool DD, bool INTN>
+template <bool DD, int RM>
 __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn &in,
                                       const double s[6], double ed[5], double dd[5][5],
                                       double gd[EVPGPU_NSYSMX])
@@ -99,7 +103,7 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
             double itau = (r < 0) ? in.itb[k] : in.itf[k];
             r = r * itau;
             double a = fabs(r), p1, p2;
-            if (INTN) {
+            if (RM == RM_LOOP) {
                 p1 = ipow(a, intn - 1);
                 p2 = p1 * a;
             } else {
@@ -201,13 +205,13 @@ __device__ __forceinline__ void residual(const NewtonParams &P, const Vox &V, co
     }
 }
 
-template <bool INTN>
+template <int RM>
 __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const RateIn &in,
                                              const Vox &V, const double sig[6],
                                              double gd[EVPGPU_NSYSMX])
 {
     double ed[5], dd[5][5], t[6], res[6];
-    srate<false, INTN>(P, p, in, sig, ed, dd, gd);
+    srate<false, RM>(P, p, in, sig, ed, dd, gd);
     residual(P, V, sig, ed, t, res);
     double f = 0.0;
 #pragma unroll
@@ -225,7 +229,7 @@ __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const
 // counts exactly as computed here.  Only gamdot (from the rejected trial
 // point) differs; the loop recomputes the last Newton launch of an increment
 // with FAST off before gamdot is used.
-template <bool INTN, bool FAST>
+template <int RM, bool FAST>
 __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, const Vox &V,
                             double sig6[6], double edotp[6], double gd[EVPGPU_NSYSMX],
                             double &conv_s, double &conv_e, int &iterl, int &trials)
@@ -242,7 +246,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
         iterl++;
 #pragma unroll
         for (int i = 0; i < 6; i++) old[i] = sig6[i];
-        srate<true, INTN>(P, p, in, sig6, ed, dd, gd);
+        srate<true, RM>(P, p, in, sig6, ed, dd, gd);
 #pragma unroll
         for (int i = 0; i < 6; i++) edotp[i] = i < 5 ? ed[i] : 0.0;
         double gradf[6], step[6];
@@ -331,7 +335,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
         while (true) {
 #pragma unroll
             for (int i = 0; i < 6; i++) sig6[i] = old[i] + alam * step[i];
-            double f_new = nr_quality<INTN>(P, p, in, V, sig6, gd);
+            double f_new = nr_quality<RM>(P, p, in, V, sig6, gd);
             trials++;
             if (alam < alamin) {
 #pragma unroll
@@ -386,7 +390,7 @@ __device__ __forceinline__ double wave_sum(double v)
 // One lane per voxel.  Writes sig (in place), edot, gamdot; block partials
 // (NRED doubles per block) of err_s, err_e, Newton iterations, sum sig,
 // sum sig over phase 1.
-template <bool INTN, bool CSYM, bool FAST>
+template <int RM, bool CSYM, bool FAST>
 __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_newton(
     NewtonParams P, long long N, const int *__restrict__ phase, const int *__restrict__ orient,
     const double *__restrict__ sgr, const double *sig_in, double *sig, const double *__restrict__ E,
@@ -396,13 +400,28 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
 {
     __shared__ double slab[EVPGPU_NSYSMX * 5 * NB_THREADS];
     long long v = (long long)blockIdx.x * NB_THREADS + threadIdx.x;
-    double red[NRED];
-#pragma unroll
-    for (int i = 0; i < NRED; i++) red[i] = 0.0;
+    // this lane's err_s, err_e, Newton iterations and line-search trials
+    double r_cs = 0.0, r_ce = 0.0, r_it = 0.0, r_tr = 0.0;
+    int pl = 0;
     if (v < N) {
-        int p = phase[v] - 1;
-        p = p < 0 ? 0 : (p > 1 ? 1 : p);
-        if (!P.is_gas[p]) {
+        pl = phase[v] - 1;
+        pl = pl < 0 ? 0 : (pl > 1 ? 1 : pl);
+    }
+    // Crystal voxels, one phase of the wave at a time: inside the loop the
+    // phase index p is wave-uniform, so the rate-law constants, slip-system
+    // counts and crystal axes are scalar operands and the slip-system loops
+    // branch on scalars (a wave holds one crystal phase almost always).
+    bool pending = v < N && !P.is_gas[pl];
+    // every pass clears at least the lead lane, so it ends after one pass
+    // per distinct phase (two at most)
+    for (unsigned long long todo = __ballot(pending); todo; todo = __ballot(pending)) {
+        const int p = __builtin_amdgcn_readlane(pl, __ffsll(todo) - 1);
+        // lanes of phase p, tested through the ballot mask: a branch on
+        // pl == p itself would let the compiler substitute the per-lane pl
+        // for the uniform p inside the branch
+        const unsigned long long sel = __ballot(pending && pl == p);
+        if ((sel >> (threadIdx.x & 63)) & 1ull) {
+            pending = false;
             Vox V;
             RateIn in;
             double t[6];
@@ -464,7 +483,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             }
             double s6[6], e6[6], gd[EVPGPU_NSYSMX], cs, ce;
             int it, ntr;
-            int st = update_voxel<INTN, FAST>(P, p, in, V, s6, e6, gd, cs, ce, it, ntr);
+            int st = update_voxel<RM, FAST>(P, p, in, V, s6, e6, gd, cs, ce, it, ntr);
             if (st) {
                 atomicCAS(status, 0, st);
             } else {
@@ -477,17 +496,27 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
 #pragma unroll
                 for (int s = 0; s < EVPGPU_NSYSMX; s++)
                     if (s < ns) gamdot[s * N + v] = gd[s];
-                red[0] = cs;
-                red[1] = ce;
-                red[2] = (double)it;
-                red[15] = (double)ntr;
+                r_cs = cs;
+                r_ce = ce;
+                r_it = (double)it;
+                r_tr = (double)ntr;
                 if (err_vox) {
                     err_vox[2 * v] = cs;
                     err_vox[2 * v + 1] = ce;
                 }
                 if (niter_vox) niter_vox[v] = it;
             }
-        } else {
+        }
+    }
+    double red[NRED];
+#pragma unroll
+    for (int i = 0; i < NRED; i++) red[i] = 0.0;
+    red[0] = r_cs;
+    red[1] = r_ce;
+    red[2] = r_it;
+    red[15] = r_tr;
+    if (v < N) {
+        if (P.is_gas[pl]) {
 #pragma unroll
             for (int c = 0; c < 6; c++) {
                 sig[c * N + v] = 0.0;
@@ -561,17 +590,19 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
 #define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig_in, sig, E, \
                     epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
 #define LAUNCH(A, B, F) hipLaunchKernelGGL((k_newton<A, B, F>), NEWTON_ARGS)
+#define LAUNCH_RM(B, F)                                                                  \
+    do {                                                                                 \
+        if (intn) LAUNCH(RM_LOOP, B, F);                                                 \
+        else LAUNCH(RM_POW, B, F);                                                       \
+    } while (0)
     if (fast) {
-        if (intn && P.crss_sym) LAUNCH(true, true, true);
-        else if (intn) LAUNCH(true, false, true);
-        else if (P.crss_sym) LAUNCH(false, true, true);
-        else LAUNCH(false, false, true);
+        if (P.crss_sym) LAUNCH_RM(true, true);
+        else LAUNCH_RM(false, true);
     } else {
-        if (intn && P.crss_sym) LAUNCH(true, true, false);
-        else if (intn) LAUNCH(true, false, false);
-        else if (P.crss_sym) LAUNCH(false, true, false);
-        else LAUNCH(false, false, false);
+        if (P.crss_sym) LAUNCH_RM(true, false);
+        else LAUNCH_RM(false, false);
     }
+#undef LAUNCH_RM
 #undef LAUNCH
 #undef NEWTON_ARGS
     if (nblk_out) *nblk_out = nblk;

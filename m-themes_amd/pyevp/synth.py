@@ -95,6 +95,9 @@ class Config:
     stress_bc_flag: tuple = (1, 1, 0, 0, 0, 0)
     stress_bc: tuple = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
     single_orientation: Optional[tuple] = None  # (phi1, Phi, phi2) for every grain
+    # a second crystalline phase (own .sx file and elastic constants) on the
+    # even-numbered grains; excludes pores (the reference has two phases)
+    phase2: Optional[Material] = None
 
 
 # The configurations the loop is measured and checked on (sizes, grains, loading).
@@ -214,6 +217,10 @@ def microstructure(cfg: Config):
     gas = pores(n, cfg.pore_fraction, cfg.pore_radius, rng)
     grain = (lab + 1).astype(np.int32)
     phase = np.ones(n ** 3, dtype=np.int32)
+    if cfg.phase2 is not None:
+        if cfg.pore_fraction > 0.0:
+            raise ValueError("a second crystalline phase and pores need three phases")
+        phase[grain % 2 == 0] = 2
     grain[gas] = cfg.grains + 1
     phase[gas] = 2
     euler = eul[lab]
@@ -277,17 +284,19 @@ def write_sx(path: str, m: Material):
 def write_input(path: str, cfg: Config, sx_name="Cu.sx", ms_name="ms.txt"):
     """Namelist in the layout of test_run/input.in (power-law keys only)."""
     m = cfg.material
-    two = cfg.pore_fraction > 0.0
+    m2 = cfg.phase2
+    two = cfg.pore_fraction > 0.0 or m2 is not None
     lines = [
         "# synthetic EVPFFT job (seed 0x%X)" % cfg.seed,
         "N_phases\t%d" % (2 if two else 1),
         "CellDim\t%d %d %d" % (cfg.n, cfg.n, cfg.n),
         "initial_ms\t%s" % ms_name,
-        "Type_phases\t0\t1",
+        "Type_phases\t0\t%d" % (0 if m2 is not None else 1),
         "ElastConst_PhaseI\t%r\t%r\t%r\t%r" % (m.c11, m.c12, m.c44, m.nu),
         "Slip_PhaseI\t%s" % sx_name,
-        "ElastConst_PhaseII\t0.0\t0.0\t0.0\t0.0",
-        "Slip_PhaseII\t%s" % sx_name,
+        ("ElastConst_PhaseII\t%r\t%r\t%r\t%r" % (m2.c11, m2.c12, m2.c44, m2.nu) if m2 is not None
+         else "ElastConst_PhaseII\t0.0\t0.0\t0.0\t0.0"),
+        "Slip_PhaseII\t%s" % ("P2.sx" if m2 is not None else sx_name),
         "VelGrad_BC_Flag\t" + " ".join(str(int(v)) for v in cfg.velgrad_bc_flag),
         "VelGrad_BC\t" + " ".join(repr(float(v)) for v in cfg.velgrad_bc),
         "Stress_BC_Flag\t" + " ".join(str(int(v)) for v in cfg.stress_bc_flag),
@@ -312,6 +321,8 @@ def write_job(dirpath: str, cfg: Config) -> str:
     euler, grain, phase = microstructure(cfg)
     write_ms(os.path.join(dirpath, "ms.txt"), cfg.n, euler, grain, phase)
     write_sx(os.path.join(dirpath, "Cu.sx"), cfg.material)
+    if cfg.phase2 is not None:
+        write_sx(os.path.join(dirpath, "P2.sx"), cfg.phase2)
     inp = os.path.join(dirpath, "input.in")
     write_input(inp, cfg)
     return inp

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from pyevp import lib
+from pyevp import lib, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -198,6 +198,17 @@ def test_loop_parity_8(job_factory):
 def test_loop_parity_16_pores(job_factory):
     inp = job_factory("loop16p", n=16, grains=8, seed=9, increments=3, pore_fraction=0.03,
                       pore_radius=2)
+    _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
+
+
+@pytest.mark.parametrize("nrs2", [20.0, 12.5])
+def test_loop_parity_two_crystal_phases(job_factory, nrs2):
+    """Two crystalline phases on alternating grains: waves that straddle a
+    phase boundary run the Newton update once per phase (n = 12.5 also takes
+    the non-integral rate exponent path)."""
+    m2 = synth.Material(c11=246500.0, c12=147300.0, c44=124700.0, nrs=nrs2, gam0=0.5,
+                        tau0f=30.0, tau0b=30.0, tau1=15.0, thet0=300.0, thet1=30.0)
+    inp = job_factory(f"loop2ph{nrs2}", n=16, grains=8, seed=29, increments=3, phase2=m2)
     _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
 
 

@@ -50,3 +50,22 @@ def test_job_files_round_trip_through_the_oracle_reader(tmp_path):
     u = rng.uniforms(3 * big.grains).reshape(-1, 3)
     c = 2.0 * u[:, 1] - 1.0
     assert abs(c.mean()) < 0.05 and abs((c ** 2).mean() - 1 / 3) < 0.03
+
+
+def test_two_crystal_phases_reach_the_oracle(tmp_path):
+    """The second crystalline phase (even grains) is read with its own elastic
+    constants and slip file, and its rate exponent changes the response."""
+    m2 = synth.Material(c11=246500.0, c12=147300.0, c44=124700.0, nrs=20.0, tau0f=30.0,
+                        tau0b=30.0)
+    cfg = synth.Config(n=8, grains=6, seed=31, increments=1, phase2=m2)
+    inp = synth.write_job(str(tmp_path), cfg)
+    _, grain, phase = synth.microstructure(cfg)
+    assert set(np.unique(phase)) == {1, 2}
+    assert np.array_equal(phase == 2, grain % 2 == 0)
+    J = O.Job(inp)
+    assert np.array_equal(J.phase(), phase)
+    J.run(1)
+    one = synth.write_job(str(tmp_path / "one"), synth.Config(n=8, grains=6, seed=31, increments=1))
+    J1 = O.Job(one)
+    J1.run(1)
+    assert abs(J.macro()["SigAvg"][2, 2] - J1.macro()["SigAvg"][2, 2]) > 1e-3 * abs(J1.macro()["SigAvg"][2, 2])
