@@ -189,7 +189,13 @@ NewtonParams newton_params(const evpgpu_ctx *c)
 {
     NewtonParams P;
     memset(&P, 0, sizeof(P));
-    memcpy(P.c066, c->ld.c066, sizeof(P.c066));
+    // C0 and S0 are symmetric: their upper triangles (S from the
+    // per-orientation table is taken the same way)
+    for (int i = 0; i < 6; i++)
+        for (int j = i; j < 6; j++) {
+            P.c0p[PK(i, j)] = c->ld.c066[i * 6 + j];
+            P.s0p[PK(i, j)] = c->ld.s066[i * 6 + j];
+        }
     P.dt = c->ld.dt;
     for (int p = 0; p < 2; p++) {
         if (p >= c->nphases) {
@@ -213,7 +219,6 @@ NewtonParams newton_params(const evpgpu_ctx *c)
     P.crss_sym = c->crss_sym;
     P.nr_tol = c->nr_tol;
     P.tolx = c->tolx;
-    memcpy(P.s066, c->ld.s066, sizeof(P.s066));
     P.kmin_tolx = c->lam_min > 0.0 ? 0.99 * c->tolx * c->lam_min : 0.0;
     return P;
 }

@@ -22,8 +22,14 @@
 // ---------------------------------------------------------------- params --
 // Per-launch constants of the per-voxel stress update (passed by value: they
 // land in SGPRs / the kernel-argument segment, uniform across the wave).
+// Packed upper triangle of a symmetric 6x6 matrix: PK(i, j), i <= j.
+__host__ __device__ constexpr int PK(int i, int j)
+{
+    return i <= j ? i * 6 - i * (i - 1) / 2 + (j - i) : j * 6 - j * (j - 1) / 2 + (i - j);
+}
+
 struct NewtonParams {
-    double c066[36];
+    double c0p[21];  // C0 (B-basis), packed upper triangle (PK)
     double dt;
     double nrs[2][EVPGPU_NSYSMX];
     double gam0[2][EVPGPU_NSYSMX];
@@ -32,7 +38,8 @@ struct NewtonParams {
     int int_n[2];    // integer rate exponent of the phase, 0 if not integral
     int crss_sym;    // 1: forward and backward CRSS are equal in every voxel
     double bn[2][EVPGPU_NSYSMX][3], bb[2][EVPGPU_NSYSMX][3]; // crystal-axis normals, directions
-    double s066[36]; // C0^-1 (B-basis): the Newton step is solved in S0-scaled form
+    double s0p[21];  // S0 = C0^-1 (B-basis), packed upper triangle: the Newton step is
+                     // solved in S0-scaled form
     double kmin_tolx; // 0.99 TOLX min over orientations of lambda_min(S0 + S)
     double nr_tol;   // Newton tolerance on the relative step (reference 1e-10)
     double tolx;     // line-search TOLX (reference 1e-7)

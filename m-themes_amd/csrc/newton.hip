@@ -117,9 +117,11 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
             if (DD) {
                 double r1 = P.gam0[p][k] * P.nrs[p][k] * p1 * itau;
 #pragma unroll
-                for (int i = 0; i < 5; i++)
+                for (int i = 0; i < 5; i++) {
+                    const double w = sc[i] * r1;
 #pragma unroll
-                    for (int j = i; j < 5; j++) dd[i][j] += sc[i] * sc[j] * r1;
+                    for (int j = i; j < 5; j++) dd[i][j] += w * sc[j];
+                }
             }
         }
     }
@@ -131,10 +133,18 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
     }
 }
 
-// Packed upper triangle of a symmetric 6x6 matrix: PK(i, j), i <= j.
-__host__ __device__ constexpr int PK(int i, int j)
+// 1/sqrt(d) for a positive normal d: the hardware estimate refined by two
+// Newton steps y += y (1 - d y^2) / 2 (the estimate's relative error e
+// becomes about 1.5 e^2 per step), to within about an ulp
+__device__ __forceinline__ double rsqrt_refined(double d)
 {
-    return i <= j ? i * 6 - i * (i - 1) / 2 + (j - i) : j * 6 - j * (j - 1) / 2 + (i - j);
+    double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const double e = fma(-(d * y), y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
 }
 
 // Cholesky factorisation of the packed symmetric 6x6 k (in place: the lower
@@ -149,8 +159,7 @@ __device__ __forceinline__ bool chol6_solve(double k[21], double b[6])
 #pragma unroll
         for (int q = 0; q < j; q++) d -= k[PK(q, j)] * k[PK(q, j)];
         if (!(d > 0.0)) return false;
-        d = sqrt(d);
-        inv[j] = 1.0 / d;
+        inv[j] = rsqrt_refined(d);
 #pragma unroll
         for (int i = j + 1; i < 6; i++) {
             double s = k[PK(j, i)];
@@ -200,7 +209,7 @@ __device__ __forceinline__ void residual(const NewtonParams &P, const Vox &V, co
     for (int i = 0; i < 6; i++) {
         double r = sig[i] - V.xl[i];
 #pragma unroll
-        for (int j = 0; j < 6; j++) r += P.c066[i * 6 + j] * t[j];
+        for (int j = 0; j < 6; j++) r += P.c0p[PK(i, j)] * t[j];
         res[i] = r;
     }
 }
@@ -258,16 +267,19 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             for (int i = 0; i < 6; i++) ce += t[i] * t[i];
             conv_e = sqrt(ce);
         }
+        // S0 R: the right-hand side of the step, and its norm bounds the step
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double b = 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; j++) b += P.s0p[PK(i, j)] * res[j];
+            step[i] = b;
+        }
         bool skip = false;
         if (FAST) {
             double y2 = 0.0;
 #pragma unroll
-            for (int i = 0; i < 6; i++) {
-                double a = 0.0;
-#pragma unroll
-                for (int j = 0; j < 6; j++) a += P.s066[i * 6 + j] * res[j];
-                y2 += a * a;
-            }
+            for (int i = 0; i < 6; i++) y2 += step[i] * step[i];
             skip = (y2 > 0.0) && (sqrt(y2) < P.kmin_tolx);
         }
         if (skip) {
@@ -283,20 +295,16 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             for (int i = 0; i < 6; i++)
 #pragma unroll
                 for (int j = i; j < 6; j++) {
-                    double kij = P.s066[i * 6 + j] + V.S[PK(i, j)];
+                    double kij = P.s0p[PK(i, j)] + V.S[PK(i, j)];
                     if (i < 5 && j < 5) kij += dd[i][j] * P.dt;
                     k[PK(i, j)] = kij;
                 }
 #pragma unroll
             for (int i = 0; i < 6; i++) {
-                double a = 0.0, b = 0.0;
+                double a = 0.0;
 #pragma unroll
-                for (int j = 0; j < 6; j++) {
-                    a += P.c066[i * 6 + j] * res[j];
-                    b += P.s066[i * 6 + j] * res[j];
-                }
+                for (int j = 0; j < 6; j++) a += P.c0p[PK(i, j)] * res[j];
                 u[i] = a;
-                step[i] = b;
             }
 #pragma unroll
             for (int i = 0; i < 6; i++) {
