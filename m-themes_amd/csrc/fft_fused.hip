@@ -7,7 +7,7 @@
 //              length-nz/2 complex FFT of the even/odd samples plus the
 //              split step), then along y for every kz, and written once as
 //              the ny x (nz/2+1) half-spectrum plane;
-//   k_x_green  one workgroup per (y, run of TBX kz values): all 6 components
+//   k_x_green  one workgroup per run of TBX (y, kz) columns: all 6 components
 //              of those lines are transformed along x, the Green operator is
 //              applied at each wave vector (green_point.h, with the
 //              antisymmetric accumulator update), and transformed back;
@@ -197,13 +197,17 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
     __shared__ C lds[NL * PX];
     __shared__ R c0s[81];
     if (threadIdx.x < 81) c0s[threadIdx.x] = (R)G.c0[threadIdx.x];
-    const int kz0 = blockIdx.x * TBX, y = blockIdx.y;
-    const int nk = min(TBX, nzh - kz0);
-    // line (c, j) = component c, kz = kz0 + j, along x; element x at spec[c*NK + (x*ny + y)*nzh + kz]
+    // The x lines are the columns q = y*nzh + kz of the [x][y][kz] half
+    // spectrum; a workgroup takes TBX consecutive columns q0.. (runs that
+    // start on 128-byte boundaries, as ny*nzh is a multiple of TBX).
+    // Line (c, j) = component c, column q0 + j; element x at spec[c*NK + x*ncol + q0 + j].
+    const int ncol = ny * nzh;
+    const int q0 = blockIdx.x * TBX;
+    const int nk = min(TBX, ncol - q0);
     for (int i = threadIdx.x; i < NL * NX; i += NTX) {
         const int c = i / (TBX * NX), r = i % (TBX * NX), x = r / TBX, j = r % TBX;
         C v = C{(R)0, (R)0};
-        if (j < nk) v = spec[c * NK + ((long long)x * ny + y) * nzh + kz0 + j];
+        if (j < nk) v = spec[c * NK + (long long)x * ncol + q0 + j];
         lds[(c * TBX + j) * PX + x] = v;
     }
     __syncthreads();
@@ -219,10 +223,11 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
         double wr[3], wi[3];
 #pragma unroll
         for (int c = 0; c < 6; c++) s6[c] = lds[(c * TBX + j) * PX + x];
-        gop::green_point<R>(c0s, G.s0, G.elast_bc, x, y, kz0 + j, NX, ny, nz, s6, e6, wr, wi);
+        const int q = q0 + j, y = q / nzh, kz = q - y * nzh;
+        gop::green_point<R>(c0s, G.s0, G.elast_bc, x, y, kz, NX, ny, nz, s6, e6, wr, wi);
 #pragma unroll
         for (int c = 0; c < 6; c++) lds[(c * TBX + j) * PX + x] = e6[c];
-        const long long id = ((long long)x * ny + y) * nzh + kz0 + j;
+        const long long id = (long long)x * ncol + q;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             if (GMODE == 3) {
@@ -243,7 +248,7 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
     gstages<true, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
     for (int i = threadIdx.x; i < NL * NX; i += NTX) {
         const int c = i / (TBX * NX), r = i % (TBX * NX), x = r / TBX, j = r % TBX;
-        if (j < nk) spec[c * NK + ((long long)x * ny + y) * nzh + kz0 + j] = lds[(c * TBX + j) * PX + x];
+        if (j < nk) spec[c * NK + (long long)x * ncol + q0 + j] = lds[(c * TBX + j) * PX + x];
     }
 }
 
@@ -320,7 +325,7 @@ int xg(evpgpu_ctx *c)
 #define CALLL(NX_)                                                                                        \
     {                                                                                                    \
         constexpr int T = tbx<R, NX_>();                                                                 \
-        hipLaunchKernelGGL((k_x_green<R, NX_, T, GMODE>), dim3((c->nzh + T - 1) / T, c->ny), dim3(NTX), 0, \
+        hipLaunchKernelGGL((k_x_green<R, NX_, T, GMODE>), dim3((c->ny * c->nzh + T - 1) / T), dim3(NTX), 0,  \
                            c->stream, (C *)c->spec, c->acc, c->skA, G, c->ny, c->nz, c->nzh, c->NK,      \
                            twp<R>(c, 0));                                                                \
     }

@@ -39,9 +39,28 @@ __device__ __forceinline__ double ipow(double x, int m)
     return r;
 }
 
+// x^m for a wave-uniform 0 <= m < 32: the products of ipow in the same
+// order, straight-line (a scalar-masked select per exponent bit instead of a
+// scalar loop: with one wave per SIMD every scalar instruction and branch
+// costs an issue slot)
+__device__ __forceinline__ double ipow5(double x, int m)
+{
+    double r = 1.0, b = x;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const double t = r * b;
+        r = ((m >> i) & 1) ? t : r;
+        if (i < 4) b *= b;
+    }
+    return r;
+}
+
 // Rate-law evaluation: RM_POW pow() for a non-integral exponent, RM_LOOP
-// repeated squaring for an integral one
-enum { RM_POW = 0, RM_LOOP = 1 };
+// repeated squaring for an integral one, RM_FAST the common case of every
+// crystal phase having EVPGPU_NSYSMX systems with one integral n <= 32 and
+// one reference rate (constants hoisted out of the system loop, no
+// per-system branch, straight-line squaring)
+enum { RM_POW = 0, RM_LOOP = 1, RM_FAST = 2 };
 
 // chg_basis opt 2 for a symmetric tensor given as (11 22 33 23 13 12)
 __device__ __forceinline__ void sym_to_v6(const double t[6], double v[6])
@@ -86,6 +105,9 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
 {
     const int ns = P.nsys[p];
     const int intn = P.int_n[p];
+    // RM_FAST: the phase's rate constants, as the general path multiplies
+    // them ((gam0 n) p1) itau
+    const double g0 = P.gam0[p][0], g0n = P.gam0[p][0] * P.nrs[p][0];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
         ed[i] = 0.;
@@ -103,19 +125,22 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
             double itau = (r < 0) ? in.itb[k] : in.itf[k];
             r = r * itau;
             double a = fabs(r), p1, p2;
-            if (RM == RM_LOOP) {
+            if (RM == RM_FAST) {
+                p1 = ipow5(a, intn - 1);
+                p2 = p1 * a;
+            } else if (RM == RM_LOOP) {
                 p1 = ipow(a, intn - 1);
                 p2 = p1 * a;
             } else {
                 p1 = pow(a, P.nrs[p][k] - 1.);
                 p2 = pow(a, P.nrs[p][k]);
             }
-            double r2 = P.gam0[p][k] * p2 * (double)(2 * (r > 0) - 1);
+            double r2 = (RM == RM_FAST ? g0 : P.gam0[p][k]) * p2 * (double)(2 * (r > 0) - 1);
             gd[k] = r2;
 #pragma unroll
             for (int i = 0; i < 5; i++) ed[i] += sc[i] * r2;
             if (DD) {
-                double r1 = P.gam0[p][k] * P.nrs[p][k] * p1 * itau;
+                double r1 = (RM == RM_FAST ? g0n : P.gam0[p][k] * P.nrs[p][k]) * p1 * itau;
 #pragma unroll
                 for (int i = 0; i < 5; i++) {
                     const double w = sc[i] * r1;
@@ -591,16 +616,25 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
                   int *niter_vox, int fast)
 {
     int nblk = (int)((n + NB_THREADS - 1) / NB_THREADS);
-    // the integral-exponent variant needs every crystal phase to have one
-    bool intn = true;
-    for (int q = 0; q < 2; q++)
-        if (!P.is_gas[q] && P.int_n[q] == 0) intn = false;
+    // the integral-exponent variants need every crystal phase to have one;
+    // the fast one also n <= 32, a full set of systems and one gam0
+    bool intn = true, fastrm = true;
+    for (int q = 0; q < 2; q++) {
+        if (P.is_gas[q]) continue;
+        if (P.int_n[q] == 0) intn = false;
+        bool one = P.int_n[q] >= 1 && P.int_n[q] <= 32 && P.nsys[q] == EVPGPU_NSYSMX;
+        for (int k = 0; k < P.nsys[q]; k++)
+            if (P.gam0[q][k] != P.gam0[q][0] || P.nrs[q][k] != P.nrs[q][0]) one = false;
+        if (!one) fastrm = false;
+    }
+    const int rm = !intn ? RM_POW : (fastrm ? RM_FAST : RM_LOOP);
 #define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig_in, sig, E, \
                     epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
 #define LAUNCH(A, B, F) hipLaunchKernelGGL((k_newton<A, B, F>), NEWTON_ARGS)
 #define LAUNCH_RM(B, F)                                                                  \
     do {                                                                                 \
-        if (intn) LAUNCH(RM_LOOP, B, F);                                                 \
+        if (rm == RM_FAST) LAUNCH(RM_FAST, B, F);                                        \
+        else if (rm == RM_LOOP) LAUNCH(RM_LOOP, B, F);                                   \
         else LAUNCH(RM_POW, B, F);                                                       \
     } while (0)
     if (fast) {
