@@ -90,6 +90,7 @@ constexpr int NB_THREADS = 64; // one wave per block: the LDS slab is wave-priva
 
 struct RateIn {
     const double *sc;                              // LDS slab, sc[(s*5+j)*64]
+    const double *k0;                              // LDS: C0 then S0, packed triangles
     double itf[EVPGPU_NSYSMX], itb[EVPGPU_NSYSMX]; // 1/CRSS forward, backward
 };
 #define SC(i, j) in.sc[((i)*5 + (j)) * NB_THREADS]
@@ -98,9 +99,11 @@ struct RateIn {
 // systems: each system's Schmid vector is read from LDS once and feeds the
 // resolved shear, the rate, edot and d(edot)/d(sigma).  The sums run over the
 // systems in ascending order, as the reference's separate loops do.
-template <bool DD, int RM>
+// DD: dt d(edot)/d(sigma) is added into the packed Newton matrix kk (which
+// holds S0 + S on entry).  GD: the slip rates are kept (gamdot).
+template <bool DD, int RM, bool GD>
 __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn &in,
-                                      const double s[6], double ed[5], double dd[5][5],
+                                      const double s[6], double ed[5], double kk[21],
                                       double gd[EVPGPU_NSYSMX])
 {
     const int ns = P.nsys[p];
@@ -109,12 +112,7 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
     // them ((gam0 n) p1) itau
     const double g0 = P.gam0[p][0], g0n = P.gam0[p][0] * P.nrs[p][0];
 #pragma unroll
-    for (int i = 0; i < 5; i++) {
-        ed[i] = 0.;
-        if (DD)
-#pragma unroll
-            for (int j = i; j < 5; j++) dd[i][j] = 0.0;
-    }
+    for (int i = 0; i < 5; i++) ed[i] = 0.;
 #pragma unroll
     for (int k = 0; k < EVPGPU_NSYSMX; k++) {
         if (k < ns) {
@@ -136,25 +134,20 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
                 p2 = pow(a, P.nrs[p][k]);
             }
             double r2 = (RM == RM_FAST ? g0 : P.gam0[p][k]) * p2 * (double)(2 * (r > 0) - 1);
-            gd[k] = r2;
+            if (GD) gd[k] = r2;
 #pragma unroll
             for (int i = 0; i < 5; i++) ed[i] += sc[i] * r2;
             if (DD) {
                 double r1 = (RM == RM_FAST ? g0n : P.gam0[p][k] * P.nrs[p][k]) * p1 * itau;
+                r1 *= P.dt;
 #pragma unroll
                 for (int i = 0; i < 5; i++) {
                     const double w = sc[i] * r1;
 #pragma unroll
-                    for (int j = i; j < 5; j++) dd[i][j] += w * sc[j];
+                    for (int j = i; j < 5; j++) kk[PK(i, j)] += w * sc[j];
                 }
             }
         }
-    }
-    if (DD) {
-#pragma unroll
-        for (int i = 0; i < 5; i++)
-#pragma unroll
-            for (int j = 0; j < i; j++) dd[i][j] = dd[j][i];
     }
 }
 
@@ -220,7 +213,8 @@ struct Vox {
 
 // R(s) = s - lambda + C0 t,  t = eps_p + dt edot(s) + S s - E
 // (src/evolution.c:126-140; t is the strain mismatch of the error norm)
-__device__ __forceinline__ void residual(const NewtonParams &P, const Vox &V, const double sig[6],
+__device__ __forceinline__ void residual(const NewtonParams &P, const RateIn &in, const Vox &V,
+                                         const double sig[6],
                                          const double ed[5], double t[6], double res[6])
 {
 #pragma unroll
@@ -234,19 +228,19 @@ __device__ __forceinline__ void residual(const NewtonParams &P, const Vox &V, co
     for (int i = 0; i < 6; i++) {
         double r = sig[i] - V.xl[i];
 #pragma unroll
-        for (int j = 0; j < 6; j++) r += P.c0p[PK(i, j)] * t[j];
+        for (int j = 0; j < 6; j++) r += in.k0[PK(i, j)] * t[j];
         res[i] = r;
     }
 }
 
-template <int RM>
+template <int RM, bool GD>
 __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const RateIn &in,
                                              const Vox &V, const double sig[6],
                                              double gd[EVPGPU_NSYSMX])
 {
-    double ed[5], dd[5][5], t[6], res[6];
-    srate<false, RM>(P, p, in, sig, ed, dd, gd);
-    residual(P, V, sig, ed, t, res);
+    double ed[5], t[6], res[6];
+    srate<false, RM, GD>(P, p, in, sig, ed, nullptr, gd);
+    residual(P, in, V, sig, ed, t, res);
     double f = 0.0;
 #pragma unroll
     for (int i = 0; i < 6; i++) f += res[i] * res[i];
@@ -262,14 +256,15 @@ __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const
 // and its first trial is rejected, leaving s, edot, the error norms and the
 // counts exactly as computed here.  Only gamdot (from the rejected trial
 // point) differs; the loop recomputes the last Newton launch of an increment
-// with FAST off before gamdot is used.
+// with FAST off before gamdot is used, so FAST launches do not form gamdot.
 template <int RM, bool FAST>
 __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, const Vox &V,
                             double sig6[6], double edotp[6], double gd[EVPGPU_NSYSMX],
                             double &conv_s, double &conv_e, int &iterl, int &trials)
 {
+    constexpr bool GD = !FAST;
     trials = 0;
-    double ed[5], dd[5][5], res[6], old[6];
+    double ed[5], res[6], old[6], k[21];
     conv_s = 0.0;
     conv_e = 0.0;
     iterl = 0;
@@ -277,16 +272,24 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
 #pragma unroll
     for (int i = 0; i < 6; i++) sig6[i] = V.xl[i];
     while ((iterl < 500) && (fabs(erral) > fabs(P.nr_tol))) {
+        // read C0, S0 and the Schmid vectors from LDS where they are used:
+        // hoisted out of the loop they would occupy 204 more registers
+        __asm__ volatile("" ::: "memory");
         iterl++;
 #pragma unroll
         for (int i = 0; i < 6; i++) old[i] = sig6[i];
-        srate<true, RM>(P, p, in, sig6, ed, dd, gd);
+        // the Newton matrix K = S0 + S + dt D (below), D added by srate
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = i; j < 6; j++) k[PK(i, j)] = in.k0[21 + PK(i, j)] + V.S[PK(i, j)];
+        srate<true, RM, GD>(P, p, in, sig6, ed, k, gd);
 #pragma unroll
         for (int i = 0; i < 6; i++) edotp[i] = i < 5 ? ed[i] : 0.0;
         double gradf[6], step[6];
         {
             double t[6];
-            residual(P, V, sig6, ed, t, res);
+            residual(P, in, V, sig6, ed, t, res);
             double ce = 0.0;
 #pragma unroll
             for (int i = 0; i < 6; i++) ce += t[i] * t[i];
@@ -297,7 +300,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
         for (int i = 0; i < 6; i++) {
             double b = 0.0;
 #pragma unroll
-            for (int j = 0; j < 6; j++) b += P.s0p[PK(i, j)] * res[j];
+            for (int j = 0; j < 6; j++) b += in.k0[21 + PK(i, j)] * res[j];
             step[i] = b;
         }
         bool skip = false;
@@ -315,20 +318,12 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             // symmetric positive definite K = S0 + S + dt D: the step solves
             // K step = -S0 R (Cholesky), and the merit-function gradient
             // J^T R = R + (S + dt D) C0 R equals K (C0 R).
-            double k[21], u[6];
-#pragma unroll
-            for (int i = 0; i < 6; i++)
-#pragma unroll
-                for (int j = i; j < 6; j++) {
-                    double kij = P.s0p[PK(i, j)] + V.S[PK(i, j)];
-                    if (i < 5 && j < 5) kij += dd[i][j] * P.dt;
-                    k[PK(i, j)] = kij;
-                }
+            double u[6];
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 double a = 0.0;
 #pragma unroll
-                for (int j = 0; j < 6; j++) a += P.c0p[PK(i, j)] * res[j];
+                for (int j = 0; j < 6; j++) a += in.k0[PK(i, j)] * res[j];
                 u[i] = a;
             }
 #pragma unroll
@@ -366,9 +361,10 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
         }
         double alamin = TOLX / test, alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0;
         while (true) {
+            __asm__ volatile("" ::: "memory");
 #pragma unroll
             for (int i = 0; i < 6; i++) sig6[i] = old[i] + alam * step[i];
-            double f_new = nr_quality<RM>(P, p, in, V, sig6, gd);
+            double f_new = nr_quality<RM, GD>(P, p, in, V, sig6, gd);
             trials++;
             if (alam < alamin) {
 #pragma unroll
@@ -432,6 +428,15 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
     int *__restrict__ status, double *__restrict__ err_vox, int *__restrict__ niter_vox)
 {
     __shared__ double slab[EVPGPU_NSYSMX * 5 * NB_THREADS];
+    // C0 and S0 (packed upper triangles) as broadcast LDS reads: kept in
+    // scalar registers they do not fit, and each use of a spilled one costs
+    // two lane reads
+    __shared__ double k0[42];
+    if (threadIdx.x < 21) {
+        k0[threadIdx.x] = P.c0p[threadIdx.x];
+        k0[21 + threadIdx.x] = P.s0p[threadIdx.x];
+    }
+    __syncthreads();
     long long v = (long long)blockIdx.x * NB_THREADS + threadIdx.x;
     // this lane's err_s, err_e, Newton iterations and line-search trials
     double r_cs = 0.0, r_ce = 0.0, r_it = 0.0, r_tr = 0.0;
@@ -487,6 +492,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
 #pragma unroll
             for (int q = 0; q < 9; q++) T[q / 3][q % 3] = tran[q * N + v];
             in.sc = slab + threadIdx.x;
+            in.k0 = k0;
             for (int s = 0; s < ns; s++) {
                 double tn[3], tb[3];
 #pragma unroll
@@ -526,9 +532,10 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
                 v6_to_sym(e6, t);
 #pragma unroll
                 for (int c = 0; c < 6; c++) edot[c * N + v] = t[c];
+                if (!FAST)
 #pragma unroll
-                for (int s = 0; s < EVPGPU_NSYSMX; s++)
-                    if (s < ns) gamdot[s * N + v] = gd[s];
+                    for (int s = 0; s < EVPGPU_NSYSMX; s++)
+                        if (s < ns) gamdot[s * N + v] = gd[s];
                 r_cs = cs;
                 r_ce = ce;
                 r_it = (double)it;

@@ -201,12 +201,14 @@ def test_loop_parity_16_pores(job_factory):
     _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
 
 
-@pytest.mark.parametrize("nrs2", [20.0, 12.5, 40.0])
+@pytest.mark.parametrize("nrs2", [20.0, 12.5, 34.0])
 def test_loop_parity_two_crystal_phases(job_factory, nrs2):
     """Two crystalline phases on alternating grains: waves that straddle a
     phase boundary run the Newton update once per phase.  The second phase's
     exponent selects the rate-law variant: n = 20 the fast one (integral
-    n <= 32, one rate per phase), n = 12.5 pow(), n = 40 the squaring loop."""
+    n <= 32, one rate per phase), n = 12.5 pow(), n = 34 the squaring loop
+    (on this job the reference's line search gives up, slope >= 0, from
+    n = 36 on)."""
     m2 = synth.Material(c11=246500.0, c12=147300.0, c44=124700.0, nrs=nrs2, gam0=0.5,
                         tau0f=30.0, tau0b=30.0, tau1=15.0, thet0=300.0, thet1=30.0)
     inp = job_factory(f"loop2ph{nrs2}", n=16, grains=8, seed=29, increments=3, phase2=m2)
