@@ -36,24 +36,32 @@ from pyevp import lib, synth  # noqa: E402
 METRIC = "voxel-iters/s at 128³, mixed vs FP64 GPU at 1e-6 err; adjoint/forward time"
 
 # FP64 operations of the per-voxel Newton kernel, counted by hand from
-# newton.hip (an FMA counts as 2):
+# newton.hip (an FMA counts as 2, a refined reciprocal square root as 10):
 #   per crystal voxel and launch: 12 Schmid tensors from the rotation
-#     (~57 each), three basis changes, |lambda|, 12 reciprocal CRSS, two
+#     (~55 each), three basis changes, |lambda|, 12 reciprocal CRSS, two
 #     back-conversions                                                   ~800
-#   per Newton iteration: rate law with the 5x5 derivative (12 x ~80),
-#     total strain (84), residual (78), Jacobian (864), gradient (72), LU
-#     solve with pivoting (~270), step control (~50)                    ~2380
-#   per line-search trial: rate law without derivative (12 x ~29), total
-#     strain, residual, merit function                                   ~534
+#   per Newton iteration: rate law with dt d(edot)/d(sigma) added into the
+#     packed Newton matrix (12 x ~67), S0 + S (21), strain mismatch and
+#     residual (168), S0 R, C0 R and the merit gradient K C0 R (3 x 72),
+#     6x6 Cholesky and the two triangular solves (~250), step control (~60)
+#                                                                       ~1520
+#   per line-search trial: rate law without derivative (12 x ~29), strain
+#     mismatch, residual, merit function, trial point                    ~540
 NEWTON_FLOPS_VOXEL = 800
-NEWTON_FLOPS_ITER = 2380
-NEWTON_FLOPS_TRIAL = 534
+NEWTON_FLOPS_ITER = 1520
+NEWTON_FLOPS_TRIAL = 540
 # HBM bytes of the Newton kernel per crystal voxel and launch: reads sigma,
 # strain, plastic strain (3 x 48), the rotation (72), CRSS (96: forward =
-# backward), phase + orientation index (8); writes sigma, edot (2 x 48) and
-# gamdot (96).  Gas voxels: phase (4), sigma/edot writes (96), sigma re-read (48).
-NEWTON_BYTES_CRYSTAL = 3 * 48 + 72 + 96 + 8 + 2 * 48 + 96
+# backward), phase + orientation index (8); writes sigma and edot (2 x 48).
+# (The one launch per increment with the shortcut off also writes gamdot,
+# 96 B.)  Gas voxels: phase (4), sigma/edot writes (96), sigma re-read (48).
+NEWTON_BYTES_CRYSTAL = 3 * 48 + 72 + 96 + 8 + 2 * 48
 NEWTON_BYTES_GAS = 4 + 96 + 48
+# k_newton's HBM traffic per launch on the 128^3 job (launches with the
+# shortcut on), measured: rocprofv3 FETCH_SIZE x 1024 x 2 + WRITE_SIZE x 1024
+# (FETCH_SIZE reads half and WRITE_SIZE all of a plain copy kernel's bytes);
+# 683.3 MB read + 205.5 MB written against 861 MB algorithmic.
+NEWTON_TRAFFIC_128 = 8.888e8
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFS = 78.6
 
@@ -228,16 +236,22 @@ def main():
     t_s = ms * 1e-3
     tf = flops / t_s / 1e12
     gbs = bytes_ / t_s / 1e9
+    # measured HBM bytes per launch of k_newton on this job (rocprofv3 FETCH_SIZE
+    # and WRITE_SIZE scaled by a copy kernel's readings), 128^3 workload only
+    traffic = NEWTON_TRAFFIC_128 if cfg.n == 128 else None
+    per = dict(newton_iters_per_voxel_launch=nit / max(n_cryst * launches, 1),
+               trials_per_voxel_launch=ntr / max(n_cryst * launches, 1),
+               flops_per_launch=flops / max(launches, 1), bytes_per_launch=bytes_ / max(launches, 1))
     if tf / PEAK_FP64_TFS >= gbs / PEAK_HBM_GBS:
         roof = dict(bound="mfma", achieved=tf, peak=PEAK_FP64_TFS, unit="TFLOP/s",
-                    frac=tf / PEAK_FP64_TFS, traffic=None, kernel="k_newton",
-                    avg_launch_ms=ms / max(launches, 1), hbm_gbs=gbs,
+                    frac=tf / PEAK_FP64_TFS, traffic=traffic, kernel="k_newton",
+                    avg_launch_ms=ms / max(launches, 1), hbm_gbs=gbs, **per,
                     note="FP64 vector ALU bound; peak is the MI355X FP64 figure (78.6 TFLOP/s), "
                          "no MFMA instructions are used")
     else:
         roof = dict(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s",
-                    frac=gbs / PEAK_HBM_GBS, traffic=None, kernel="k_newton",
-                    avg_launch_ms=ms / max(launches, 1), tflops=tf)
+                    frac=gbs / PEAK_HBM_GBS, traffic=traffic, kernel="k_newton",
+                    avg_launch_ms=ms / max(launches, 1), tflops=tf, **per)
     kern = {}
     names = ["newton", "fft_forward", "green", "fft_inverse_update", "increment_pro_epilogue",
              "anchor_refresh"]

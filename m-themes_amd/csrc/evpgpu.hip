@@ -219,7 +219,8 @@ NewtonParams newton_params(const evpgpu_ctx *c)
     P.crss_sym = c->crss_sym;
     P.nr_tol = c->nr_tol;
     P.tolx = c->tolx;
-    P.kmin_tolx = c->lam_min > 0.0 ? 0.99 * c->tolx * c->lam_min : 0.0;
+    const double kt = c->lam_min > 0.0 ? 0.99 * c->tolx * c->lam_min : 0.0;
+    P.kmin_tolx2 = kt * kt;
     return P;
 }
 

@@ -40,7 +40,7 @@ struct NewtonParams {
     double bn[2][EVPGPU_NSYSMX][3], bb[2][EVPGPU_NSYSMX][3]; // crystal-axis normals, directions
     double s0p[21];  // S0 = C0^-1 (B-basis), packed upper triangle: the Newton step is
                      // solved in S0-scaled form
-    double kmin_tolx; // 0.99 TOLX min over orientations of lambda_min(S0 + S)
+    double kmin_tolx2; // (0.99 TOLX min over orientations of lambda_min(S0 + S))^2
     double nr_tol;   // Newton tolerance on the relative step (reference 1e-10)
     double tolx;     // line-search TOLX (reference 1e-7)
 };

@@ -55,6 +55,19 @@ __device__ __forceinline__ double ipow5(double x, int m)
     return r;
 }
 
+// x^M for a compile-time M: the products of ipow with the bit tests folded
+// (x^9 is three squarings and one product)
+template <int M> __device__ __forceinline__ double ipowc(double x)
+{
+    double r = 1.0, b = x;
+#pragma unroll
+    for (int m = M; m; m >>= 1) {
+        if (m & 1) r *= b;
+        if (m > 1) b *= b;
+    }
+    return r;
+}
+
 // Rate-law evaluation: RM_POW pow() for a non-integral exponent, RM_LOOP
 // repeated squaring for an integral one, RM_FAST the common case of every
 // crystal phase having EVPGPU_NSYSMX systems with one integral n <= 32 and
@@ -101,7 +114,7 @@ struct RateIn {
 // systems in ascending order, as the reference's separate loops do.
 // DD: dt d(edot)/d(sigma) is added into the packed Newton matrix kk (which
 // holds S0 + S on entry).  GD: the slip rates are kept (gamdot).
-template <bool DD, int RM, bool GD>
+template <bool DD, int RM, bool GD, int NE>
 __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn &in,
                                       const double s[6], double ed[5], double kk[21],
                                       double gd[EVPGPU_NSYSMX])
@@ -124,7 +137,7 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
             r = r * itau;
             double a = fabs(r), p1, p2;
             if (RM == RM_FAST) {
-                p1 = ipow5(a, intn - 1);
+                p1 = NE > 0 ? ipowc<(NE > 0 ? NE - 1 : 0)>(a) : ipow5(a, intn - 1);
                 p2 = p1 * a;
             } else if (RM == RM_LOOP) {
                 p1 = ipow(a, intn - 1);
@@ -233,13 +246,13 @@ __device__ __forceinline__ void residual(const NewtonParams &P, const RateIn &in
     }
 }
 
-template <int RM, bool GD>
+template <int RM, bool GD, int NE>
 __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const RateIn &in,
                                              const Vox &V, const double sig[6],
                                              double gd[EVPGPU_NSYSMX])
 {
     double ed[5], t[6], res[6];
-    srate<false, RM, GD>(P, p, in, sig, ed, nullptr, gd);
+    srate<false, RM, GD, NE>(P, p, in, sig, ed, nullptr, gd);
     residual(P, in, V, sig, ed, t, res);
     double f = 0.0;
 #pragma unroll
@@ -251,13 +264,14 @@ __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const
 // FAST: an iteration whose step the reference's line search is certain to
 // reject (TOLX) ends right after the residual.  Since D >= 0, K = S0 + S +
 // dt D >= S0 + S, so |step|_inf <= |S0 R|_2 / lambda_min(S0 + S); when that
-// bound is below TOLX (P.kmin_tolx holds 0.99 TOLX lambda_min over all
-// orientations) the reference's test max|step_i|/max(|s_i|,1) < TOLX holds
-// and its first trial is rejected, leaving s, edot, the error norms and the
+// bound is below TOLX (P.kmin_tolx2 holds (0.99 TOLX lambda_min)^2, the
+// minimum over all orientations) the reference's test
+// max|step_i|/max(|s_i|,1) < TOLX holds and its first trial is rejected,
+// leaving s, edot, the error norms and the
 // counts exactly as computed here.  Only gamdot (from the rejected trial
 // point) differs; the loop recomputes the last Newton launch of an increment
 // with FAST off before gamdot is used, so FAST launches do not form gamdot.
-template <int RM, bool FAST>
+template <int RM, bool FAST, int NE>
 __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, const Vox &V,
                             double sig6[6], double edotp[6], double gd[EVPGPU_NSYSMX],
                             double &conv_s, double &conv_e, int &iterl, int &trials)
@@ -265,8 +279,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
     constexpr bool GD = !FAST;
     trials = 0;
     double ed[5], res[6], old[6], k[21];
-    conv_s = 0.0;
-    conv_e = 0.0;
+    double ce = 0.0;
     iterl = 0;
     double erral = 10 * P.nr_tol;
 #pragma unroll
@@ -283,17 +296,16 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
         for (int i = 0; i < 6; i++)
 #pragma unroll
             for (int j = i; j < 6; j++) k[PK(i, j)] = in.k0[21 + PK(i, j)] + V.S[PK(i, j)];
-        srate<true, RM, GD>(P, p, in, sig6, ed, k, gd);
+        srate<true, RM, GD, NE>(P, p, in, sig6, ed, k, gd);
 #pragma unroll
         for (int i = 0; i < 6; i++) edotp[i] = i < 5 ? ed[i] : 0.0;
         double gradf[6], step[6];
         {
             double t[6];
             residual(P, in, V, sig6, ed, t, res);
-            double ce = 0.0;
+            ce = 0.0;
 #pragma unroll
             for (int i = 0; i < 6; i++) ce += t[i] * t[i];
-            conv_e = sqrt(ce);
         }
         // S0 R: the right-hand side of the step, and its norm bounds the step
 #pragma unroll
@@ -308,7 +320,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             double y2 = 0.0;
 #pragma unroll
             for (int i = 0; i < 6; i++) y2 += step[i] * step[i];
-            skip = (y2 > 0.0) && (sqrt(y2) < P.kmin_tolx);
+            skip = (y2 > 0.0) && (y2 < P.kmin_tolx2);
         }
         if (skip) {
             trials++; // the reference's one rejected trial
@@ -364,7 +376,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             __asm__ volatile("" ::: "memory");
 #pragma unroll
             for (int i = 0; i < 6; i++) sig6[i] = old[i] + alam * step[i];
-            double f_new = nr_quality<RM, GD>(P, p, in, V, sig6, gd);
+            double f_new = nr_quality<RM, GD, NE>(P, p, in, V, sig6, gd);
             trials++;
             if (alam < alamin) {
 #pragma unroll
@@ -397,15 +409,18 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             alam = tmplam > 0.1 * alam ? tmplam : 0.1 * alam;
         }
         } // !skip
-        double conv_nr = 0.0, cs = 0.0;
+        double conv_nr = 0.0;
 #pragma unroll
-        for (int i = 0; i < 6; i++) {
-            conv_nr += (sig6[i] - old[i]) * (sig6[i] - old[i]);
-            cs += (sig6[i] - V.xl[i]) * (sig6[i] - V.xl[i]);
-        }
+        for (int i = 0; i < 6; i++) conv_nr += (sig6[i] - old[i]) * (sig6[i] - old[i]);
         erral = sqrt(conv_nr) / V.signorm;
-        conv_s = sqrt(cs);
     }
+    // the norms of the last iteration (the reference overwrites them in
+    // every iteration)
+    double cs = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) cs += (sig6[i] - V.xl[i]) * (sig6[i] - V.xl[i]);
+    conv_s = sqrt(cs);
+    conv_e = sqrt(ce);
     return 0;
 }
 
@@ -419,7 +434,7 @@ __device__ __forceinline__ double wave_sum(double v)
 // One lane per voxel.  Writes sig (in place), edot, gamdot; block partials
 // (NRED doubles per block) of err_s, err_e, Newton iterations, sum sig,
 // sum sig over phase 1.
-template <int RM, bool CSYM, bool FAST>
+template <int RM, bool CSYM, bool FAST, int NE>
 __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_newton(
     NewtonParams P, long long N, const int *__restrict__ phase, const int *__restrict__ orient,
     const double *__restrict__ sgr, const double *sig_in, double *sig, const double *__restrict__ E,
@@ -522,7 +537,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             }
             double s6[6], e6[6], gd[EVPGPU_NSYSMX], cs, ce;
             int it, ntr;
-            int st = update_voxel<RM, FAST>(P, p, in, V, s6, e6, gd, cs, ce, it, ntr);
+            int st = update_voxel<RM, FAST, NE>(P, p, in, V, s6, e6, gd, cs, ce, it, ntr);
             if (st) {
                 atomicCAS(status, 0, st);
             } else {
@@ -635,12 +650,28 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
         if (!one) fastrm = false;
     }
     const int rm = !intn ? RM_POW : (fastrm ? RM_FAST : RM_LOOP);
+    // compile-time exponents for the common n = 10 and n = 20 (every crystal
+    // phase the same n, forward = backward CRSS)
+    int ne = 0;
+    if (rm == RM_FAST && P.crss_sym) {
+        int n0 = 0;
+        bool same = true;
+        for (int q = 0; q < 2; q++) {
+            if (P.is_gas[q]) continue;
+            if (n0 && P.int_n[q] != n0) same = false;
+            n0 = P.int_n[q];
+        }
+        if (same && (n0 == 10 || n0 == 20)) ne = n0;
+    }
 #define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig_in, sig, E, \
                     epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
-#define LAUNCH(A, B, F) hipLaunchKernelGGL((k_newton<A, B, F>), NEWTON_ARGS)
+#define LAUNCH(A, B, F) hipLaunchKernelGGL((k_newton<A, B, F, 0>), NEWTON_ARGS)
+#define LAUNCH_NE(E, F) hipLaunchKernelGGL((k_newton<RM_FAST, true, F, E>), NEWTON_ARGS)
 #define LAUNCH_RM(B, F)                                                                  \
     do {                                                                                 \
-        if (rm == RM_FAST) LAUNCH(RM_FAST, B, F);                                        \
+        if (ne == 10) LAUNCH_NE(10, F);                                                  \
+        else if (ne == 20) LAUNCH_NE(20, F);                                             \
+        else if (rm == RM_FAST) LAUNCH(RM_FAST, B, F);                                   \
         else if (rm == RM_LOOP) LAUNCH(RM_LOOP, B, F);                                   \
         else LAUNCH(RM_POW, B, F);                                                       \
     } while (0)
@@ -652,6 +683,7 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
         else LAUNCH_RM(false, false);
     }
 #undef LAUNCH_RM
+#undef LAUNCH_NE
 #undef LAUNCH
 #undef NEWTON_ARGS
     if (nblk_out) *nblk_out = nblk;
