@@ -158,7 +158,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4, help="timed increments")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=128, help="grid edge (default: the 128^3 job)")
+    ap.add_argument("--n", type=int, default=128,
+                    help="grid edge (default: the 128^3 job; 256: the 256^3 job of 4096 grains)")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--cpu-iters", type=int, default=3,
                     help="Green iterations of the CPU baseline sample (0: skip)")
@@ -177,10 +178,13 @@ def main():
               "on one GPU (no multi-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    cfg = synth.config_c2()
-    if args.n != 128:
-        cfg.n = args.n
-        cfg.grains = max(8, 512 * args.n ** 3 // 128 ** 3)
+    if args.n == 256:
+        cfg = synth.config_c3()     # 256^3, 4096 grains, no pores
+    else:
+        cfg = synth.config_c2()
+        if args.n != 128:
+            cfg.n = args.n
+            cfg.grains = max(8, 512 * args.n ** 3 // 128 ** 3)
     work = tempfile.mkdtemp(prefix="evpbench_")
     t = time.perf_counter()
     inp = synth.write_job(work, cfg)

@@ -46,7 +46,10 @@ struct NewtonParams {
 };
 
 struct GreenParams {
-    double c0[81];
+    // A = C0 : (n x n) is symmetric: A_ik (ik = 00 11 22 12 02 01) =
+    // sum_p ga[ik*6 + p] n_j n_l over the pairs p = jl (00 11 22 12 02 01),
+    // ga holding C0_ijkj for j = l and C0_ijkl + C0_ilkj for j < l
+    double ga[36];
     double s0[81];
     int elast_bc;
 };
@@ -185,6 +188,8 @@ void tmark(evpgpu_ctx *c, int k, int end);
 void tcollect(evpgpu_ctx *c);
 
 bool fused_ok(const evpgpu_ctx *c);
+// the Green operator's parameters from the context's loading
+GreenParams green_params(const evpgpu_ctx *c);
 int fused_iteration_fp64(evpgpu_ctx *c, const double *sig, double *E, const double *add, double scale);
 int fused_iteration_delta(evpgpu_ctx *c, const double *sig, const float *sigA, double *E, const double *add,
                           const float *cA, double scale);

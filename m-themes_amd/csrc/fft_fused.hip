@@ -195,8 +195,8 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
     constexpr int PX = NX + 1;
     constexpr int NL = 6 * TBX;
     __shared__ C lds[NL * PX];
-    __shared__ R c0s[81];
-    if (threadIdx.x < 81) c0s[threadIdx.x] = (R)G.c0[threadIdx.x];
+    __shared__ R c0s[36];
+    if (threadIdx.x < 36) c0s[threadIdx.x] = (R)G.ga[threadIdx.x];
     // The x lines are the columns q = y*nzh + kz of the [x][y][kz] half
     // spectrum; a workgroup takes TBX consecutive columns q0.. (runs that
     // start on 128-byte boundaries, as ny*nzh is a multiple of TBX).
@@ -214,8 +214,8 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
     gstages<false, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
 #pragma unroll 1
     for (int i = threadIdx.x; i < TBX * NX; i += NTX) {
-        // keep the 81 C0 values in LDS: hoisted out of this loop they would
-        // occupy 162 registers for the whole loop
+        // keep the 36 Green coefficients in LDS: hoisted out of this loop
+        // they would occupy 72 registers for the whole loop
         __asm__ volatile("" ::: "memory");
         const int x = i / TBX, j = i % TBX;
         if (j >= nk) continue;
@@ -318,10 +318,7 @@ template <typename R, int GMODE>
 int xg(evpgpu_ctx *c)
 {
     typedef typename CT<R>::T C;
-    GreenParams G;
-    memcpy(G.c0, c->ld.c0, sizeof(G.c0));
-    memcpy(G.s0, c->ld.s0, sizeof(G.s0));
-    G.elast_bc = c->ld.elast_bc;
+    const GreenParams G = green_params(c);
 #define CALLL(NX_)                                                                                        \
     {                                                                                                    \
         constexpr int T = tbx<R, NX_>();                                                                 \

@@ -40,8 +40,8 @@ __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 
                                                int ny, int nz, int nzh, long long NK)
 {
     typedef typename CT<R>::T C;
-    __shared__ R c0s[81];
-    if (threadIdx.x < 81) c0s[threadIdx.x] = (R)P.c0[threadIdx.x];
+    __shared__ R c0s[36];
+    if (threadIdx.x < 36) c0s[threadIdx.x] = (R)P.ga[threadIdx.x];
     __syncthreads();
     long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= NK) return;
@@ -156,13 +156,27 @@ int green_project(evpgpu_ctx *c)
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 
+GreenParams green_params(const evpgpu_ctx *c)
+{
+    GreenParams P;
+    static const int pr[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+    const double *C0 = c->ld.c0;
+#define C4(i, j, k, l) C0[(((i)*3 + (j)) * 3 + (k)) * 3 + (l)]
+    for (int q = 0; q < 6; q++)
+        for (int p = 0; p < 6; p++) {
+            const int i = pr[q][0], k = pr[q][1], j = pr[p][0], l = pr[p][1];
+            P.ga[q * 6 + p] = j == l ? C4(i, j, k, l) : C4(i, j, k, l) + C4(i, l, k, j);
+        }
+#undef C4
+    memcpy(P.s0, c->ld.s0, sizeof(P.s0));
+    P.elast_bc = c->ld.elast_bc;
+    return P;
+}
+
 // mixed loop: MODE 3 on the FP64 anchor spectrum, MODE 2 on the FP32 delta
 int green_anchor(evpgpu_ctx *c)
 {
-    GreenParams P;
-    memcpy(P.c0, c->ld.c0, sizeof(P.c0));
-    memcpy(P.s0, c->ld.s0, sizeof(P.s0));
-    P.elast_bc = c->ld.elast_bc;
+    const GreenParams P = green_params(c);
     unsigned nb = (unsigned)((c->NK + 255) / 256);
     hipLaunchKernelGGL((k_green<double, 3>), dim3(nb), dim3(256), 0, c->stream, (double2 *)c->spec, nullptr,
                        nullptr, c->skA, P, c->nx, c->ny, c->nz, c->nzh, c->NK);
@@ -171,10 +185,7 @@ int green_anchor(evpgpu_ctx *c)
 
 int green_delta(evpgpu_ctx *c)
 {
-    GreenParams P;
-    memcpy(P.c0, c->ld.c0, sizeof(P.c0));
-    memcpy(P.s0, c->ld.s0, sizeof(P.s0));
-    P.elast_bc = c->ld.elast_bc;
+    const GreenParams P = green_params(c);
     unsigned nb = (unsigned)((c->NK + 255) / 256);
     hipLaunchKernelGGL((k_green<float, 2>), dim3(nb), dim3(256), 0, c->stream, (float2 *)c->spec, c->acc,
                        nullptr, c->skA, P, c->nx, c->ny, c->nz, c->nzh, c->NK);
@@ -183,10 +194,7 @@ int green_delta(evpgpu_ctx *c)
 
 int green_apply(evpgpu_ctx *c, int precision, int accumulate)
 {
-    GreenParams P;
-    memcpy(P.c0, c->ld.c0, sizeof(P.c0));
-    memcpy(P.s0, c->ld.s0, sizeof(P.s0));
-    P.elast_bc = c->ld.elast_bc;
+    const GreenParams P = green_params(c);
     unsigned nb = (unsigned)((c->NK + 255) / 256);
     if (precision == EVPGPU_MIXED)
         hipLaunchKernelGGL((k_green<float, 0>), dim3(nb), dim3(256), 0, c->stream, (float2 *)c->spec,
@@ -201,10 +209,7 @@ int green_apply(evpgpu_ctx *c, int precision, int accumulate)
 
 int green_full9(evpgpu_ctx *c, const double2 *in6, double2 *out9)
 {
-    GreenParams P;
-    memcpy(P.c0, c->ld.c0, sizeof(P.c0));
-    memcpy(P.s0, c->ld.s0, sizeof(P.s0));
-    P.elast_bc = c->ld.elast_bc;
+    const GreenParams P = green_params(c);
     unsigned nb = (unsigned)((c->NK + 255) / 256);
     hipLaunchKernelGGL((k_green<double, 1>), dim3(nb), dim3(256), 0, c->stream, (double2 *)in6, nullptr,
                        out9, nullptr, P, c->nx, c->ny, c->nz, c->nzh, c->NK);

@@ -18,38 +18,32 @@
 
 namespace gop {
 
-// e = Gamma(n) s for complex s (3x3 full), e accumulated with weight w
+// e = Gamma(n) s for complex s (3x3 full), e accumulated with weight w;
+// ga: the 36 coefficients of the symmetric A = C0 : (n x n) (GreenParams)
 template <typename R, typename C>
-__device__ __forceinline__ void gamma_apply(const R *c0, const double nd[3], const C s[3][3],
+__device__ __forceinline__ void gamma_apply(const R *ga, const double nd[3], const C s[3][3],
                                             C e[3][3], R w)
 {
     R n[3] = {(R)nd[0], (R)nd[1], (R)nd[2]};
-    R A[3][3];
+    const R nn[6] = {n[0] * n[0], n[1] * n[1], n[2] * n[2], n[1] * n[2], n[0] * n[2], n[0] * n[1]};
+    R a[6];
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int q = 0; q < 6; q++) {
+        R acc = 0;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            R acc = 0;
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-#pragma unroll
-                for (int l = 0; l < 3; l++) acc += c0[((i * 3 + j) * 3 + k) * 3 + l] * n[j] * n[l];
-            A[i][k] = acc;
-        }
-    // M3Inverse (src/init.c:517-539)
-    R det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
-            A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
-            A[0][2] * (A[1][0] * A[2][1] - A[2][0] * A[1][1]);
+        for (int p = 0; p < 6; p++) acc += ga[q * 6 + p] * nn[p];
+        a[q] = acc;
+    }
+    // M3Inverse (src/init.c:517-539) of the symmetric A: the adjugate O and
+    // the determinant (A = [a0 a5 a4; a5 a1 a3; a4 a3 a2])
     R O[3][3];
-    O[0][0] = A[1][1] * A[2][2] - A[1][2] * A[2][1];
-    O[0][1] = A[0][2] * A[2][1] - A[0][1] * A[2][2];
-    O[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
-    O[1][0] = A[1][2] * A[2][0] - A[1][0] * A[2][2];
-    O[1][1] = A[0][0] * A[2][2] - A[0][2] * A[2][0];
-    O[1][2] = A[0][2] * A[1][0] - A[0][0] * A[1][2];
-    O[2][0] = A[1][0] * A[2][1] - A[1][1] * A[2][0];
-    O[2][1] = A[0][1] * A[2][0] - A[0][0] * A[2][1];
-    O[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    O[0][0] = a[1] * a[2] - a[3] * a[3];
+    O[1][1] = a[0] * a[2] - a[4] * a[4];
+    O[2][2] = a[0] * a[1] - a[5] * a[5];
+    O[0][1] = O[1][0] = a[4] * a[3] - a[5] * a[2];
+    O[0][2] = O[2][0] = a[5] * a[3] - a[4] * a[1];
+    O[1][2] = O[2][1] = a[4] * a[5] - a[0] * a[3];
+    R det = a[0] * O[0][0] + a[5] * O[0][1] + a[4] * O[0][2];
     R id = (R)1 / det;
     C t[3], u[3];
 #pragma unroll
