@@ -277,7 +277,8 @@ def main():
         "dtype": "f32+f64" if main_key == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"EVPFFT {cfg.n}^3, {cfg.grains} Voronoi grains, "
-                               f"{cfg.pore_fraction:.0%} pores, increments 1..{args.steps} of 50",
+                               f"{cfg.pore_fraction:.0%} pores, "
+                               f"increments 1..{args.steps} of {cfg.increments}",
                    "grid": cfg.n, "grains": cfg.grains, "pores": cfg.pore_fraction,
                    "precision": main_key, "green_tol": cfg.err, "seed": hex(cfg.seed)},
         "green_iters": R["iters"],

@@ -424,13 +424,6 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
     return 0;
 }
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // One lane per voxel.  Writes sig (in place), edot, gamdot; block partials
 // (NRED doubles per block) of err_s, err_e, Newton iterations, sum sig,
 // sum sig over phase 1.
@@ -587,20 +580,24 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             red[9 + c] = (phase[v] == 1) ? sc : 0.0;
         }
     }
-    __shared__ double wsum[NB_THREADS / 64][NRED];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < NRED; i++) {
-        double s = wave_sum(red[i]);
-        if (lane == 0) wsum[w][i] = s;
-    }
+    // Block partial sums, transposed through the slab (no longer needed):
+    // value i is summed by lanes 4i..4i+3 over 16 entries each, then across
+    // the four (a + b == b + a, so all four hold the same sum): a fixed order
+    // at a quarter of the instructions of 16 butterfly reductions.
+    static_assert(NB_THREADS == 64 && NRED * 4 == NB_THREADS && NRED * 65 <= EVPGPU_NSYSMX * 5 * NB_THREADS,
+                  "one wave per block, 16 partial sums");
+    const int lane = threadIdx.x;
     __syncthreads();
-    if (threadIdx.x < NRED) {
-        double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < NB_THREADS / 64; q++) s += wsum[q][threadIdx.x];
-        partials[(long long)blockIdx.x * NRED + threadIdx.x] = s;
-    }
+    for (int i = 0; i < NRED; i++) slab[i * 65 + lane] = red[i];
+    __syncthreads();
+    const int vi = lane >> 2, q = lane & 3;
+    double sum = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) sum += slab[vi * 65 + q * 16 + j];
+    sum += __shfl_xor(sum, 1, 64);
+    sum += __shfl_xor(sum, 2, 64);
+    if (q == 0) partials[(long long)blockIdx.x * NRED + vi] = sum;
 }
 
 // Deterministic two-stage sum of nblk rows of nval (<= 16) partials.
