@@ -126,12 +126,20 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
     const double g0 = P.gam0[p][0], g0n = P.gam0[p][0] * P.nrs[p][0];
 #pragma unroll
     for (int i = 0; i < 5; i++) ed[i] = 0.;
+    double scn[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) scn[j] = SC(0, j);
 #pragma unroll
     for (int k = 0; k < EVPGPU_NSYSMX; k++) {
-        if (k < ns) {
-            double sc[5];
+        // the next system's Schmid vector is read before this one's
+        // arithmetic, so the LDS latency is hidden behind it
+        double sc[5];
 #pragma unroll
-            for (int j = 0; j < 5; j++) sc[j] = SC(k, j);
+        for (int j = 0; j < 5; j++) sc[j] = scn[j];
+        if (k + 1 < EVPGPU_NSYSMX)
+#pragma unroll
+            for (int j = 0; j < 5; j++) scn[j] = SC(k + 1, j);
+        if (k < ns) {
             double r = sc[0] * s[0] + sc[1] * s[1] + sc[2] * s[2] + sc[3] * s[3] + sc[4] * s[4];
             double itau = (r < 0) ? in.itb[k] : in.itf[k];
             r = r * itau;
