@@ -98,6 +98,11 @@ class Config:
     # a second crystalline phase (own .sx file and elastic constants) on the
     # even-numbered grains; excludes pores (the reference has two phases)
     phase2: Optional[Material] = None
+    # (nx, ny, nz) of a non-cubic grid; None: n^3
+    dims: Optional[tuple] = None
+
+    def shape(self) -> tuple:
+        return tuple(int(d) for d in self.dims) if self.dims is not None else (self.n,) * 3
 
 
 # The configurations the loop is measured and checked on (sizes, grains, loading).
@@ -121,55 +126,63 @@ def config_c4() -> Config:
     return Config(n=64, grains=128, seed=0x5EED0004, increments=20)
 
 
-def voronoi(n: int, seeds: np.ndarray) -> np.ndarray:
+def _shape(n) -> tuple:
+    return (n, n, n) if np.isscalar(n) else tuple(int(d) for d in n)
+
+
+def voronoi(n, seeds: np.ndarray) -> np.ndarray:
     """Periodic Voronoi labels (0-based) of voxel points (i, j, k), x-major, z fastest.
 
-    Nearest seed under the minimum-image distance; ties go to the lower seed index.
+    ``n``: the edge of a cubic grid or its (nx, ny, nz).  Nearest seed under the
+    minimum-image distance; ties go to the lower seed index.
     """
+    nx, ny, nz = _shape(n)
     g = seeds.shape[0]
-    ids = np.empty(n * n * n, dtype=np.int32)
-    coords = np.arange(n, dtype=np.float64)
-    zz = coords[None, :]
-    for x in range(n):
-        best = np.full((n, n), np.inf)
-        lab = np.zeros((n, n), dtype=np.int32)
+    ids = np.empty(nx * ny * nz, dtype=np.int32)
+    cy = np.arange(ny, dtype=np.float64)
+    zz = np.arange(nz, dtype=np.float64)[None, :]
+    for x in range(nx):
+        best = np.full((ny, nz), np.inf)
+        lab = np.zeros((ny, nz), dtype=np.int32)
         dx = np.abs(x - seeds[:, 0])
-        dx = np.minimum(dx, n - dx)
+        dx = np.minimum(dx, nx - dx)
         for s in range(g):
-            dy = np.abs(coords - seeds[s, 1])
-            dy = np.minimum(dy, n - dy)
+            dy = np.abs(cy - seeds[s, 1])
+            dy = np.minimum(dy, ny - dy)
             dz = np.abs(zz - seeds[s, 2])
-            dz = np.minimum(dz, n - dz)
+            dz = np.minimum(dz, nz - dz)
             d = dx[s] * dx[s] + dy[:, None] ** 2 + dz ** 2
             m = d < best
             best = np.where(m, d, best)
             lab = np.where(m, s, lab)
-        ids[x * n * n:(x + 1) * n * n] = lab.reshape(-1)
+        ids[x * ny * nz:(x + 1) * ny * nz] = lab.reshape(-1)
     return ids
 
 
-def voronoi_fast(n: int, seeds: np.ndarray) -> np.ndarray:
+def voronoi_fast(n, seeds: np.ndarray) -> np.ndarray:
     """Same labels as :func:`voronoi` via a periodic k-d tree (large grids)."""
     from scipy.spatial import cKDTree
 
-    tree = cKDTree(np.mod(seeds, n), boxsize=n)
-    ax = np.arange(n, dtype=np.float64)
-    out = np.empty(n ** 3, dtype=np.int32)
-    yy, zz = np.meshgrid(ax, ax, indexing="ij")
-    plane = np.stack([np.zeros(n * n), yy.ravel(), zz.ravel()], axis=1)
-    for x in range(n):
+    nx, ny, nz = _shape(n)
+    box = np.array([nx, ny, nz], dtype=np.float64)
+    tree = cKDTree(np.mod(seeds, box), boxsize=box)
+    out = np.empty(nx * ny * nz, dtype=np.int32)
+    yy, zz = np.meshgrid(np.arange(ny, dtype=np.float64), np.arange(nz, dtype=np.float64), indexing="ij")
+    plane = np.stack([np.zeros(ny * nz), yy.ravel(), zz.ravel()], axis=1)
+    for x in range(nx):
         plane[:, 0] = x
         d, i = tree.query(plane, k=2)
         # exact ties resolved towards the lower seed index, as in voronoi()
         tie = d[:, 0] == d[:, 1]
         lab = np.where(tie, np.minimum(i[:, 0], i[:, 1]), i[:, 0])
-        out[x * n * n:(x + 1) * n * n] = lab
+        out[x * ny * nz:(x + 1) * ny * nz] = lab
     return out
 
 
-def pores(n: int, frac: float, radius: int, rng: SplitMix64) -> np.ndarray:
+def pores(n, frac: float, radius: int, rng: SplitMix64) -> np.ndarray:
     """Boolean gas mask: non-overlapping periodic digital spheres |x-c|^2 <= r^2."""
-    mask = np.zeros((n, n, n), dtype=bool)
+    nx, ny, nz = _shape(n)
+    mask = np.zeros((nx, ny, nz), dtype=bool)
     if frac <= 0.0:
         return mask.reshape(-1)
     r = radius
@@ -177,18 +190,18 @@ def pores(n: int, frac: float, radius: int, rng: SplitMix64) -> np.ndarray:
     ox, oy, oz = np.meshgrid(off, off, off, indexing="ij")
     ball = (ox ** 2 + oy ** 2 + oz ** 2) <= r * r
     bx, by, bz = ox[ball], oy[ball], oz[ball]
-    target = frac * n ** 3
+    target = frac * nx * ny * nz
     tries = 0
     while mask.sum() < target and tries < 100000:
         tries += 1
-        c = [int(rng.uniform() * n) for _ in range(3)]
-        px = (c[0] + bx) % n
-        py = (c[1] + by) % n
-        pz = (c[2] + bz) % n
+        c = [int(rng.uniform() * d) for d in (nx, ny, nz)]
+        px = (c[0] + bx) % nx
+        py = (c[1] + by) % ny
+        pz = (c[2] + bz) % nz
         # reject overlapping or touching spheres
-        gx = (c[0] + np.arange(-r - 1, r + 2)) % n
-        gy = (c[1] + np.arange(-r - 1, r + 2)) % n
-        gz = (c[2] + np.arange(-r - 1, r + 2)) % n
+        gx = (c[0] + np.arange(-r - 1, r + 2)) % nx
+        gy = (c[1] + np.arange(-r - 1, r + 2)) % ny
+        gz = (c[2] + np.arange(-r - 1, r + 2)) % nz
         if mask[np.ix_(gx, gy, gz)].any():
             continue
         mask[px, py, pz] = True
@@ -197,9 +210,10 @@ def pores(n: int, frac: float, radius: int, rng: SplitMix64) -> np.ndarray:
 
 def microstructure(cfg: Config):
     """(euler_deg[N^3,3], grain[N^3], phase[N^3]) of a configuration."""
-    n = cfg.n
+    shape = cfg.shape()
+    N = int(np.prod(shape))
     rng = SplitMix64(cfg.seed)
-    seeds = rng.uniforms(3 * cfg.grains).reshape(cfg.grains, 3) * n
+    seeds = rng.uniforms(3 * cfg.grains).reshape(cfg.grains, 3) * np.array(shape, dtype=np.float64)
     u = rng.uniforms(3 * cfg.grains).reshape(cfg.grains, 3)
     if cfg.single_orientation is not None:
         eul = np.tile(np.asarray(cfg.single_orientation, dtype=np.float64), (cfg.grains, 1))
@@ -209,14 +223,14 @@ def microstructure(cfg: Config):
     # round to the precision written to the file so in-memory and file inputs agree
     eul = np.round(eul, 8)
     if cfg.grains == 1:
-        lab = np.zeros(n ** 3, dtype=np.int32)
-    elif n ** 3 * cfg.grains <= 4_000_000:
-        lab = voronoi(n, seeds)
+        lab = np.zeros(N, dtype=np.int32)
+    elif N * cfg.grains <= 4_000_000:
+        lab = voronoi(shape, seeds)
     else:
-        lab = voronoi_fast(n, seeds)
-    gas = pores(n, cfg.pore_fraction, cfg.pore_radius, rng)
+        lab = voronoi_fast(shape, seeds)
+    gas = pores(shape, cfg.pore_fraction, cfg.pore_radius, rng)
     grain = (lab + 1).astype(np.int32)
-    phase = np.ones(n ** 3, dtype=np.int32)
+    phase = np.ones(N, dtype=np.int32)
     if cfg.phase2 is not None:
         if cfg.pore_fraction > 0.0:
             raise ValueError("a second crystalline phase and pores need three phases")
@@ -228,18 +242,19 @@ def microstructure(cfg: Config):
     return euler, grain, phase
 
 
-def write_ms(path: str, n: int, euler: np.ndarray, grain: np.ndarray, phase: np.ndarray):
+def write_ms(path: str, n, euler: np.ndarray, grain: np.ndarray, phase: np.ndarray):
     """Write the 8-column microstructure file (each line < 80 characters).
 
     Lines are ``phi1 Phi phi2 x y z grain phase`` with the angles to 8 decimals;
     when every grain carries one orientation (the synthetic case) the angle text
     is formatted once per grain.
     """
-    N = n ** 3
+    nx, ny, nz = _shape(n)
+    N = nx * ny * nz
     idx = np.arange(N)
-    xs = (idx // (n * n) + 1).tolist()
-    ys = ((idx // n) % n + 1).tolist()
-    zs = (idx % n + 1).tolist()
+    xs = (idx // (ny * nz) + 1).tolist()
+    ys = ((idx // nz) % ny + 1).tolist()
+    zs = (idx % nz + 1).tolist()
     g = grain.tolist()
     p = phase.tolist()
     uniq, first = np.unique(grain, return_index=True)
@@ -289,7 +304,7 @@ def write_input(path: str, cfg: Config, sx_name="Cu.sx", ms_name="ms.txt"):
     lines = [
         "# synthetic EVPFFT job (seed 0x%X)" % cfg.seed,
         "N_phases\t%d" % (2 if two else 1),
-        "CellDim\t%d %d %d" % (cfg.n, cfg.n, cfg.n),
+        "CellDim\t%d %d %d" % cfg.shape(),
         "initial_ms\t%s" % ms_name,
         "Type_phases\t0\t%d" % (0 if m2 is not None else 1),
         "ElastConst_PhaseI\t%r\t%r\t%r\t%r" % (m.c11, m.c12, m.c44, m.nu),
@@ -319,7 +334,7 @@ def write_job(dirpath: str, cfg: Config) -> str:
     """Write input.in, Cu.sx and ms.txt of ``cfg`` into ``dirpath``; return input.in path."""
     os.makedirs(dirpath, exist_ok=True)
     euler, grain, phase = microstructure(cfg)
-    write_ms(os.path.join(dirpath, "ms.txt"), cfg.n, euler, grain, phase)
+    write_ms(os.path.join(dirpath, "ms.txt"), cfg.shape(), euler, grain, phase)
     write_sx(os.path.join(dirpath, "Cu.sx"), cfg.material)
     if cfg.phase2 is not None:
         write_sx(os.path.join(dirpath, "P2.sx"), cfg.phase2)

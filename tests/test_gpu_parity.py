@@ -201,6 +201,26 @@ def test_loop_parity_16_pores(job_factory):
     _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
 
 
+@pytest.mark.parametrize("dims", [(16, 8, 8), (8, 16, 32)])
+def test_loop_parity_non_cubic(job_factory, dims):
+    """Non-cubic grids: nx != ny = nz runs the fused transforms, ny != nz the
+    seven passes; both against the oracle, pores included."""
+    inp = job_factory(f"loopnc{dims}", n=0, dims=dims, grains=6, seed=37, increments=3,
+                      pore_fraction=0.02, pore_radius=1)
+    _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
+
+
+def test_mixed_precision_non_cubic(job_factory):
+    """Mixed loop on a 32 x 16 x 16 grid against the FP64 loop at equal counts."""
+    inp = job_factory("mixnc", n=0, dims=(32, 16, 16), grains=8, seed=41, increments=2)
+    a = lib.GpuJob(inp, lib.FP64)
+    ia, _ = a.run(2)
+    b = lib.GpuJob(inp, lib.MIXED)
+    for k in range(2):
+        b.run(1, forced=int(ia[k]))
+    assert rel(b.state()["Sig"], a.state()["Sig"]) < 1e-6
+
+
 @pytest.mark.parametrize("nrs2", [20.0, 12.5, 34.0])
 def test_loop_parity_two_crystal_phases(job_factory, nrs2):
     """Two crystalline phases on alternating grains: waves that straddle a

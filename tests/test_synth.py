@@ -69,3 +69,22 @@ def test_two_crystal_phases_reach_the_oracle(tmp_path):
     J1 = O.Job(one)
     J1.run(1)
     assert abs(J.macro()["SigAvg"][2, 2] - J1.macro()["SigAvg"][2, 2]) > 1e-3 * abs(J1.macro()["SigAvg"][2, 2])
+
+
+def test_non_cubic_grid_round_trips_through_the_oracle(tmp_path):
+    """(nx, ny, nz) grids: k-d tree and brute-force Voronoi agree, the pores
+    stay in the box and the oracle reads CellDim and the voxel order back."""
+    shape = (12, 6, 10)
+    seeds = synth.SplitMix64(4).uniforms(3 * 5).reshape(5, 3) * np.array(shape, float)
+    assert np.array_equal(synth.voronoi(shape, seeds), synth.voronoi_fast(shape, seeds))
+    m = synth.pores(shape, 0.03, 1, synth.SplitMix64(2))
+    # radius-1 digital spheres hold 7 voxels
+    assert m.shape == (720,) and 0.03 <= m.mean() < 0.05 and m.sum() % 7 == 0
+    cfg = synth.Config(n=0, dims=(8, 4, 16), grains=5, seed=11, increments=1, pore_fraction=0.03,
+                       pore_radius=1)
+    inp = synth.write_job(str(tmp_path), cfg)
+    _, grain, phase = synth.microstructure(cfg)
+    J = O.Job(inp)
+    assert tuple(J.dims) == (8, 4, 16)
+    assert np.array_equal(J.grain(), grain) and np.array_equal(J.phase(), phase)
+    J.run(1)
