@@ -60,8 +60,7 @@ NEWTON_BYTES_GAS = 4 + 96 + 48
 # k_newton's HBM traffic per launch on the 128^3 job (launches with the
 # shortcut on), measured: rocprofv3 FETCH_SIZE x 1024 x 2 + WRITE_SIZE x 1024
 # (FETCH_SIZE reads half and WRITE_SIZE all of a plain copy kernel's bytes);
-# 684.5 MB read + 205.5 MB written against 861 MB algorithmic
-# (profiles/hbm_traffic_128.md).
+# 684.5 MB read + 205.5 MB written against 861 MB algorithmic.
 NEWTON_TRAFFIC_128 = 8.900e8
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFS = 78.6
