@@ -380,6 +380,15 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             if (t > test) test = t;
         }
         double alamin = TOLX / test, alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0;
+        // The first trial (alam = 1) is rejected when 1 < alamin, whatever
+        // its merit value: the step is left undone and only gamdot (from the
+        // trial point) is affected, which FAST launches do not form -- so
+        // they skip the evaluation.  This is how nearly every voxel's solve
+        // ends: an accepted step has relative size >= TOLX, which keeps the
+        // relative-step test (1e-10) open, so the last step is a TOLX reject.
+        if (FAST && alam < alamin) {
+            trials++;
+        } else {
         while (true) {
             __asm__ volatile("" ::: "memory");
 #pragma unroll
@@ -416,6 +425,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
             f2 = f_new; // the backtrack value the cubic step needs (Q18)
             alam = tmplam > 0.1 * alam ? tmplam : 0.1 * alam;
         }
+        } // line search
         } // !skip
         double conv_nr = 0.0;
 #pragma unroll
