@@ -30,7 +30,7 @@ namespace {
 using namespace fftc;
 
 constexpr int NTH = 1024;        // threads per workgroup of the plane kernels
-constexpr int NTX = 512;         // of the x-line kernel (the Green step needs 256 VGPRs)
+constexpr int NTX = 256;         // of the x-line kernel (up to 3 workgroups per CU)
 constexpr int LDS_MAX = 160 * 1024;
 
 template <typename R, int NY, int NZ> struct Plane {
@@ -212,22 +212,12 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
     }
     __syncthreads();
     gstages<false, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
-#pragma unroll 1
-    for (int i = threadIdx.x; i < TBX * NX; i += NTX) {
-        // keep the 36 Green coefficients in LDS: hoisted out of this loop
-        // they would occupy 72 registers for the whole loop
-        __asm__ volatile("" ::: "memory");
-        const int x = i / TBX, j = i % TBX;
-        if (j >= nk) continue;
-        C s6[6], e6[6];
-        double wr[3], wi[3];
-#pragma unroll
-        for (int c = 0; c < 6; c++) s6[c] = lds[(c * TBX + j) * PX + x];
-        const int q = q0 + j, y = q / nzh, kz = q - y * nzh;
-        gop::green_point<R>(c0s, G.s0, G.elast_bc, x, y, kz, NX, ny, nz, s6, e6, wr, wi);
+    // Green result at (x, column q0 + j) back into LDS, antisymmetric part
+    // into the accumulator
+    auto put = [&](int x, int j, const C e6[6], const double wr[3], const double wi[3]) {
 #pragma unroll
         for (int c = 0; c < 6; c++) lds[(c * TBX + j) * PX + x] = e6[c];
-        const long long id = (long long)x * ncol + q;
+        const long long id = (long long)x * ncol + q0 + j;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             if (GMODE == 3) {
@@ -243,6 +233,30 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
                 acc[c * NK + id] = make_double2(w.x + ax, w.y + ay);
             }
         }
+    };
+#pragma unroll 1
+    for (int i = threadIdx.x; i < TBX * NX; i += NTX) {
+        // keep the 36 Green coefficients in LDS: hoisted out of this loop
+        // they would occupy 72 registers for the whole loop
+        __asm__ volatile("" ::: "memory");
+        const int x = i / TBX, j = i % TBX;
+        const int q = q0 + j;
+        if (j >= nk || (q == 0 && x == 0)) continue; // k = 0 below
+        C s6[6], e6[6];
+        double wr[3], wi[3];
+#pragma unroll
+        for (int c = 0; c < 6; c++) s6[c] = lds[(c * TBX + j) * PX + x];
+        const int y = q / nzh, kz = q - y * nzh;
+        gop::green_point<R>(c0s, x, y, kz, NX, ny, nz, s6, e6, wr, wi);
+        put(x, j, e6, wr, wi);
+    }
+    if (q0 == 0 && threadIdx.x == 0) {
+        C s6[6], e6[6];
+        double wr[3], wi[3];
+#pragma unroll
+        for (int c = 0; c < 6; c++) s6[c] = lds[c * TBX * PX];
+        gop::green_zero<R>(c0s, G.s0, G.elast_bc, NX, ny, nz, s6, e6, wr, wi);
+        put(0, 0, e6, wr, wi);
     }
     __syncthreads();
     gstages<true, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
@@ -256,12 +270,13 @@ template <typename R> const typename CT<R>::T *twp(evpgpu_ctx *c, int axis);
 template <> const double2 *twp<double>(evpgpu_ctx *c, int axis) { return c->tw64[axis]; }
 template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[axis]; }
 
-// lines per x-green workgroup: 6 components x TBX lines x (nx+1) in LDS
+// lines per x-green workgroup: 6 components x TBX lines x (nx+1) in LDS,
+// sized so that three workgroups share a CU's LDS (their load, transform and
+// Green phases then overlap)
 template <typename R, int NX> constexpr int tbx()
 {
-    return (6 * 16 * (NX + 1) * (int)sizeof(typename CT<R>::T) <= LDS_MAX - 1024) ? 16
-           : (6 * 8 * (NX + 1) * (int)sizeof(typename CT<R>::T) <= LDS_MAX - 1024) ? 8
-                                                                                    : 4;
+    constexpr int per = 6 * (NX + 1) * (int)sizeof(typename CT<R>::T), cap = LDS_MAX / 3 - 1024;
+    return per * 16 <= cap ? 16 : per * 8 <= cap ? 8 : per * 4 <= cap ? 4 : 2;
 }
 
 #define PLANE_DISPATCH(CALL)                                   \

@@ -63,7 +63,8 @@ __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 
     }
     C e6[6];
     double wr[3], wi[3];
-    gop::green_point<R>(c0s, P.s0, P.elast_bc, x, y, kz, nx, ny, nz, s6, e6, wr, wi);
+    if (id == 0) gop::green_zero<R>(c0s, P.s0, P.elast_bc, nx, ny, nz, s6, e6, wr, wi);
+    else gop::green_point<R>(c0s, x, y, kz, nx, ny, nz, s6, e6, wr, wi);
 #pragma unroll
     for (int c = 0; c < 6; c++) spec[c * NK + id] = e6[c];
 #pragma unroll

@@ -101,15 +101,59 @@ __device__ __forceinline__ void green_tensor(const R *c0s, const double *s0, int
     }
 }
 
-// symmetric part (11 22 33 23 13 12) and antisymmetric part (23 13 12, as
-// FP64 real and imaginary parts) of e-hat at one wave vector
+// One direction's contribution to the symmetric (11 22 33 23 13 12) and
+// antisymmetric (23 13 12) parts of e-hat = -w u (x) n, u = Omega (sigma-hat n),
+// from the 6 stress components directly (no 3x3 tensors held: fewer registers
+// in the fused x-line kernel)
 template <typename R, typename C>
-__device__ __forceinline__ void green_point(const R *c0s, const double *s0, int elast_bc, int x, int y,
-                                            int kz, int nx, int ny, int nz, const C s6[6], C e6[6],
-                                            double wr[3], double wi[3])
+__device__ __forceinline__ void gamma_acc6(const R *ga, const double nd[3], const C s6[6], C e6[6],
+                                           C an[3], R w)
 {
-    C e[3][3];
-    green_tensor<R>(c0s, s0, elast_bc, x, y, kz, nx, ny, nz, s6, e);
+    const R n0 = (R)nd[0], n1 = (R)nd[1], n2 = (R)nd[2];
+    const R nn[6] = {n0 * n0, n1 * n1, n2 * n2, n1 * n2, n0 * n2, n0 * n1};
+    R a[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        R acc = 0;
+#pragma unroll
+        for (int p = 0; p < 6; p++) acc += ga[q * 6 + p] * nn[p];
+        a[q] = acc;
+    }
+    // symmetric adjugate and determinant of A = [a0 a5 a4; a5 a1 a3; a4 a3 a2]
+    const R o00 = a[1] * a[2] - a[3] * a[3], o11 = a[0] * a[2] - a[4] * a[4], o22 = a[0] * a[1] - a[5] * a[5];
+    const R o01 = a[4] * a[3] - a[5] * a[2], o02 = a[5] * a[3] - a[4] * a[1], o12 = a[4] * a[5] - a[0] * a[3];
+    const R f = w / (a[0] * o00 + a[5] * o01 + a[4] * o02);
+    // t = sigma-hat n (sigma symmetric: 11 22 33 23 13 12)
+    C t0, t1, t2;
+    t0.x = s6[0].x * n0 + s6[5].x * n1 + s6[4].x * n2;
+    t0.y = s6[0].y * n0 + s6[5].y * n1 + s6[4].y * n2;
+    t1.x = s6[5].x * n0 + s6[1].x * n1 + s6[3].x * n2;
+    t1.y = s6[5].y * n0 + s6[1].y * n1 + s6[3].y * n2;
+    t2.x = s6[4].x * n0 + s6[3].x * n1 + s6[2].x * n2;
+    t2.y = s6[4].y * n0 + s6[3].y * n1 + s6[2].y * n2;
+    C u0, u1, u2;
+    u0.x = (o00 * t0.x + o01 * t1.x + o02 * t2.x) * f;
+    u0.y = (o00 * t0.y + o01 * t1.y + o02 * t2.y) * f;
+    u1.x = (o01 * t0.x + o11 * t1.x + o12 * t2.x) * f;
+    u1.y = (o01 * t0.y + o11 * t1.y + o12 * t2.y) * f;
+    u2.x = (o02 * t0.x + o12 * t1.x + o22 * t2.x) * f;
+    u2.y = (o02 * t0.y + o12 * t1.y + o22 * t2.y) * f;
+    const R h = (R)0.5;
+    e6[0].x -= u0.x * n0; e6[0].y -= u0.y * n0;
+    e6[1].x -= u1.x * n1; e6[1].y -= u1.y * n1;
+    e6[2].x -= u2.x * n2; e6[2].y -= u2.y * n2;
+    e6[3].x -= (u1.x * n2 + u2.x * n1) * h; e6[3].y -= (u1.y * n2 + u2.y * n1) * h;
+    e6[4].x -= (u0.x * n2 + u2.x * n0) * h; e6[4].y -= (u0.y * n2 + u2.y * n0) * h;
+    e6[5].x -= (u0.x * n1 + u1.x * n0) * h; e6[5].y -= (u0.y * n1 + u1.y * n0) * h;
+    an[0].x -= (u1.x * n2 - u2.x * n1) * h; an[0].y -= (u1.y * n2 - u2.y * n1) * h;
+    an[1].x -= (u0.x * n2 - u2.x * n0) * h; an[1].y -= (u0.y * n2 - u2.y * n0) * h;
+    an[2].x -= (u0.x * n1 - u1.x * n0) * h; an[2].y -= (u0.y * n1 - u1.y * n0) * h;
+}
+
+// e6 / antisymmetric parts from a full e-hat
+template <typename R, typename C>
+__device__ __forceinline__ void split_e(const C e[3][3], C e6[6], double wr[3], double wi[3])
+{
     const int ia[6] = {0, 1, 2, 1, 0, 0}, ib[6] = {0, 1, 2, 2, 2, 1};
 #pragma unroll
     for (int c = 0; c < 6; c++) {
@@ -121,6 +165,43 @@ __device__ __forceinline__ void green_point(const R *c0s, const double *s0, int 
         int a = ia[c + 3], b = ib[c + 3];
         wr[c] = (double)((e[a][b].x - e[b][a].x) * (R)0.5);
         wi[c] = (double)((e[a][b].y - e[b][a].y) * (R)0.5);
+    }
+}
+
+// the k = 0 term: -S0 : sigma-hat when ElastBC == 0, else zero
+template <typename R, typename C>
+__device__ __forceinline__ void green_zero(const R *ga, const double *s0, int elast_bc, int nx, int ny,
+                                           int nz, const C s6[6], C e6[6], double wr[3], double wi[3])
+{
+    C e[3][3];
+    green_tensor<R>(ga, s0, elast_bc, 0, 0, 0, nx, ny, nz, s6, e);
+    split_e<R>(e, e6, wr, wi);
+}
+
+// symmetric part (11 22 33 23 13 12) and antisymmetric part (23 13 12, as
+// FP64 real and imaginary parts) of e-hat at one wave vector k != 0 (the
+// caller takes k = 0 to green_zero: kept apart, its code does not add to the
+// registers of the per-point loops)
+template <typename R, typename C>
+__device__ __forceinline__ void green_point(const R *ga, int x, int y, int kz, int nx, int ny, int nz,
+                                            const C s6[6], C e6[6], double wr[3], double wi[3])
+{
+    C an[3];
+#pragma unroll
+    for (int c = 0; c < 6; c++) e6[c] = C{(R)0, (R)0};
+#pragma unroll
+    for (int c = 0; c < 3; c++) an[c] = C{(R)0, (R)0};
+    double n1[3], n2[3];
+    if (!kspace::mode_dirs(x, y, kz, nx, ny, nz, n1, n2)) {
+        gamma_acc6<R>(ga, n1, s6, e6, an, (R)1);
+    } else {
+        gamma_acc6<R>(ga, n1, s6, e6, an, (R)0.5);
+        gamma_acc6<R>(ga, n2, s6, e6, an, (R)0.5);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        wr[c] = (double)an[c].x;
+        wi[c] = (double)an[c].y;
     }
 }
 
