@@ -202,7 +202,18 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
     // start on 128-byte boundaries, as ny*nzh is a multiple of TBX).
     // Line (c, j) = component c, column q0 + j; element x at spec[c*NK + x*ncol + q0 + j].
     const int ncol = ny * nzh;
-    const int q0 = blockIdx.x * TBX;
+    // Workgroups go to the 8 XCDs round robin (block b to XCD b mod 8), and
+    // each XCD has its own L2.  A tile row is 64 B when TBX complex values
+    // fill half a 128-byte line, so the two tiles that share the lines are
+    // given to blocks b and b + 8 (the same XCD, dispatched together):
+    // within each group of 16 blocks, block g*16 + r takes tile
+    // g*16 + 2 (r mod 8) + r / 8.
+    int tile = blockIdx.x;
+    if (blockIdx.x < gridDim.x / 16 * 16) {
+        const int g = blockIdx.x / 16, r = blockIdx.x % 16;
+        tile = g * 16 + (r % 8) * 2 + r / 8;
+    }
+    const int q0 = tile * TBX;
     const int nk = min(TBX, ncol - q0);
     for (int i = threadIdx.x; i < NL * NX; i += NTX) {
         const int c = i / (TBX * NX), r = i % (TBX * NX), x = r / TBX, j = r % TBX;
