@@ -22,8 +22,11 @@
 // Data placement: the 12x5 Schmid tensors of a voxel are derived at kernel
 // start from its rotation (72 B read instead of 480 B) and kept in a
 // wave-private LDS slab laid out [component][lane] (conflict-free), which
-// keeps them out of the register file; all other per-voxel arrays live in
-// registers with compile-time indices only.
+// keeps them out of the register file; C0 and S0 (packed triangles) are a
+// 42-entry LDS table read by broadcast; all other per-voxel arrays live in
+// registers with compile-time indices only.  One wave per SIMD: the solve
+// holds about 420 registers, so the code is shaped for latency (next Schmid
+// vector read ahead, constants hoisted, no scalar loops on the hot path).
 #include "evpgpu_impl.h"
 
 namespace {
@@ -71,8 +74,8 @@ template <int M> __device__ __forceinline__ double ipowc(double x)
 // Rate-law evaluation: RM_POW pow() for a non-integral exponent, RM_LOOP
 // repeated squaring for an integral one, RM_FAST the common case of every
 // crystal phase having EVPGPU_NSYSMX systems with one integral n <= 32 and
-// one reference rate (constants hoisted out of the system loop, no
-// per-system branch, straight-line squaring)
+// one reference rate (constants hoisted out of the system loop, straight-
+// line squaring; for n = 10 and n = 20 the exponent is a template constant)
 enum { RM_POW = 0, RM_LOOP = 1, RM_FAST = 2 };
 
 // chg_basis opt 2 for a symmetric tensor given as (11 22 33 23 13 12)
@@ -442,9 +445,10 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
     return 0;
 }
 
-// One lane per voxel.  Writes sig (in place), edot, gamdot; block partials
-// (NRED doubles per block) of err_s, err_e, Newton iterations, sum sig,
-// sum sig over phase 1.
+// One lane per voxel.  Reads lambda from sig_in and writes the new stress to
+// sig (separate buffers), edot and -- with FAST off -- gamdot; block partials
+// (NRED doubles per block) of err_s, err_e, Newton iterations, sum sig, sum
+// sig over phase 1 and line-search trials.
 template <int RM, bool CSYM, bool FAST, int NE>
 __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_newton(
     NewtonParams P, long long N, const int *__restrict__ phase, const int *__restrict__ orient,
