@@ -82,19 +82,31 @@ __device__ __forceinline__ void stages(C *lds, const C *tw)
     }
 }
 
+// Workgroups go to the 8 XCDs round robin (block b to XCD b mod 8), each
+// XCD with its own L2.  Tiles t and t + 1 share the 128-byte lines at their
+// boundary, so within each group of 16 blocks block g*16 + r takes tile
+// g*16 + 2 (r mod 8) + r / 8: the pair lands on blocks b and b + 8, one XCD.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    if (b >= nb / 16 * 16) return b;
+    const int g = b / 16, r = b % 16;
+    return g * 16 + (r % 8) * 2 + r / 8;
+}
+
 // ---- strided C2C pass: lines along an axis with element stride `es`.
-// Workgroup (bx, o): lines inner = bx*TB .. +TB of group o, group base
-// (o / O2) * S1 + (o % O2) * S2.
+// Tile (bx, o) of nbx x nout (1-D grid, XCD-aware order): lines inner =
+// bx*TB .. +TB of group o, group base (o / O2) * S1 + (o % O2) * S2.
 template <typename R, int L, bool INV>
-__global__ __launch_bounds__(NT) void k_fft_strided(typename CT<R>::T *data, int ninner,
+__global__ __launch_bounds__(NT) void k_fft_strided(typename CT<R>::T *data, int ninner, int nbx,
                                                     long long es, int O2, long long S1,
                                                     long long S2, const typename CT<R>::T *tw)
 {
     typedef typename CT<R>::T C;
     constexpr int P = L + 1;
     __shared__ C lds[TB * P];
-    const int t0 = blockIdx.x * TB;
-    const int o = blockIdx.y;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int t0 = (tile % nbx) * TB;
+    const int o = tile / nbx;
     const long long base = (long long)(o / O2) * S1 + (long long)(o % O2) * S2 + t0;
     const int nt = min(TB, ninner - t0);
     for (int e = threadIdx.x; e < L * TB; e += NT) {
@@ -201,8 +213,9 @@ template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[a
 template <typename R, bool INV>
 int pass_y(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 {
-    dim3 g((c->nzh + TB - 1) / TB, ncomp * c->nx);
-#define CALLY(L) hipLaunchKernelGGL((k_fft_strided<R, L, INV>), g, dim3(NT), 0, c->stream, d, c->nzh, \
+    const int nbx = (c->nzh + TB - 1) / TB;
+    dim3 g(nbx * ncomp * c->nx);
+#define CALLY(L) hipLaunchKernelGGL((k_fft_strided<R, L, INV>), g, dim3(NT), 0, c->stream, d, c->nzh, nbx, \
                                     (long long)c->nzh, 1, (long long)c->ny * c->nzh, 0LL, twp<R>(c, 1))
     FFT_DISPATCH(c->ny, CALLY);
 #undef CALLY
@@ -212,9 +225,10 @@ int pass_y(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 template <typename R, bool INV>
 int pass_x(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 {
-    dim3 g((c->nzh + TB - 1) / TB, ncomp * c->ny);
-#define CALLX(L) hipLaunchKernelGGL((k_fft_strided<R, L, INV>), g, dim3(NT), 0, c->stream, d, c->nzh, \
-                                    (long long)c->ny * c->nzh, c->ny, (long long)c->NK,             \
+    const int nbx = (c->nzh + TB - 1) / TB;
+    dim3 g(nbx * ncomp * c->ny);
+#define CALLX(L) hipLaunchKernelGGL((k_fft_strided<R, L, INV>), g, dim3(NT), 0, c->stream, d, c->nzh, nbx, \
+                                    (long long)c->ny * c->nzh, c->ny, (long long)c->NK,                 \
                                     (long long)c->nzh, twp<R>(c, 0))
     FFT_DISPATCH(c->nx, CALLX);
 #undef CALLX
