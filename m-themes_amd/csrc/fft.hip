@@ -121,7 +121,9 @@ __global__ __launch_bounds__(NT) void k_fft_strided(typename CT<R>::T *data, int
     }
 }
 
-// ---- z pass, real -> half spectrum (ncomp x N real -> ncomp x NK complex).
+// ---- z pass, real -> half spectrum (ncomp x N real -> ncomp x NK complex):
+// TB lines per workgroup, each transformed as an L/2-point complex FFT of
+// the packed pairs and split into its L/2 + 1 half-spectrum values.
 // IN 0: FP64 field; IN 1: FP64 field minus an FP32 anchor field (the mixed
 // loop's delta); IN 2: an FP32 field.
 template <typename R, int L, int IN>
@@ -130,29 +132,43 @@ __global__ __launch_bounds__(NT) void k_fft_z_r2c(const double *in, const float 
                                                   long long NK, const typename CT<R>::T *tw)
 {
     typedef typename CT<R>::T C;
-    constexpr int P = L + 1, LH = L / 2 + 1;
+    constexpr int M = L / 2, P = M + 1;
     __shared__ C lds[TB * P];
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TB;
-    const long long off = comp * N + line0 * L;
-    for (int e = threadIdx.x; e < L * TB; e += NT) {
-        int t = e / L, i = e % L;
-        R x;
-        if (IN == 0) x = (R)in[off + e];
-        else if (IN == 1) x = (R)(in[off + e] - (double)inA[off + e]);
-        else x = (R)inA[off + e];
-        lds[t * P + i] = C{x, (R)0};
+    const long long off = comp * N + line0 * L; // even: the pairs below are aligned
+    for (int e = threadIdx.x; e < M * TB; e += NT) {
+        const int t = e / M, n = e % M;
+        R a, b;
+        if (IN == 0) {
+            double2 v = reinterpret_cast<const double2 *>(in + off)[e];
+            a = (R)v.x;
+            b = (R)v.y;
+        } else if (IN == 1) {
+            double2 v = reinterpret_cast<const double2 *>(in + off)[e];
+            float2 w = reinterpret_cast<const float2 *>(inA + off)[e];
+            a = (R)(v.x - (double)w.x);
+            b = (R)(v.y - (double)w.y);
+        } else {
+            float2 w = reinterpret_cast<const float2 *>(inA + off)[e];
+            a = (R)w.x;
+            b = (R)w.y;
+        }
+        lds[t * P + n] = C{a, b};
     }
     __syncthreads();
-    stages<false, L, 0, 1>(lds, tw);
-    C *dst = out + comp * NK + line0 * LH;
-    for (int e = threadIdx.x; e < LH * TB; e += NT) {
-        int t = e / LH, k = e % LH;
-        dst[e] = lds[t * P + k];
-    }
+    // the M-point transforms: the length-L table read at stride 2
+    gstages<false, M, 0, 1, TB, P, 1, 0, NT, 2>(lds, tw);
+    for (int i = threadIdx.x; i < TB * (M / 2 + 1); i += NT)
+        split_forward<R, M>(lds + (i / (M / 2 + 1)) * P, i % (M / 2 + 1), tw);
+    __syncthreads();
+    C *dst = out + comp * NK + line0 * P;
+    for (int e = threadIdx.x; e < P * TB; e += NT) dst[e] = lds[e];
 }
 
-// ---- z pass, half spectrum -> real, Re(IFFT) of the Hermitian extension.
+// ---- z pass, half spectrum -> real, Re(IFFT) of the Hermitian extension
+// (the imaginary parts of k = 0 and k = L/2 dropped), through the inverse
+// split and an L/2-point complex FFT.
 // MODE 0: target[c] += add[c] + re*scale     (the strain update of
 //         src/evolution.c:1400-1401)
 // MODE 1: target[c]  = re*scale
@@ -167,29 +183,39 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, d
                                                   double scale, const typename CT<R>::T *tw)
 {
     typedef typename CT<R>::T C;
-    constexpr int P = L + 1, LH = L / 2 + 1;
+    constexpr int M = L / 2, P = M + 1;
     __shared__ C lds[TB * P];
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TB;
-    const C *src = in + comp * NK + line0 * LH;
-    for (int e = threadIdx.x; e < LH * TB; e += NT) {
-        int t = e / LH, k = e % LH;
-        C x = src[e];
-        lds[t * P + k] = x;
-        if (k > 0 && k < L / 2) lds[t * P + L - k] = C{x.x, -x.y};
-    }
+    const C *src = in + comp * NK + line0 * P;
+    for (int e = threadIdx.x; e < P * TB; e += NT) lds[e] = src[e];
     __syncthreads();
-    stages<true, L, 0, 1>(lds, tw);
-    double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
+    for (int i = threadIdx.x; i < TB * (M / 2 + 1); i += NT)
+        split_inverse<R, M>(lds + (i / (M / 2 + 1)) * P, i % (M / 2 + 1), tw);
+    __syncthreads();
+    gstages<true, M, 0, 1, TB, P, 1, 0, NT, 2>(lds, tw);
+    const double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
     const long long off = comp * N + line0 * L;
-    for (int e = threadIdx.x; e < L * TB; e += NT) {
-        int t = e / L, i = e % L;
-        double re = (double)lds[t * P + i].x;
-        if (MODE == 0) target[off + e] += add + re * scale;
-        else if (MODE == 1) target[off + e] = re * scale;
-        else if (MODE == 2) target[off + e] += add + (double)fa[off + e] + re * scale;
-        else if (MODE == 3) fo[off + e] = (float)(re * scale);
-        else target[off + e] = base[off + e] + re * scale;
+    for (int e = threadIdx.x; e < M * TB; e += NT) {
+        const int t = e / M, n = e % M;
+        const C z = lds[t * P + n];
+        const double r0 = (double)z.x * scale, r1 = (double)z.y * scale;
+        double2 *t2 = reinterpret_cast<double2 *>(target + off);
+        if (MODE == 0) {
+            double2 v = t2[e];
+            t2[e] = make_double2(v.x + (add + r0), v.y + (add + r1));
+        } else if (MODE == 1) {
+            t2[e] = make_double2(r0, r1);
+        } else if (MODE == 2) {
+            double2 v = t2[e];
+            float2 f = reinterpret_cast<const float2 *>(fa + off)[e];
+            t2[e] = make_double2(v.x + (add + (double)f.x + r0), v.y + (add + (double)f.y + r1));
+        } else if (MODE == 3) {
+            reinterpret_cast<float2 *>(fo + off)[e] = make_float2((float)r0, (float)r1);
+        } else {
+            double2 b = reinterpret_cast<const double2 *>(base + off)[e];
+            t2[e] = make_double2(b.x + r0, b.y + r1);
+        }
     }
 }
 

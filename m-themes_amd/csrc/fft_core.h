@@ -137,4 +137,55 @@ __device__ __forceinline__ void gstages(C *lds, const C *tw)
     }
 }
 
+// Real transforms of length 2M through M-point complex ones: the reals
+// (x_2n, x_2n+1) are packed as Z_n = x_2n + i x_2n+1; twz is the table of
+// length 2M (exp(-2 pi i m / 2M)).
+// z line of M packed complex values Z -> the M+1 half-spectrum values X of
+// the 2M reals (in place; thread pairs k and M-k)
+template <typename R, int M, typename C>
+__device__ __forceinline__ void split_forward(C *row, int k, const C *twz)
+{
+    if (k == 0) {
+        C z = row[0];
+        row[0] = C{z.x + z.y, (R)0};
+        row[M] = C{z.x - z.y, (R)0};
+        return;
+    }
+    C a = row[k], b = row[M - k];
+    // Fe = (Z_k + conj(Z_{M-k}))/2, Fo = (Z_k - conj(Z_{M-k}))/(2i)
+    C fe = {(a.x + b.x) * (R)0.5, (a.y - b.y) * (R)0.5};
+    C fo = {(a.y + b.y) * (R)0.5, -(a.x - b.x) * (R)0.5};
+    C w = twz[k]; // exp(-2 pi i k / 2M)
+    C wf = cmul(w, fo);
+    row[k] = cadd(fe, wf);
+    if (2 * k != M) row[M - k] = C{fe.x - wf.x, -(fe.y - wf.y)}; // conj(Fe - W^k Fo)
+}
+
+// the inverse: half spectrum X_0..X_M -> M packed complex values Z whose
+// inverse FFT gives (x_2n, x_2n+1) of the C2R of the Hermitian extension
+template <typename R, int M, typename C>
+__device__ __forceinline__ void split_inverse(C *row, int k, const C *twz)
+{
+    if (k == 0) {
+        R a = row[0].x, b = row[M].x; // imaginary parts dropped
+        row[0] = C{a + b, a - b};
+        return;
+    }
+    C xk = row[k], xm = row[M - k];
+    C w = twz[k];
+    w.y = -w.y; // exp(+2 pi i k / 2M)
+    // Z_k = (X_k + conj(X_{M-k})) + i w^k (X_k - conj(X_{M-k}))
+    C s = {xk.x + xm.x, xk.y - xm.y}, d = {xk.x - xm.x, xk.y + xm.y};
+    C wd = cmul(w, d);
+    C zk = {s.x - wd.y, s.y + wd.x};
+    if (2 * k != M) {
+        // Z_{M-k} = (X_{M-k} + conj(X_k)) + i w^{M-k} (X_{M-k} - conj(X_k)), w^{M-k} = -conj(w^k)
+        C s2 = {xm.x + xk.x, xm.y - xk.y}, d2 = {xm.x - xk.x, xm.y + xk.y};
+        C w2 = {-w.x, w.y};
+        C wd2 = cmul(w2, d2);
+        row[M - k] = C{s2.x - wd2.y, s2.y + wd2.x};
+    }
+    row[k] = zk;
+}
+
 } // namespace fftc
