@@ -201,6 +201,27 @@ def test_loop_parity_16_pores(job_factory):
     _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
 
 
+def test_loop_parity_full_size_128(tmp_path):
+    """The 128^3 job of the benchmark (512 grains, 2 % pores) at full size:
+    increment 1 forced to 3 Green iterations in the oracle (16 threads, a few
+    seconds) and on the GPU, FP64 to the loop bar and the mixed loop to
+    1e-6."""
+    inp = synth.write_job(str(tmp_path), synth.config_c2())
+    J = O.Job(inp)
+    J.run(1, forced=[3])
+    ref = J.field(O.SIG)
+    # the strain rate goes as sigma^n (n = 10): its relative error bar in the
+    # mixed loop is n times the stress's
+    for prec, tol, tol_edot in ((lib.FP64, 1e-10, 1e-10), (lib.MIXED, 1e-6, 1e-5)):
+        g = lib.GpuJob(inp, prec)
+        g.run(1, forced=3)
+        s = g.state()
+        assert rel(s["Sig"], ref) < tol, (prec, rel(s["Sig"], ref))
+        assert rel(s["Edot"], J.field(O.EDOT)) < tol_edot, (prec, rel(s["Edot"], J.field(O.EDOT)))
+        assert rel(s["DisGrad"], J.field(O.DISGRAD)) < tol, (prec, rel(s["DisGrad"], J.field(O.DISGRAD)))
+        g.close()
+
+
 def test_loop_parity_simple_shear(job_factory):
     """Fully strain-controlled simple shear (no stress-controlled component,
     lattice spin from the 12 shear rate) against the oracle.  (ElastBC = 0 is
