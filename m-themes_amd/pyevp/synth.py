@@ -74,6 +74,28 @@ class Material:
 
 
 @dataclasses.dataclass
+class DDLaw:
+    """Namelist keys of the dislocation-density law (src/io.c:47-93), phase I
+    values.  Defaults follow the reference's test_run/input.in, except the
+    temperature and the activation energies (1E-19 J), which are chosen so
+    that slip and climb are active at the stresses of these jobs."""
+
+    a0: float = 0.362                     # lattice parameter (nm)
+    L0: float = 0.003                     # voxel length of the GND rates (mm)
+    gnd_scale_ratio: float = 6000.0
+    T_K: float = 300.0
+    q_slip: float = 0.2
+    q_bulk: float = 1.4
+    c: tuple = (0.48, 3.0, 3.0, 800.0, 16.0, 1100.0, 1.25e-8, 0.24)  # C_1 .. C_8
+    # Selfinter, Coplanar, CrossSlip, GlissileJunction, HirthLock, LomerCottrellLock
+    inter: tuple = (0.125, 0.125, 0.725, 0.125, 0.05, 0.18)
+    rho_ssd_initial: float = 0.05         # 1E12/m^2
+    D_0: float = 4e-5
+    tau_drx: Optional[float] = 1000.0     # None: key absent (reads as 0)
+    delta_tau_drx: float = 0.0
+
+
+@dataclasses.dataclass
 class Config:
     """One synthetic EVPFFT job."""
 
@@ -100,6 +122,8 @@ class Config:
     phase2: Optional[Material] = None
     # (nx, ny, nz) of a non-cubic grid; None: n^3
     dims: Optional[tuple] = None
+    # dislocation-density law keys; None: the namelist holds none
+    dd: Optional[DDLaw] = None
 
     def shape(self) -> tuple:
         return tuple(int(d) for d in self.dims) if self.dims is not None else (self.n,) * 3
@@ -297,7 +321,8 @@ def write_sx(path: str, m: Material):
 
 
 def write_input(path: str, cfg: Config, sx_name="Cu.sx", ms_name="ms.txt"):
-    """Namelist in the layout of test_run/input.in (power-law keys only)."""
+    """Namelist in the layout of test_run/input.in (power-law keys, then the
+    dislocation-density keys when ``cfg.dd`` is set)."""
     m = cfg.material
     m2 = cfg.phase2
     two = cfg.pore_fraction > 0.0 or m2 is not None
@@ -326,6 +351,20 @@ def write_input(path: str, cfg: Config, sx_name="Cu.sx", ms_name="ms.txt"):
         "Hard_Flag\t%d" % cfg.hard_flag,
         "Tex_Flag\t0",
     ]
+    d = cfg.dd
+    if d is not None:
+        lines += [
+            "a0\t%r\t0.0" % d.a0, "L0\t%r" % d.L0, "GND_ScaleRatio\t%r" % d.gnd_scale_ratio,
+            "Q_slip\t%r\t0.0" % d.q_slip, "Q_bulk\t%r\t0.0" % d.q_bulk,
+            "rho_SSD_initial\t%r" % d.rho_ssd_initial, "T__K\t%r" % d.T_K,
+        ]
+        lines += ["C_%d\t%r\t0.0" % (i + 1, v) for i, v in enumerate(d.c)]
+        names = ("Selfinter", "Coplanar", "CrossSlip", "GlissileJunction", "HirthLock",
+                 "LomerCottrellLock")
+        lines += ["%s\t%r\t0.0" % (k, v) for k, v in zip(names, d.inter)]
+        lines += ["D_0\t%r" % d.D_0, "DeltaTau_DRX\t%r" % d.delta_tau_drx]
+        if d.tau_drx is not None:
+            lines.append("tau_DRX\t%r" % d.tau_drx)
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 

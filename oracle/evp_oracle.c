@@ -403,6 +403,146 @@ void orc_harden_voxel(int nsys, const double gamdot[], double dt,
 }
 
 /* ------------------------------------------------------------------ */
+/* dislocation-density law (DD_BASED_FLAG, DD_CU, FORWARD_PLUS_BACKWARD,
+ * DD_GND_FIRST_ORDER, PF_DRX with no DRX event yet: src/evpFlags.h)    */
+/* ------------------------------------------------------------------ */
+#define TRIAL_FREQ_DRX_REF 1.0E10            /* src/evp.h:80 */
+#define EV_IN_J_REF 1.60217733e-19           /* src/evp.h:81 */
+#define J_IN_EV_REF (1.0 / EV_IN_J_REF)      /* src/evp.h:82 */
+#define MY_J_IN_EV_REF 0.62415063631         /* src/evp.h:83 */
+#define BOLZ_REF 1.380658e-23                /* src/evp.h:84-85 */
+#define EPSILON_DD (5.0E-3)                  /* src/PFgrid.h:50, via src/pf_xrx.h:47 */
+
+/* kT of src/evp.h:86 */
+double orc_dd_kT(double T_K) { return (T_K * BOLZ_REF * J_IN_EV_REF); }
+
+/* StrainRate_Orowan (src/constitutive.c:507-721), FORWARD_PLUS_BACKWARD
+ * branch.  rho_m is read (passing stress) before it is recomputed, system by
+ * system, so the first call after the densities change uses the previous
+ * call's mobile density (Q22).  Returns ORC_ERR_NAN where the reference's
+ * assert(isnan(...)==0) aborts.                                         */
+int orc_strain_rate_dd(const double stress[6], double edot[6], double dedot[6][6], int nsys,
+                       const double sc[][5], const orc_dd_phase *ph, const double rho_s[],
+                       const double rho_g1[], const double rho_g2[], const double rho_g3[],
+                       double rho_m[], double rho_F[], double rho_P[], double gamdot[])
+{
+    const double tau_0 = 1000;
+    const double kT = ph->kT;
+    double gam[ORC_NSYSMX], gam2[ORC_NSYSMX];
+    const double (*cs)[ORC_NSYSMX] = ph->cth[0], (*cg1)[ORC_NSYSMX] = ph->cth[1];
+    const double (*cg2)[ORC_NSYSMX] = ph->cth[2], (*cg3)[ORC_NSYSMX] = ph->cth[3];
+    const double (*ss)[ORC_NSYSMX] = ph->cth[4], (*sg1)[ORC_NSYSMX] = ph->cth[5];
+    const double (*sg2)[ORC_NSYSMX] = ph->cth[6], (*sg3)[ORC_NSYSMX] = ph->cth[7];
+    for (int i = 0; i < nsys; i++) {
+        rho_F[i] = 0.0;
+        rho_P[i] = 0.0;
+        for (int j = 0; j < nsys; j++) {
+            rho_F[i] += ph->chi[i][j] * (rho_s[j] * cs[i][j] + fabs(rho_g1[j] * cg1[i][j]) +
+                                         fabs(rho_g2[j] * cg2[i][j]) + fabs(rho_g3[j] * cg3[i][j]));
+            rho_P[i] += ph->chi[i][j] * (rho_s[j] * ss[i][j] + fabs(rho_g1[j] * sg1[i][j]) +
+                                         fabs(rho_g2[j] * sg2[i][j]) + fabs(rho_g3[j] * sg3[i][j]));
+        }
+    }
+    for (int i = 0; i < nsys; i++) {
+        const double b = ph->bbB[i];
+        double tau_pass = ph->c1 * ph->G * b * sqrt(fabs(rho_P[i] + rho_m[i])) * 1.0E-3;
+        tau_pass += ph->drx;
+        double rss = sc[i][0] * stress[0] + sc[i][1] * stress[1] + sc[i][2] * stress[2] +
+                     sc[i][3] * stress[3] + sc[i][4] * stress[4];
+        /* the reference leaves tau_eff unset below the passing stress, where
+         * it is not used */
+        double tau_eff = 0.0;
+        if (fabs(rss) > tau_pass) tau_eff = (fabs(rss) - tau_pass);
+        double tau_cut = (kT / MY_J_IN_EV_REF) * sqrt(rho_F[i]) / ph->c2 / ph->c3 / b / b * 0.1;
+        rho_m[i] = 2.0 * kT / ph->c1 / ph->c2 / ph->c3 / ph->G / (pow(b, 3.0)) *
+                   sqrt(rho_P[i] * rho_F[i]) * 160.2177;
+        double gam_o = 1E-3 * (rho_m[i]) * TRIAL_FREQ_DRX_REF * ph->c2 * b * exp(-1.0 * ph->q_slip) /
+                       sqrt(rho_F[i]);
+        double t = (fabs(rss) > tau_pass) ? gam_o * sinh(tau_eff / (tau_0 + tau_cut)) : 0.0;
+        if (isnan(t)) return ORC_ERR_NAN;
+        gam[i] = t * ((double)(2 * (rss > 0) - 1));
+        t = (fabs(rss) > tau_pass) ? gam_o * cosh(tau_eff / (tau_cut + tau_0)) / (tau_cut + tau_0)
+                                   : 0.0;
+        if (isnan(t)) return ORC_ERR_NAN;
+        gam2[i] = t * (fabs(rss) > tau_pass);
+        gamdot[i] = gam[i];
+    }
+    for (int i = 0; i < 5; i++) {
+        double s = 0.;
+        for (int j = 0; j < nsys; j++) s += sc[j][i] * gam[j];
+        if (isnan(s)) return ORC_ERR_NAN;
+        edot[i] = s;
+    }
+    edot[5] = 0.0;
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+            double s = 0.0;
+            for (int k = 0; k < nsys; k++) s += sc[k][i] * sc[k][j] * gam2[k];
+            dedot[i][j] = s;
+        }
+    for (int i = 0; i < 6; i++) {
+        dedot[i][5] = 0.0;
+        dedot[5][i] = 0.0;
+    }
+    return ORC_OK;
+}
+
+/* InteractionStrengthMatrix("FCC", ...) (src/init.c:593-702): the upper
+ * triangle as interaction types, row i listing j = i..11 (S self, C
+ * coplanar, X cross slip, G glissile junction, H Hirth lock, L
+ * Lomer-Cottrell lock; Arsenlis and Parks, JMPS 50 (2002) 1979), then
+ * mirrored.  The values come from the namelist in the order Selfinter,
+ * Coplanar, CrossSlip, GlissileJunction, HirthLock, LomerCottrellLock.    */
+void orc_dd_interaction(const double inter[6], double chi[12][12])
+{
+    static const char *rows[12] = {"SCCHGLXGGHLG", "SCGXGGHLLHG", "SLGHGLHGGX", "SCCHLGXGG",
+                                   "SCLHGGHL",     "SGGXGLH",     "SCCHGL",     "SCGXG",
+                                   "SLGH",         "SCC",         "SC",         "S"};
+    static const char *codes = "SCXGHL";
+    for (int i = 0; i < 12; i++)
+        for (int k = 0; rows[i][k]; k++) {
+            int t = (int)(strchr(codes, rows[i][k]) - codes);
+            chi[i][i + k] = inter[t];
+        }
+    for (int i = 1; i < 12; i++)
+        for (int j = 0; j < i; j++) chi[i][j] = chi[j][i];
+}
+
+/* projection tables (src/init.c:1108-1125): cos/sin of the angles between
+ * plane normal i and, for system j, the screw line direction n x b (SSD,
+ * GND type 2), b (GND type 1) and n (GND type 3).  Order: cos s, g1, g2, g3,
+ * sin s, g1, g2, g3.                                                        */
+void orc_dd_angles(int nsys, const double bn[][3], const double bb[][3], double cth[8][12][12])
+{
+    for (int i = 0; i < nsys; i++)
+        for (int j = 0; j < nsys; j++) {
+            double sv[3];
+            sv[0] = bn[j][1] * bb[j][2] - bn[j][2] * bb[j][1];
+            sv[1] = bn[j][2] * bb[j][0] - bn[j][0] * bb[j][2];
+            sv[2] = bn[j][0] * bb[j][1] - bn[j][1] * bb[j][0];
+            double c = fabs(bn[i][0] * sv[0] + bn[i][1] * sv[1] + bn[i][2] * sv[2]);
+            cth[0][i][j] = c;
+            cth[4][i][j] = fabs(sqrt(fabs(1.0 - c * c)));
+            c = fabs(bn[i][0] * bb[j][0] + bn[i][1] * bb[j][1] + bn[i][2] * bb[j][2]);
+            cth[1][i][j] = c;
+            cth[5][i][j] = fabs(sqrt(fabs(1.0 - c * c)));
+            cth[2][i][j] = cth[0][i][j];
+            cth[6][i][j] = cth[4][i][j];
+            c = fabs(bn[i][0] * bn[j][0] + bn[i][1] * bn[j][1] + bn[i][2] * bn[j][2]);
+            cth[3][i][j] = c;
+            cth[7][i][j] = fabs(sqrt(fabs(1.0 - c * c)));
+        }
+}
+
+/* shear modulus of the DD build (src/init.c:826): the Hill average of the
+ * cubic Voigt and Reuss bounds, as the reference writes it                 */
+double orc_dd_shear_modulus(const double ec[3])
+{
+    return 0.1 * (ec[0] - ec[1] + 3 * ec[2]) +
+           (0.5) * (5 * (ec[0] - ec[1]) * ec[2]) / (4 * ec[2] + 3 * (ec[0] - ec[1]));
+}
+
+/* ------------------------------------------------------------------ */
 /* per-voxel stress update (src/evolution.c:2-41, 44-310)             */
 /* ------------------------------------------------------------------ */
 typedef struct {
@@ -415,12 +555,30 @@ typedef struct {
     const double (*crss)[2];
     const double *nsr, *gam0;
     double *gamdot;
+    /* dislocation-density law: phase constants and this voxel's densities */
+    const orc_dd_phase *dd;
+    const double *rho_s, *rho_g1, *rho_g2, *rho_g3;
+    double *rho_m, *rho_F, *rho_P;
+    int status;
 } nrctx;
 
-static double nr_quality(const nrctx *c, const double sig[6])
+/* the rate law of the build: StrainRate_Orowan with DD_BASED_FLAG, else
+ * StrainRate_eval (src/evolution.c:10-18, 116-125)                      */
+static void nr_rate(nrctx *c, const double sig[6], double edot[6], double dedot[6][6])
+{
+    if (c->dd) {
+        int st = orc_strain_rate_dd(sig, edot, dedot, c->nsys, c->sc, c->dd, c->rho_s, c->rho_g1,
+                                    c->rho_g2, c->rho_g3, c->rho_m, c->rho_F, c->rho_P, c->gamdot);
+        if (st && !c->status) c->status = st;
+    } else {
+        orc_strain_rate(sig, edot, dedot, c->nsys, c->sc, c->crss, c->nsr, c->gam0, c->gamdot);
+    }
+}
+
+static double nr_quality(nrctx *c, const double sig[6])
 {
     double edotp[6], dd[6][6], tot[6], res[6];
-    orc_strain_rate(sig, edotp, dd, c->nsys, c->sc, c->crss, c->nsr, c->gam0, c->gamdot);
+    nr_rate(c, sig, edotp, dd);
     for (int i = 0; i < 6; i++) {
         tot[i] = c->eps[i] + edotp[i] * c->dt;
         for (int j = 0; j < 6; j++) tot[i] += c->sg[i][j] * sig[j];
@@ -444,12 +602,14 @@ void orc_set_newton_tol(double tol, double tolx)
     g_tolx = tolx;
 }
 
-int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6], double dt,
-                            double sig[3][3], const double eps[3][3],
-                            const double disgrad[3][3], int nsys, const double sc[][5],
-                            const double crss[][2], const double nsr[], const double gam0[],
-                            double edot_out[3][3], double gamdot[], double *err_s,
-                            double *err_e, int *n_iter)
+/* UpdateStress for one voxel; dd (may be NULL) selects the dislocation-
+ * density law and carries its phase constants and the voxel's densities. */
+static int update_stress(const double sg66[6][6], const double c066[6][6], double dt,
+                         double sig[3][3], const double eps[3][3], const double disgrad[3][3],
+                         int nsys, const double sc[][5], const double crss[][2],
+                         const double nsr[], const double gam0[], double edot_out[3][3],
+                         double gamdot[], double *err_s, double *err_e, int *n_iter,
+                         const nrctx *ddv)
 {
     double xl_t[3][3], st_t[3][3];
     double xl6[6], sig6[6], eps6[6], str6[6], old6[6];
@@ -470,13 +630,24 @@ int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6], do
     signorm = sqrt(signorm);
 
     nrctx c = {sg66, c066, xl6, eps6, str6, dt, nsys, sc, crss, nsr, gam0, gamdot};
+    if (ddv) {
+        c.dd = ddv->dd;
+        c.rho_s = ddv->rho_s;
+        c.rho_g1 = ddv->rho_g1;
+        c.rho_g2 = ddv->rho_g2;
+        c.rho_g3 = ddv->rho_g3;
+        c.rho_m = ddv->rho_m;
+        c.rho_F = ddv->rho_F;
+        c.rho_P = ddv->rho_P;
+    }
     const double erroral = g_nr_tol;
     int iterl = 0;
     double erral = 10 * erroral;
     while ((iterl < 500) && (fabs(erral) > fabs(erroral))) {
         iterl++;
         for (int i = 0; i < 6; i++) old6[i] = sig6[i];
-        orc_strain_rate(sig6, edotp6, dd, nsys, sc, crss, nsr, gam0, gamdot);
+        nr_rate(&c, sig6, edotp6, dd);
+        if (c.status) return c.status;
         for (int i = 0; i < 6; i++) {
             tot[i] = eps6[i] + edotp6[i] * dt;
             for (int j = 0; j < 6; j++) tot[i] += sg66[i][j] * sig6[j];
@@ -521,6 +692,7 @@ int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6], do
         while (1) {
             for (int i = 0; i < 6; i++) sig6[i] = old6[i] + alam * step[i];
             double f_new = nr_quality(&c, sig6);
+            if (c.status) return c.status;
             if (alam < alamin) {
                 for (int i = 0; i < 6; i++) sig6[i] = old6[i];
                 break;
@@ -571,6 +743,17 @@ int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6], do
     *err_e = conv_e;
     *n_iter = iterl;
     return ORC_OK;
+}
+
+int orc_update_stress_voxel(const double sg66[6][6], const double c066[6][6], double dt,
+                            double sig[3][3], const double eps[3][3],
+                            const double disgrad[3][3], int nsys, const double sc[][5],
+                            const double crss[][2], const double nsr[], const double gam0[],
+                            double edot_out[3][3], double gamdot[], double *err_s,
+                            double *err_e, int *n_iter)
+{
+    return update_stress(sg66, c066, dt, sig, eps, disgrad, nsys, sc, crss, nsr, gam0, edot_out,
+                         gamdot, err_s, err_e, n_iter, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -693,6 +876,15 @@ struct orc_job {
     int CREEP_FLAG, istep;
     double green_time, green_iters, newton_iters;
     int status;
+    /* dislocation-density law: namelist (src/io.c:47-93), per-phase tables
+     * (src/init.c:165-203, 1040-1125) and per-voxel densities (1E12/m^2)  */
+    int law;
+    double a0[2], L0, GND_ScaleRatio, T_K, Q_slip[2], Q_bulk[2], Ck[8][2], Inter[6][2];
+    double rho_SSD_initial, D_0, tau_DRX, DeltaTau_DRX;
+    double Shear_G[2], bb_B[2][12], bp[2][12][3], Chi[2][12][12], cth[2][8][12][12];
+    double (*rho_s)[12], (*rho_m)[12], (*rho_g1)[12], (*rho_g2)[12], (*rho_g3)[12];
+    double (*rho_F)[12], (*rho_P)[12], (*gadot_grad)[12][3], *t_last;
+    unsigned char (*gbc)[6]; /* GB_checkX/Y/Z [0] and [1] */
 };
 
 #define LOOPV for (int v = 0; v < J->Nxyz; v++)
@@ -731,9 +923,21 @@ static int parse_namelist(orc_job *J, const char *path)
         {"ElastConst_PhaseII", 1, J->ElastConst[1], 4},
         {"Slip_PhaseI", 2, J->Slip[0], 1}, {"Slip_PhaseII", 2, J->Slip[1], 1},
         {"initial_ms", 2, J->initial_ms, 1},
+        /* DD_BASED_FLAG keys (src/io.c:47-68) and the two DRX keys the
+         * passing stress reads (src/io.c:86-88) */
+        {"a0", 1, J->a0, 2}, {"L0", 1, &J->L0, 1}, {"GND_ScaleRatio", 1, &J->GND_ScaleRatio, 1},
+        {"T__K", 1, &J->T_K, 1}, {"Q_slip", 1, J->Q_slip, 2}, {"Q_bulk", 1, J->Q_bulk, 2},
+        {"C_1", 1, J->Ck[0], 2}, {"C_2", 1, J->Ck[1], 2}, {"C_3", 1, J->Ck[2], 2},
+        {"C_4", 1, J->Ck[3], 2}, {"C_5", 1, J->Ck[4], 2}, {"C_6", 1, J->Ck[5], 2},
+        {"C_7", 1, J->Ck[6], 2}, {"C_8", 1, J->Ck[7], 2}, {"Selfinter", 1, J->Inter[0], 2},
+        {"Coplanar", 1, J->Inter[1], 2}, {"CrossSlip", 1, J->Inter[2], 2},
+        {"GlissileJunction", 1, J->Inter[3], 2}, {"HirthLock", 1, J->Inter[4], 2},
+        {"LomerCottrellLock", 1, J->Inter[5], 2}, {"rho_SSD_initial", 1, &J->rho_SSD_initial, 1},
+        {"D_0", 1, &J->D_0, 1}, {"tau_DRX", 1, &J->tau_DRX, 1},
+        {"DeltaTau_DRX", 1, &J->DeltaTau_DRX, 1},
     };
     const int nt = (int)(sizeof(tab) / sizeof(tab[0]));
-    int seen[32] = {0};
+    int seen[64] = {0};
     char buff[200];
     FILE *fp = fopen(path, "r");
     if (!fp) return ORC_ERR_FILE;
@@ -914,6 +1118,23 @@ static int read_ms(orc_job *J)
 
 #define ALLOC(p, n) (p) = calloc((size_t)(n), sizeof(*(p)))
 
+static int g_law = ORC_LAW_POWER;
+void orc_set_law(int law) { g_law = law; }
+static void dd_grain_checks(orc_job *J);
+
+/* the DD build's per-phase slip data (src/init.c:179-203): Burgers length
+ * a0/sqrt(2) (FCC) and the sense vector p = b x n of the normalised axes   */
+static void dd_phase_slip(orc_job *J, int ph)
+{
+    for (int j = 0; j < J->nSYS[ph]; j++) {
+        const double *n = J->bn[ph][j], *b = J->bb[ph][j];
+        J->bb_B[ph][j] = J->a0[ph] / sqrt(2.0);
+        J->bp[ph][j][0] = b[1] * n[2] - b[2] * n[1];
+        J->bp[ph][j][1] = b[2] * n[0] - b[0] * n[2];
+        J->bp[ph][j][2] = b[0] * n[1] - b[1] * n[0];
+    }
+}
+
 orc_job *orc_job_setup(const char *input_path, int *status)
 {
     orc_job *J = calloc(1, sizeof(orc_job));
@@ -957,16 +1178,50 @@ orc_job *orc_job_setup(const char *input_path, int *status)
                     J->g[v][2] = t[2] * s;
                 }
     }
-    /* phases (src/init.c:815-862) */
+    /* phases (src/init.c:815-862).  The DD build (DD_CU) replaces phase I's
+     * constants by copper's and zeroes phase II's; both shear moduli come
+     * from phase I (src/init.c:821-826, 840-845).                          */
+    J->law = g_law;
     if (!J->Type_phases[0]) {
+        if (J->law == ORC_LAW_DD) {
+            J->ElastConst[0][0] = 241.1674 * 1000;
+            J->ElastConst[0][1] = 89.1989 * 1000;
+            J->ElastConst[0][2] = 151.9685 * 1000;
+            J->Shear_G[0] = orc_dd_shear_modulus(J->ElastConst[0]);
+        }
         cubic_stiffness(J->ElastConst[0], J->Cijkl[0]);
         if ((st = parse_sx(J, J->Slip[0], 0))) goto fail;
+        if (J->law == ORC_LAW_DD) dd_phase_slip(J, 0);
     }
     if (J->N_phases == 2 && !J->Type_phases[1]) {
+        if (J->law == ORC_LAW_DD) {
+            J->ElastConst[1][0] = J->ElastConst[1][1] = J->ElastConst[1][2] = 0.0;
+            J->Shear_G[1] = orc_dd_shear_modulus(J->ElastConst[0]);
+        }
         cubic_stiffness(J->ElastConst[1], J->Cijkl[1]);
         if ((st = parse_sx(J, J->Slip[1], 1))) goto fail;
+        if (J->law == ORC_LAW_DD) dd_phase_slip(J, 1);
     }
     if ((st = read_ms(J))) goto fail;
+    if (J->law == ORC_LAW_DD) {
+        /* densities, interaction matrix, projection tables and grain
+         * checks (src/init.c:1025, 1073-1128) */
+        ALLOC(J->rho_s, N); ALLOC(J->rho_m, N); ALLOC(J->rho_g1, N); ALLOC(J->rho_g2, N);
+        ALLOC(J->rho_g3, N); ALLOC(J->rho_F, N); ALLOC(J->rho_P, N); ALLOC(J->gadot_grad, N);
+        ALLOC(J->t_last, N); ALLOC(J->gbc, N);
+        for (int v = 0; v < N; v++) {
+            J->t_last[v] = -1E3 * J->tau_DRX * J->TimeStep;
+            for (int i = 0; i < ORC_NSYSMX; i++) J->rho_s[v][i] = J->rho_SSD_initial;
+        }
+        for (int ph = 0; ph < ORC_NPHMAX; ph++) {
+            double in6[6];
+            for (int t = 0; t < 6; t++) in6[t] = J->Inter[t][ph];
+            orc_dd_interaction(in6, J->Chi[ph]);
+            orc_dd_angles(J->nSYS[ph], (const double(*)[3])J->bn[ph], (const double(*)[3])J->bb[ph],
+                          J->cth[ph]);
+        }
+        dd_grain_checks(J);
+    }
 
     /* boundary conditions (src/init.c:864-900) */
     if ((J->VelGrad_BC_Flag[0] + J->VelGrad_BC_Flag[1] + J->VelGrad_BC_Flag[2]) == 2) {
@@ -1055,7 +1310,9 @@ void orc_job_free(orc_job *J)
     if (!J) return;
     void *ptrs[] = {J->grain, J->phase, J->C_gr, J->sg, J->TranMat, J->Sig, J->Eps, J->Edot,
                     J->DisGrad, J->VelGrad, J->Schm_gr, J->crss, J->gamdot, J->gamacum,
-                    J->kbuf, J->ksr, J->ksi, J->g, J->err_s_v, J->err_e_v, J->nr_v};
+                    J->kbuf, J->ksr, J->ksi, J->g, J->err_s_v, J->err_e_v, J->nr_v,
+                    J->rho_s, J->rho_m, J->rho_g1, J->rho_g2, J->rho_g3, J->rho_F, J->rho_P,
+                    J->gadot_grad, J->t_last, J->gbc};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(J);
 }
@@ -1183,6 +1440,148 @@ static double now_s(void)
 }
 
 /* one Green iteration (src/evolution.c:1355-1462) */
+/* the rate-law context of crystal voxel v: phase constants, the DRX term of
+ * the passing stress (src/constitutive.c:584-586; t_last stays at its
+ * initial value before any DRX event) and the voxel's densities           */
+static void dd_context(orc_job *J, int v, orc_dd_phase *ph, nrctx *dd)
+{
+    const int p = J->phase[v] - 1;
+    const double kT = orc_dd_kT(J->T_K);
+    ph->c1 = J->Ck[0][p];
+    ph->c2 = J->Ck[1][p];
+    ph->c3 = J->Ck[2][p];
+    ph->G = J->Shear_G[p];
+    ph->q_slip = J->Q_slip[p] * MY_J_IN_EV_REF / kT;
+    ph->kT = kT;
+    ph->drx = J->DeltaTau_DRX * exp((J->t_last[v] - J->TimeTot) / J->TimeStep / J->tau_DRX);
+    ph->bbB = J->bb_B[p];
+    ph->chi = (const double(*)[ORC_NSYSMX])J->Chi[p];
+    ph->cth = (const double(*)[ORC_NSYSMX][ORC_NSYSMX])J->cth[p];
+    memset(dd, 0, sizeof(*dd));
+    dd->dd = ph;
+    dd->rho_s = J->rho_s[v];
+    dd->rho_g1 = J->rho_g1[v];
+    dd->rho_g2 = J->rho_g2[v];
+    dd->rho_g3 = J->rho_g3[v];
+    dd->rho_m = J->rho_m[v];
+    dd->rho_F = J->rho_F[v];
+    dd->rho_P = J->rho_P[v];
+}
+
+/* Gradient_ExchangeGrainID, one slab (src/constitutive.c:724-873): whether
+ * the +x, -x, +y, -y, +z neighbours share the voxel's grain; the sixth flag
+ * compares with the -y neighbour, as the reference does (Q23)             */
+static void dd_grain_checks(orc_job *J)
+{
+    const int nx = J->CellDim[0], ny = J->CellDim[1], nz = J->CellDim[2];
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                int xp = (x + 1) % nx, xm = (x + nx - 1) % nx, yp = (y + 1) % ny,
+                    ym = (y + ny - 1) % ny, zp = (z + 1) % nz;
+                int v = (x * ny + y) * nz + z, g = J->grain[v];
+                J->gbc[v][0] = g == J->grain[(xp * ny + y) * nz + z];
+                J->gbc[v][1] = g == J->grain[(xm * ny + y) * nz + z];
+                J->gbc[v][2] = g == J->grain[(x * ny + yp) * nz + z];
+                J->gbc[v][3] = g == J->grain[(x * ny + ym) * nz + z];
+                J->gbc[v][4] = g == J->grain[(x * ny + y) * nz + zp];
+                J->gbc[v][5] = g == J->grain[(x * ny + ym) * nz + z];
+            }
+}
+
+/* one-sided or central difference of f along one axis, by the two grain
+ * checks (src/constitutive.c:1012-1053): 0 when both neighbours belong to
+ * other grains                                                             */
+static double dd_diff(int same_p, int same_m, double f, double fp, double fm)
+{
+    if (same_p == 0 && same_m == 1) return (f - fm);
+    if (same_p == 1 && same_m == 0) return (fp - f);
+    if (same_p == 1 && same_m == 1) return (fp - fm) / 2.0;
+    return 0.0;
+}
+
+/* Gradient_ShearRate, one slab (src/constitutive.c:875-1108): the serial
+ * exchange buffers hold the opposite x faces, so every voxel sees periodic
+ * neighbours                                                               */
+static void dd_gradient(orc_job *J)
+{
+    const int nx = J->CellDim[0], ny = J->CellDim[1], nz = J->CellDim[2];
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                int xp = (x + 1) % nx, xm = (x + nx - 1) % nx, yp = (y + 1) % ny,
+                    ym = (y + ny - 1) % ny, zp = (z + 1) % nz, zm = (z + nz - 1) % nz;
+                int v = (x * ny + y) * nz + z;
+                int vxp = (xp * ny + y) * nz + z, vxm = (xm * ny + y) * nz + z;
+                int vyp = (x * ny + yp) * nz + z, vym = (x * ny + ym) * nz + z;
+                int vzp = (x * ny + y) * nz + zp, vzm = (x * ny + y) * nz + zm;
+                const unsigned char *k = J->gbc[v];
+                for (int i = 0; i < ORC_NSYSMX; i++) {
+                    double f = J->gamdot[v][i];
+                    J->gadot_grad[v][i][0] = dd_diff(k[0], k[1], f, J->gamdot[vxp][i], J->gamdot[vxm][i]);
+                    J->gadot_grad[v][i][1] = dd_diff(k[2], k[3], f, J->gamdot[vyp][i], J->gamdot[vym][i]);
+                    J->gadot_grad[v][i][2] = dd_diff(k[4], k[5], f, J->gamdot[vzp][i], J->gamdot[vzm][i]);
+                }
+            }
+}
+
+/* DislocationEvolution (src/constitutive.c:1346-1520): SSD production,
+ * annihilation and climb, first-order GND rates from the shear-rate
+ * gradient.  The climb branch for istep > 20000000 and the diagnostic
+ * rho_dipole / rho_forest sums are not restated (Q24).                    */
+static void dd_evolution(orc_job *J)
+{
+    const double kT = orc_dd_kT(J->T_K), dt = J->TimeStep;
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < J->Nxyz; v++) {
+        const int jph = J->phase[v];
+        if (J->Type_phases[jph - 1]) continue;
+        const int p = jph - 1;
+        const double c_4 = J->Ck[3][p], c_5 = J->Ck[4][p], c_6 = J->Ck[5][p], c_7 = J->Ck[6][p];
+        const double q_bulk = J->Q_bulk[p] * MY_J_IN_EV_REF / kT;
+        const double shear_G = J->Shear_G[p], Poisson_nu = J->ElastConst[0][3];
+        double sig6[6], rss[ORC_NSYSMX], hydro[ORC_NSYSMX];
+        orc_t2_to_v6(J->Sig[v], sig6);
+        const int ns = J->nSYS[p];
+        for (int i = 0; i < ns; i++) {
+            const double *sc = J->Schm_gr[v][i];
+            rss[i] = sc[0] * sig6[0] + sc[1] * sig6[1] + sc[2] * sig6[2] + sc[3] * sig6[3] +
+                     sc[4] * sig6[4];
+            hydro[i] = (sig6[0] + sig6[1] + sig6[2]) / 3.0;
+        }
+        for (int i = 0; i < ns; i++) {
+            const double b = J->bb_B[p][i];
+            double d_dipole = sqrt(3.0) * shear_G * b / 16.0 / PI_REF / (1 - Poisson_nu) /
+                              (fabs(rss[i]) + EPSILON_DD);
+            double dot_SSD = 0.0;
+            if (J->istep <= 2000000) {
+                double rs = J->rho_s[v][i];
+                dot_SSD = fabs(J->gamdot[v][i]) *
+                              (c_4 * sqrt(J->rho_F[v][i]) + c_6 * (d_dipole)*J->rho_m[v][i] - c_5 * rs) -
+                          c_7 * 1.6 * 1E12 * 1E-9 * rs * rs -
+                          4 * 1E12 * (d_dipole) * (J->D_0 / b) * pow(rs, 2.0) *
+                              exp((fabs(hydro[i]) * b * b * b * 100 / (1.38 * J->T_K)) - 1.0) *
+                              exp(-1.0 * q_bulk);
+                J->rho_s[v][i] += dot_SSD * dt;
+            }
+            if (J->istep <= 200000000) {
+                const double *g = J->gadot_grad[v][i];
+                const double *bp = J->bp[p][i], *bb = J->bb[p][i];
+                double gnd1 = (g[0] * bp[0] + g[1] * bp[1] + g[2] * bp[2]);
+                double dot_GND = gnd1 / b / J->L0;
+                J->rho_g1[v][i] += dot_GND * dt;
+                gnd1 = (g[0] * bb[0] + g[1] * bb[1] + g[2] * bb[2]);
+                dot_GND = -1.0 * gnd1 / b / J->L0;
+                J->rho_g2[v][i] += dot_GND * dt;
+                gnd1 = 0.0;
+                dot_GND = gnd1;
+                J->rho_g3[v][i] += dot_GND * dt;
+            }
+        }
+    }
+}
+
 static int green_iteration(orc_job *J)
 {
     const int N = J->Nxyz;
@@ -1226,12 +1625,15 @@ static int green_iteration(orc_job *J)
         J->nr_v[v] = 0;
         J->err_s_v[v] = J->err_e_v[v] = 0.0;
         if (!J->Type_phases[jp - 1]) {
-            int st = orc_update_stress_voxel(
+            orc_dd_phase ph;
+            nrctx dd;
+            if (J->law == ORC_LAW_DD) dd_context(J, v, &ph, &dd);
+            int st = update_stress(
                 (const double(*)[6])J->sg[v], (const double(*)[6])J->C066, J->TimeStep, J->Sig[v],
                 (const double(*)[3])J->Eps[v], (const double(*)[3])J->DisGrad[v], J->nSYS[jp - 1],
                 (const double(*)[5])J->Schm_gr[v], (const double(*)[2])J->crss[v], J->nSRS[jp - 1],
                 J->gam0[jp - 1], J->Edot[v], J->gamdot[v], &J->err_s_v[v], &J->err_e_v[v],
-                &J->nr_v[v]);
+                &J->nr_v[v], J->law == ORC_LAW_DD ? &dd : NULL);
             if (st) {
 #pragma omp critical
                 st_all = st;
@@ -1362,7 +1764,12 @@ static int increment(orc_job *J, int forced)
             memcpy(J->TranMat[v], tm, sizeof(t2));
         }
     }
-    if (J->Hard_Flag == 1 && J->istep > 1) {
+    if (J->law == ORC_LAW_DD) {
+        /* src/evolution.c:1551-1564: harden() is not called; the gradient
+         * of the shear rates (Hard_Flag, istep > 1), then the densities */
+        if (J->Hard_Flag == 1 && J->istep > 1 && J->istep <= 20000000) dd_gradient(J);
+        dd_evolution(J);
+    } else if (J->Hard_Flag == 1 && J->istep > 1) {
 #pragma omp parallel for schedule(static)
         for (int v = 0; v < N; v++) {
             int jp = J->phase[v];
@@ -1420,8 +1827,34 @@ double *orc_job_field(orc_job *J, int which)
     case 6: return &J->crss[0][0][0];
     case 7: return &J->gamdot[0][0];
     case 8: return J->gamacum;
+    case 9: return J->rho_s ? &J->rho_s[0][0] : NULL;
+    case 10: return J->rho_m ? &J->rho_m[0][0] : NULL;
+    case 11: return J->rho_g1 ? &J->rho_g1[0][0] : NULL;
+    case 12: return J->rho_g2 ? &J->rho_g2[0][0] : NULL;
+    case 13: return J->rho_g3 ? &J->rho_g3[0][0] : NULL;
+    case 14: return J->rho_F ? &J->rho_F[0][0] : NULL;
+    case 15: return J->rho_P ? &J->rho_P[0][0] : NULL;
+    case 16: return J->gadot_grad ? &J->gadot_grad[0][0][0] : NULL;
     }
     return NULL;
+}
+
+int orc_job_dd_gradient(orc_job *J)
+{
+    if (J->law != ORC_LAW_DD) return -1;
+    dd_gradient(J);
+    return ORC_OK;
+}
+
+int orc_job_dd_tables(const orc_job *J, double *shear_G, double *bbB, double *bp, double *chi,
+                      double *cth)
+{
+    memcpy(shear_G, J->Shear_G, sizeof(J->Shear_G));
+    memcpy(bbB, J->bb_B, sizeof(J->bb_B));
+    memcpy(bp, J->bp, sizeof(J->bp));
+    memcpy(chi, J->Chi, sizeof(J->Chi));
+    memcpy(cth, J->cth, sizeof(J->cth));
+    return J->law;
 }
 
 /* layout of the macro vector, shared with oracle/oracle.py */

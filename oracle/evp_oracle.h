@@ -57,6 +57,30 @@ void orc_harden_voxel(int nsys, const double gamdot[], double dt,
                       const double hard[][ORC_NSYSMX], const double tau[][3],
                       const double thet[][2], double crss[][2], double *gamacum);
 
+/* ---- dislocation-density law: the build the reference ships
+ * (src/evpFlags.h: DD_BASED_FLAG, DD_CU, FORWARD_PLUS_BACKWARD,
+ * DD_GND_FIRST_ORDER, PF_DRX), before any DRX event ---- */
+#define ORC_LAW_POWER 0
+#define ORC_LAW_DD 1
+#define ORC_ERR_NAN 9001 /* assert(isnan(..)==0) in StrainRate_Orowan aborts */
+/* law of the jobs that later orc_job_setup calls create (default power law) */
+void orc_set_law(int law);
+typedef struct {
+    double c1, c2, c3, G, q_slip, kT; /* q_slip = Q_slip MY_J_IN_EV / kT */
+    double drx;                        /* DeltaTau_DRX exp((t_last - TimeTot)/dt/tau_DRX) */
+    const double *bbB;                 /* Burgers length per system [nm] */
+    const double (*chi)[ORC_NSYSMX];
+    const double (*cth)[ORC_NSYSMX][ORC_NSYSMX]; /* cos s,g1,g2,g3, sin s,g1,g2,g3 */
+} orc_dd_phase;
+int orc_strain_rate_dd(const double stress[6], double edot[6], double dedot[6][6], int nsys,
+                       const double sc[][5], const orc_dd_phase *ph, const double rho_s[],
+                       const double rho_g1[], const double rho_g2[], const double rho_g3[],
+                       double rho_m[], double rho_F[], double rho_P[], double gamdot[]);
+void orc_dd_interaction(const double inter[6], double chi[12][12]);
+void orc_dd_angles(int nsys, const double bn[][3], const double bb[][3], double cth[8][12][12]);
+double orc_dd_shear_modulus(const double ec[3]);
+double orc_dd_kT(double T_K);
+
 /* ---- per-voxel stress update (src/evolution.c:2-310) ----
  * sig  : in = lambda (stress at start of the Green iteration), out = stress
  * returns status; *n_iter receives the Newton iteration count.           */
@@ -89,8 +113,17 @@ int orc_job_nvox(const orc_job *job);
 int orc_threads(void); /* OpenMP threads the oracle uses */
 void orc_job_dims(const orc_job *job, int dims[3]);
 /* field access: 0 Sig, 1 Eps, 2 Edot, 3 DisGrad, 4 VelGrad, 5 TranMat (all N^3 x 9),
- * 6 crss (N^3 x 12 x 2), 7 gamdot (N^3 x 12), 8 gamacum (N^3)                        */
+ * 6 crss (N^3 x 12 x 2), 7 gamdot (N^3 x 12), 8 gamacum (N^3); dislocation-
+ * density jobs: 9 rho_s, 10 rho_m, 11 rho_g1, 12 rho_g2, 13 rho_g3, 14 rho_F,
+ * 15 rho_P (N^3 x 12 each), 16 gradient of gamdot (N^3 x 12 x 3)                  */
 double *orc_job_field(orc_job *job, int which);
+/* Gradient_ShearRate of the job's current gamdot field into field 16 */
+int orc_job_dd_gradient(orc_job *job);
+/* the dislocation-density job's tables: shear modulus [2], Burgers lengths
+ * [2][12], sense vectors b x n [2][12][3], interaction matrix [2][12][12],
+ * projection tables [2][8][12][12]; returns the law                        */
+int orc_job_dd_tables(const orc_job *job, double *shear_G, double *bbB, double *bp, double *chi,
+                      double *cth);
 int *orc_job_grain(orc_job *job);
 int *orc_job_phase(orc_job *job);
 /* Newton iterations of each voxel in the last Green iteration */

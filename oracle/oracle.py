@@ -20,6 +20,9 @@ IP = C.POINTER(C.c_int)
 
 # fields of orc_job_field()
 SIG, EPS, EDOT, DISGRAD, VELGRAD, TRANMAT, CRSS, GAMDOT, GAMACUM = range(9)
+# dislocation-density jobs
+RHO_S, RHO_M, RHO_G1, RHO_G2, RHO_G3, RHO_F, RHO_P, GADOT_GRAD = range(9, 17)
+LAW_POWER, LAW_DD = 0, 1
 
 # layout of orc_job_macro()
 MACRO = dict(TimeTot=0, e_vm=1, evmp=2, d_vm=3, dvmp=4, s_vm=5, s_vm1=6, Err_e=7,
@@ -75,6 +78,16 @@ def lib():
             getattr(L, nm).argtypes = [DP, DP]
         L.orc_basis.argtypes = [DP]
         L.orc_ten4_transform_inv.argtypes = [DP, DP, DP]
+        L.orc_set_law.argtypes = [C.c_int]
+        L.orc_job_dd_gradient.argtypes = [C.c_void_p]
+        L.orc_job_dd_tables.argtypes = [C.c_void_p, DP, DP, DP, DP, DP]
+        L.orc_dd_interaction.argtypes = [DP, DP]
+        L.orc_dd_angles.argtypes = [C.c_int, DP, DP, DP]
+        L.orc_dd_shear_modulus.restype = C.c_double
+        L.orc_dd_shear_modulus.argtypes = [DP]
+        L.orc_dd_kT.restype = C.c_double
+        L.orc_dd_kT.argtypes = [C.c_double]
+        L.orc_strain_rate_dd.argtypes = [DP, DP, DP, C.c_int, DP, C.c_void_p] + [DP] * 8
         _lib = L
     return _lib
 
@@ -96,10 +109,15 @@ class OracleError(RuntimeError):
 class Job:
     """A whole EVPFFT job on the CPU: setup from input.in, then increments."""
 
-    def __init__(self, input_path: str):
+    def __init__(self, input_path: str, law: int = LAW_POWER):
         L = lib()
         st = C.c_int(0)
-        self.h = L.orc_job_setup(os.fsencode(input_path), C.byref(st))
+        L.orc_set_law(law)
+        try:
+            self.h = L.orc_job_setup(os.fsencode(input_path), C.byref(st))
+        finally:
+            L.orc_set_law(LAW_POWER)
+        self.law = law
         if not self.h:
             raise OracleError(st.value)
         d = (C.c_int * 3)()
@@ -122,8 +140,12 @@ class Job:
 
     def field(self, which: int) -> np.ndarray:
         p = lib().orc_job_field(self.h, which)
-        per = {CRSS: 24, GAMDOT: 12, GAMACUM: 1}.get(which, 9)
+        if not p:
+            raise ValueError(f"field {which} is not held by this job")
+        per = {CRSS: 24, GAMDOT: 12, GAMACUM: 1, GADOT_GRAD: 36}.get(which, 12 if which >= RHO_S else 9)
         a = np.ctypeslib.as_array(p, shape=(self.n * per,)).copy()
+        if per == 36:
+            return a.reshape(self.n, 12, 3)
         if per == 9:
             return a.reshape(self.n, 3, 3)
         if per == 24:
@@ -132,6 +154,26 @@ class Job:
 
     def grain(self):
         return np.ctypeslib.as_array(lib().orc_job_grain(self.h), shape=(self.n,)).copy()
+
+    def set_field(self, which: int, values: np.ndarray):
+        """Overwrite a field of the job (tests of single steps)."""
+        p = lib().orc_job_field(self.h, which)
+        a = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        C.memmove(p, a.ctypes.data, a.nbytes)
+
+    def dd_gradient(self):
+        """Gradient_ShearRate of the current gamdot field (field GADOT_GRAD)."""
+        if lib().orc_job_dd_gradient(self.h):
+            raise ValueError("not a dislocation-density job")
+        return self.field(GADOT_GRAD)
+
+    def dd_tables(self) -> dict:
+        """Shear moduli, Burgers lengths, sense vectors, interaction matrices and
+        projection tables of a dislocation-density job (per phase)."""
+        G, b, bp = f64(2), f64((2, 12)), f64((2, 12, 3))
+        chi, cth = f64((2, 12, 12)), f64((2, 8, 12, 12))
+        lib().orc_job_dd_tables(self.h, ptr(G), ptr(b), ptr(bp), ptr(chi), ptr(cth))
+        return dict(shear_G=G, bbB=b, bp=bp, chi=chi, cth=cth)
 
     def newton_counts(self):
         return np.ctypeslib.as_array(lib().orc_job_newton_counts(self.h), shape=(self.n,)).copy()
@@ -288,3 +330,48 @@ def set_newton_tol(tol: float = 1e-10, tolx: float = 1e-7):
 
 def threads() -> int:
     return lib().orc_threads()
+
+
+class DDPhase(C.Structure):
+    """orc_dd_phase: the rate-law constants of one crystal phase."""
+    _fields_ = [("c1", C.c_double), ("c2", C.c_double), ("c3", C.c_double), ("G", C.c_double),
+                ("q_slip", C.c_double), ("kT", C.c_double), ("drx", C.c_double),
+                ("bbB", DP), ("chi", DP), ("cth", DP)]
+
+
+def dd_interaction(inter6):
+    chi = f64((12, 12))
+    lib().orc_dd_interaction(ptr(np.ascontiguousarray(inter6, dtype=np.float64)), ptr(chi))
+    return chi
+
+
+def dd_angles(bn, bb):
+    cth = f64((8, 12, 12))
+    lib().orc_dd_angles(len(bn), ptr(np.ascontiguousarray(bn, dtype=np.float64)),
+                        ptr(np.ascontiguousarray(bb, dtype=np.float64)), ptr(cth))
+    return cth
+
+
+def dd_shear_modulus(ec3):
+    return lib().orc_dd_shear_modulus(ptr(np.ascontiguousarray(ec3, dtype=np.float64)))
+
+
+def dd_kT(T_K: float) -> float:
+    return lib().orc_dd_kT(T_K)
+
+
+def strain_rate_dd(stress6, sc, c1, c2, c3, G, q_slip, kT, drx, bbB, chi, cth, rho_s, rho_g1,
+                   rho_g2, rho_g3, rho_m):
+    """StrainRate_Orowan for one voxel.  Returns (status, edot, dedot, gamdot,
+    rho_m after the call, rho_F, rho_P)."""
+    nsys = sc.shape[0]
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (bbB, chi, cth)]
+    ph = DDPhase(c1, c2, c3, G, q_slip, kT, drx, ptr(keep[0]), ptr(keep[1]), ptr(keep[2]))
+    rm = np.array(rho_m, dtype=np.float64, order="C")
+    edot, dedot, gd, rF, rP = f64(6), f64((6, 6)), f64(12), f64(12), f64(12)
+    st = lib().orc_strain_rate_dd(
+        ptr(np.ascontiguousarray(stress6, dtype=np.float64)), ptr(edot), ptr(dedot), nsys,
+        ptr(np.ascontiguousarray(sc, dtype=np.float64)), C.cast(C.pointer(ph), C.c_void_p),
+        *[ptr(np.ascontiguousarray(a, dtype=np.float64)) for a in (rho_s, rho_g1, rho_g2, rho_g3)],
+        ptr(rm), ptr(rF), ptr(rP), ptr(gd))
+    return st, edot, dedot, gd[:nsys], rm, rF[:nsys], rP[:nsys]
