@@ -43,6 +43,16 @@ void evp_set_abort_on_error(int on);
 int evp_status(void);
 /* precision of the Green loop for SetupJob: EVPGPU_FP64 (default) or EVPGPU_MIXED */
 void evp_set_precision(int precision);
+/* Constitutive law of the jobs SetupJob and evp_job_open set up next:
+ * EVP_LAW_POWER (default), the power law with Voce hardening of the
+ * reference's input files (src/constitutive.c:5-67, 275-326), or EVP_LAW_DD,
+ * the dislocation-density law the reference's build compiles in
+ * (src/evpFlags.h:7-24; StrainRate_Orowan, DislocationEvolution, copper
+ * elastic constants, src/init.c:821-826).  No counterpart call in the
+ * reference, which fixes the law at compile time.                           */
+#define EVP_LAW_POWER 0
+#define EVP_LAW_DD 1
+void evp_set_law(int law);
 
 /* ---- handle API ---- */
 typedef struct evp_job evp_job;
@@ -68,6 +78,8 @@ int evp_job_grain_list(const evp_job *job, int *ids);
 int evp_job_setup_info(const evp_job *job, double *c066, double *s066, double *c0, double *s0);
 /* restore the state right after setup (fields, CRSS, orientations, macros) */
 int evp_job_reset(evp_job *job);
+/* EVP_LAW_POWER or EVP_LAW_DD */
+int evp_job_law(const evp_job *job);
 /* vm.out row of the last increment (TIME EVM EVMP DVM DVMP SVM SVM_1) */
 int evp_job_vm_row(const evp_job *job, double row[7]);
 /* Calibration parameters of crystalline phase `phase` (0 or 1), in the order

@@ -94,6 +94,30 @@ typedef struct {
     int stopped;                   /* 1 if the creep stop (e_vm > 0.3) ended the run */
 } evpgpu_stats;
 
+/* Dislocation-density law of one crystal phase: the constitutive law of the
+ * build the reference ships (src/evpFlags.h: DD_BASED_FLAG, DD_CU,
+ * FORWARD_PLUS_BACKWARD, DD_GND_FIRST_ORDER), with the tables SetupJob
+ * forms (src/init.c:179-203, 593-702, 1040-1125).  Needs 12 slip systems. */
+typedef struct {
+    double c[8];                    /* C_1 .. C_8 */
+    double q_slip, q_bulk;          /* activation energies, 1E-19 J (namelist units) */
+    double shear_G;                 /* MPa (Hill average of phase I, src/init.c:826) */
+    double poisson_nu;              /* ElastConst_PhaseI[3] */
+    double bbB[EVPGPU_NSYSMX];      /* Burgers length, nm */
+    double bp[EVPGPU_NSYSMX][3];    /* sense vectors b x n, crystal axes */
+    double chi[EVPGPU_NSYSMX][EVPGPU_NSYSMX];   /* interaction strengths */
+    double cth[8][EVPGPU_NSYSMX][EVPGPU_NSYSMX]; /* cos s,g1,g2,g3 then sin s,g1,g2,g3 */
+} evpgpu_dd_phase;
+
+/* Scalars of the dislocation-density law (src/io.c:47-93). */
+typedef struct {
+    double T_K;                     /* temperature, K */
+    double L0;                      /* voxel length of the GND rates, mm */
+    double D_0;                     /* climb diffusion coefficient */
+    double rho_ssd_initial;         /* initial SSD density, 1E12/m^2 */
+    double tau_drx, delta_tau_drx;  /* DRX ageing time and extra passing stress */
+} evpgpu_dd_law;
+
 const char *evpgpu_strerror(int status);
 
 /* Create a context for an nx*ny*nz grid (each a power of 2 in [8, 256]). */
@@ -140,6 +164,22 @@ int evpgpu_download_state(evpgpu_ctx *ctx, double *sig6, double *disgrad9, doubl
 
 int evpgpu_set_macro(evpgpu_ctx *ctx, const evpgpu_macro *m);
 int evpgpu_get_macro(evpgpu_ctx *ctx, evpgpu_macro *m);
+
+/* Switch the context to the dislocation-density law (after
+ * evpgpu_set_phases; every crystal phase needs 12 systems) and set the
+ * densities to their initial values (src/init.c:1073-1093): rho_s =
+ * rho_ssd_initial, the mobile and GND densities 0.  The Newton update then
+ * uses StrainRate_Orowan (src/constitutive.c:507-721) in place of
+ * StrainRate_eval, and each increment ends with the shear-rate gradient
+ * (Hard_Flag, istep > 1; src/constitutive.c:875-1108) and
+ * DislocationEvolution (src/constitutive.c:1346-1520) in place of harden. */
+int evpgpu_set_dd_law(evpgpu_ctx *ctx, const evpgpu_dd_law *law, const evpgpu_dd_phase *phases,
+                      int nphases);
+/* Densities of a dislocation-density context, each N x 12 (voxel-major),
+ * 1E12/m^2: SSD, mobile, GND types 1-3, forest, parallel.  Any pointer may
+ * be NULL. */
+int evpgpu_download_dd_state(evpgpu_ctx *ctx, double *rho_s, double *rho_m, double *rho_g1,
+                             double *rho_g2, double *rho_g3, double *rho_f, double *rho_p);
 
 /* One loading increment: src/evolution.c:1318-1547 plus hardening.
  * forced_iters > 0 runs exactly that many Green iterations (parity protocol). */

@@ -853,6 +853,7 @@ int evpgpu_adjoint_record(evpgpu_ctx *c, int n, int ck, double *sig_avg, evpgpu_
     for (int p = 0; p < c->nphases; p++)
         if (c->ph[p].is_gas) return EVPGPU_ERR_ARG;
     if (c->ld.elast_bc != 1 || c->creep_flag == 1) return EVPGPU_ERR_ARG;
+    if (c->dd_on) return EVPGPU_ERR_STATE; // the reverse pass is of the power law
     adjoint_free(c);
     evpgpu_adj *A = (evpgpu_adj *)calloc(1, sizeof(evpgpu_adj));
     c->adj = A;

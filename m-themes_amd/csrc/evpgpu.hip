@@ -221,6 +221,8 @@ NewtonParams newton_params(const evpgpu_ctx *c)
     P.tolx = c->tolx;
     const double kt = c->lam_min > 0.0 ? 0.99 * c->tolx * c->lam_min : 0.0;
     P.kmin_tolx2 = kt * kt;
+    P.dd = c->dd_on;
+    P.dd_first = 0;
     return P;
 }
 
@@ -313,11 +315,13 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
     tmark(c, 3, 1);
     }
     NewtonParams P = newton_params(c);
+    P.dd_first = c->dd_first;
+    c->dd_first = 0;
     int nblk = 0;
     tmark(c, 0, 0);
-    st = newton_launch(c, P, c->sig, c->sig_alt, c->E, c->epsp, c->edot, c->tran, c->crss, c->gamdot,
-                       c->phase, c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr, nullptr,
-                       c->newton_shortcut);
+    st = newton_launch(c, P, c->sig, c->sig_alt, c->E, c->epsp, c->edot, c->tran, c->dd_on ? c->dd_k : c->crss,
+                       c->gamdot, c->phase, c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr,
+                       nullptr, c->newton_shortcut);
     if (st) return st;
     {
         double *t = c->sig;
@@ -452,7 +456,7 @@ int evpgpu_destroy(evpgpu_ctx *c)
     void *ptrs[] = {c->sig, c->sig_alt, c->E, c->W, c->epsp, c->edot, c->tran, c->crss, c->gamacum, c->gamdot,
                     c->phase, c->grain, c->orient, c->sgr, c->spec, c->acc, c->rwork,
                     c->sigA, c->cA, c->skA, c->Estart, c->partials, c->red, c->red2, c->d_status, c->tw64[0], c->tw64[1], c->tw64[2],
-                    c->tw32[0], c->tw32[1], c->tw32[2]};
+                    c->tw32[0], c->tw32[1], c->tw32[2], c->rho, c->dd_k, c->dd_tab};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
         if (ptrs[i]) (void)hipFree(ptrs[i]);
     for (int k = 0; k < EVP_NTIMERS; k++)
@@ -722,6 +726,13 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     tmark(c, 4, 0);
     int st = prologue_launch(c, pro, c->creep_flag == 1, nullptr);
     if (st) return st;
+    if (c->dd_on) {
+        // the rate-law constants of this increment's densities; the first
+        // Newton launch reads the first call's passing stress
+        st = dd_prologue_launch(c, dd_params(c));
+        if (st) return st;
+        c->dd_first = 1;
+    }
     // update_schmid (src/evolution.c:1334-1336) happens inside the Newton
     // kernel, from the orientations the previous epilogue left
     st = fill_zero(c, (double *)c->acc, 6 * c->NK);
@@ -808,17 +819,24 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     // edot and norms; gamdot as the reference leaves it).
     {
         NewtonParams P = newton_params(c);
+        P.dd_first = c->dd_on && iter == 1;
         int nblk = 0;
         tmark(c, 0, 0);
-        st = newton_launch(c, P, c->sig_alt, c->sig, c->E, c->epsp, c->edot, c->tran, c->crss, c->gamdot,
-                           c->phase, c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr,
-                           nullptr, 0);
+        st = newton_launch(c, P, c->sig_alt, c->sig, c->E, c->epsp, c->edot, c->tran,
+                           c->dd_on ? c->dd_k : c->crss, c->gamdot, c->phase, c->orient, c->sgr, c->N,
+                           c->partials, &nblk, c->d_status, nullptr, nullptr, 0);
         if (st) return st;
         tmark(c, 0, 1);
         EVP_CHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         EVP_CHECK(hipStreamSynchronize(c->stream));
         tcollect(c);
         if (*c->h_status) return *c->h_status;
+    }
+    if (c->dd_on) {
+        // gradient of the shear rates and DislocationEvolution, with the
+        // orientations of the loop (before update_orient)
+        st = dd_evolve_launch(c, dd_params(c));
+        if (st) return st;
     }
     S.seconds = now_s() - t0;
     S.iters = iter;
@@ -857,7 +875,8 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     E.dt = dt;
     memcpy(E.udot_a, c->udot_a, sizeof(E.udot_a));
     E.update_orient = (c->ld.update_flag == 1 && m.istep > 1);
-    E.harden = (c->ld.hard_flag == 1 && m.istep > 1);
+    // the dislocation-density build does not call harden (src/evolution.c:1571-1576)
+    E.harden = (c->ld.hard_flag == 1 && m.istep > 1) && !c->dd_on;
     E.wgt = 1.0 / (double)c->N;
     for (int p = 0; p < 2; p++) {
         E.is_gas[p] = p < c->nphases ? c->ph[p].is_gas : 1;
@@ -1019,6 +1038,7 @@ int evpgpu_newton_host(evpgpu_ctx *c, int n, const int *phase, const int *orient
     if (!st) {
         NewtonParams P = newton_params(c);
         P.crss_sym = 0;
+        P.dd = 0; // the power-law update on the caller's CRSS
         int nb = 0;
         st = newton_launch(c, P, dsig, dsig, dstr, deps, ded, dsc, dcr, dgd, dph, dor, dsg, n, dpart, &nb, dst,
                            der, dni, 0);

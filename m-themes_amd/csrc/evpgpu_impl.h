@@ -43,6 +43,12 @@ struct NewtonParams {
     double kmin_tolx2; // (0.99 TOLX min over orientations of lambda_min(S0 + S))^2
     double nr_tol;   // Newton tolerance on the relative step (reference 1e-10)
     double tolx;     // line-search TOLX (reference 1e-7)
+    int dd;          // dislocation-density law: the crss argument holds the
+                     // per-system passing stress, 1/(tau_0 + tau_cut), gam_o
+                     // and the first call's passing stress, each 12N
+    int dd_first;    // this launch makes each voxel's first rate call of the
+                     // increment (the passing stress then uses the previous
+                     // call's mobile density)
 };
 
 struct GreenParams {
@@ -158,7 +164,29 @@ struct evpgpu_ctx {
 
     // reverse pass (adjoint.hip), allocated by evpgpu_adjoint_record
     struct evpgpu_adj *adj;
+
+    // dislocation-density law (disl.hip), evpgpu_set_dd_law
+    int dd_on;
+    int dd_first;            // the next Newton launch is the increment's first
+    evpgpu_dd_law dd_law;
+    evpgpu_dd_phase dd_ph[2];
+    double *dd_tab;          // device: per phase chi[144] then cth[8*144]
+    double *rho;             // 7 x 12N: SSD, mobile, GND 1-3, forest, parallel
+    double *dd_k;            // 4 x 12N: passing stress, 1/(tau_0+tau_cut), gam_o,
+                             // first call's passing stress
 };
+
+// disl.hip: the dislocation-density law around the Green loop
+struct DDParams {
+    double c[2][8], shear_G[2], nu[2], q_slip[2], q_bulk[2], bbB[2][EVPGPU_NSYSMX];
+    double bp[2][EVPGPU_NSYSMX][3], bn[2][EVPGPU_NSYSMX][3], bb[2][EVPGPU_NSYSMX][3];
+    double kT, T_K, L0, D_0, drx, dt;
+    int nsys[2], is_gas[2];
+    int istep, use_grad;
+};
+DDParams dd_params(const evpgpu_ctx *c);
+int dd_prologue_launch(evpgpu_ctx *c, const DDParams &D);
+int dd_evolve_launch(evpgpu_ctx *c, const DDParams &D);
 
 // evpgpu.hip (defined inside the ABI's extern "C" block)
 extern "C" int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_stats *stats);

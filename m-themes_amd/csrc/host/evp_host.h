@@ -29,6 +29,20 @@ typedef struct {
 } evp_namelist;
 
 int evp_parse_namelist(const char *path, evp_namelist *nl);
+
+/* namelist keys of the dislocation-density build (src/io.c:47-93) */
+typedef struct {
+    double a0[2], L0, GND_ScaleRatio, T__K, Q_slip[2], Q_bulk[2];
+    double C[8][2];      /* C_1 .. C_8 */
+    double inter[6][2];  /* Selfinter, Coplanar, CrossSlip, GlissileJunction, HirthLock,
+                            LomerCottrellLock */
+    double rho_SSD_initial, D_0, tau_DRX, DeltaTau_DRX;
+} evp_dd_keys;
+int evp_parse_dd_keys(const char *path, evp_dd_keys *dd);
+/* the law's tables for phase ph (12 systems; bn, bb normalised) */
+void evp_dd_phase_setup(const evp_dd_keys *dd, int ph, const evpgpu_phase *phase, double shear_G,
+                        double poisson_nu, evpgpu_dd_phase *out);
+double evp_dd_shear_modulus(const double ec[3]);
 void evp_print_namelist(const evp_namelist *nl);
 FILE *evp_open_rel(const char *dir, const char *name);
 /* PlasticInit: fills the phase description (src/init.c:52-231) */

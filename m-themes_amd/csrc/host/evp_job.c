@@ -21,8 +21,16 @@
 
 #define PI_REF 3.14159265359 /* src/evp.h:77 (Q8) */
 
+/* law of the jobs opened next (evp_set_law) */
+static int G_law = EVP_LAW_POWER;
+void evp_set_law(int law) { G_law = law == EVP_LAW_DD ? EVP_LAW_DD : EVP_LAW_POWER; }
+
 struct evp_job {
     evp_namelist nl;
+    int law;
+    evp_dd_keys dd;
+    evpgpu_dd_law dd_law;
+    evpgpu_dd_phase dd_ph[EVP_NPHMAX];
     char dir[512];
     int precision;
     evpgpu_ctx *ctx;
@@ -241,6 +249,9 @@ static int upload_initial(evp_job *J)
     int st = evpgpu_upload_state(J->ctx, sig, dg, eps, edot, tran, crss, gac);
     free(sig);
     if (st) return st;
+    /* densities of the dislocation-density law (src/init.c:1073-1093) */
+    if (J->law == EVP_LAW_DD && (st = evpgpu_set_dd_law(J->ctx, &J->dd_law, J->dd_ph, J->nl.N_phases)))
+        return st;
     return evpgpu_set_macro(J->ctx, &J->m0);
 }
 
@@ -267,7 +278,25 @@ evp_job *evp_job_open(const char *input_path, int precision, int *status)
         st = EVPGPU_ERR_SIZE;
         goto fail;
     }
-    /* phases (src/init.c:815-862) */
+    /* phases (src/init.c:815-862).  The dislocation-density build (DD_CU)
+     * replaces phase I's elastic constants by copper's and zeroes phase II's
+     * (src/init.c:821-824, 840-843).                                       */
+    J->law = G_law;
+    double shear_G = 0.0;
+    if (J->law == EVP_LAW_DD) {
+        if (evp_parse_dd_keys(input_path, &J->dd) < 0) {
+            st = EVP_ERR_FILE;
+            goto fail;
+        }
+        if (!nl->Type_phases[0]) {
+            nl->ElastConst_PhaseI[0] = 241.1674 * 1000;
+            nl->ElastConst_PhaseI[1] = 89.1989 * 1000;
+            nl->ElastConst_PhaseI[2] = 151.9685 * 1000;
+        }
+        if (nl->N_phases == 2 && !nl->Type_phases[1])
+            nl->ElastConst_PhaseII[0] = nl->ElastConst_PhaseII[1] = nl->ElastConst_PhaseII[2] = 0.0;
+        shear_G = evp_dd_shear_modulus(nl->ElastConst_PhaseI);
+    }
     double Cijkl[2][3][3][3][3];
     memset(Cijkl, 0, sizeof(Cijkl));
     for (int p = 0; p < nl->N_phases; p++) {
@@ -303,6 +332,17 @@ evp_job *evp_job_open(const char *input_path, int precision, int *status)
     /* device */
     if ((st = evpgpu_create(&J->ctx, nl->CellDim[0], nl->CellDim[1], nl->CellDim[2], precision))) goto fail;
     if ((st = evpgpu_set_phases(J->ctx, J->ph, nl->N_phases))) goto fail;
+    if (J->law == EVP_LAW_DD) {
+        for (int p = 0; p < nl->N_phases; p++)
+            if (!J->ph[p].is_gas)
+                evp_dd_phase_setup(&J->dd, p, &J->ph[p], shear_G, nl->ElastConst_PhaseI[3], &J->dd_ph[p]);
+        J->dd_law.T_K = J->dd.T__K;
+        J->dd_law.L0 = J->dd.L0;
+        J->dd_law.D_0 = J->dd.D_0;
+        J->dd_law.rho_ssd_initial = J->dd.rho_SSD_initial;
+        J->dd_law.tau_drx = J->dd.tau_DRX;
+        J->dd_law.delta_tau_drx = J->dd.DeltaTau_DRX;
+    }
     if ((st = evpgpu_set_loading(J->ctx, ld))) goto fail;
     if ((st = evpgpu_upload_microstructure(J->ctx, J->grain, J->phase, J->orient, J->n_orient, J->sgr))) goto fail;
     /* macro state after InitFields */
@@ -348,6 +388,8 @@ int evp_job_reset(evp_job *J)
     if (!J) return EVPGPU_ERR_ARG;
     return upload_initial(J);
 }
+
+int evp_job_law(const evp_job *J) { return J ? J->law : -1; }
 
 int evp_job_set_params(evp_job *J, int phase, const double q[5])
 {

@@ -75,8 +75,11 @@ template <int M> __device__ __forceinline__ double ipowc(double x)
 // repeated squaring for an integral one, RM_FAST the common case of every
 // crystal phase having EVPGPU_NSYSMX systems with one integral n <= 32 and
 // one reference rate (constants hoisted out of the system loop, straight-
-// line squaring; for n = 10 and n = 20 the exponent is a template constant)
-enum { RM_POW = 0, RM_LOOP = 1, RM_FAST = 2 };
+// line squaring; for n = 10 and n = 20 the exponent is a template constant).
+// RM_DISL: the dislocation-density law StrainRate_Orowan
+// (src/constitutive.c:507-721), with its per-voxel constants formed once
+// per increment.
+enum { RM_POW = 0, RM_LOOP = 1, RM_FAST = 2, RM_DISL = 3 };
 
 // chg_basis opt 2 for a symmetric tensor given as (11 22 33 23 13 12)
 __device__ __forceinline__ void sym_to_v6(const double t[6], double v[6])
@@ -107,7 +110,11 @@ constexpr int NB_THREADS = 64; // one wave per block: the LDS slab is wave-priva
 struct RateIn {
     const double *sc;                              // LDS slab, sc[(s*5+j)*64]
     const double *k0;                              // LDS: C0 then S0, packed triangles
-    double itf[EVPGPU_NSYSMX], itb[EVPGPU_NSYSMX]; // 1/CRSS forward, backward
+    double itf[EVPGPU_NSYSMX], itb[EVPGPU_NSYSMX]; // 1/CRSS forward, backward;
+                                                   // RM_DISL: itf the passing stress
+    const double *tpn;                             // RM_DISL: this voxel's constants, system
+    long long tps;                                 // stride tps: passing stress, 1/(tau_0 +
+                                                   // tau_cut), gam_o, first passing stress
 };
 #define SC(i, j) in.sc[((i)*5 + (j)) * NB_THREADS]
 
@@ -144,6 +151,21 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
             for (int j = 0; j < 5; j++) scn[j] = SC(k + 1, j);
         if (k < ns) {
             double r = sc[0] * s[0] + sc[1] * s[1] + sc[2] * s[2] + sc[3] * s[3] + sc[4] * s[4];
+            double r1 = 0.0, r2 = 0.0; // the rate and its derivative in the resolved shear
+            if (RM == RM_DISL) {
+                // gam = gam_o sinh((|rss| - tau_pass)/(tau_0 + tau_cut)) sign(rss)
+                // above the passing stress, 0 below (FORWARD_PLUS_BACKWARD)
+                // (1/(tau_0 + tau_cut) and gam_o are read where used: held in
+                // registers they would spill)
+                const double a = fabs(r);
+                if (a > in.itf[k]) {
+                    const double itc = in.tpn[(EVPGPU_NSYSMX + k) * in.tps];
+                    const double g0k = in.tpn[(2 * EVPGPU_NSYSMX + k) * in.tps];
+                    const double x = (a - in.itf[k]) * itc;
+                    r2 = g0k * sinh(x) * (double)(2 * (r > 0) - 1);
+                    if (DD) r1 = g0k * cosh(x) * itc;
+                }
+            } else {
             double itau = (r < 0) ? in.itb[k] : in.itf[k];
             r = r * itau;
             double a = fabs(r), p1, p2;
@@ -157,12 +179,13 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
                 p1 = pow(a, P.nrs[p][k] - 1.);
                 p2 = pow(a, P.nrs[p][k]);
             }
-            double r2 = (RM == RM_FAST ? g0 : P.gam0[p][k]) * p2 * (double)(2 * (r > 0) - 1);
+            r2 = (RM == RM_FAST ? g0 : P.gam0[p][k]) * p2 * (double)(2 * (r > 0) - 1);
+            if (DD) r1 = (RM == RM_FAST ? g0n : P.gam0[p][k] * P.nrs[p][k]) * p1 * itau;
+            }
             if (GD) gd[k] = r2;
 #pragma unroll
             for (int i = 0; i < 5; i++) ed[i] += sc[i] * r2;
             if (DD) {
-                double r1 = (RM == RM_FAST ? g0n : P.gam0[p][k] * P.nrs[p][k]) * p1 * itau;
                 r1 *= P.dt;
 #pragma unroll
                 for (int i = 0; i < 5; i++) {
@@ -283,7 +306,7 @@ __device__ __forceinline__ double nr_quality(const NewtonParams &P, int p, const
 // point) differs; the loop recomputes the last Newton launch of an increment
 // with FAST off before gamdot is used, so FAST launches do not form gamdot.
 template <int RM, bool FAST, int NE>
-__device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, const Vox &V,
+__device__ int update_voxel(const NewtonParams &P, int p, RateIn &in, const Vox &V,
                             double sig6[6], double edotp[6], double gd[EVPGPU_NSYSMX],
                             double &conv_s, double &conv_e, int &iterl, int &trials)
 {
@@ -308,6 +331,13 @@ __device__ int update_voxel(const NewtonParams &P, int p, const RateIn &in, cons
 #pragma unroll
             for (int j = i; j < 6; j++) k[PK(i, j)] = in.k0[21 + PK(i, j)] + V.S[PK(i, j)];
         srate<true, RM, GD, NE>(P, p, in, sig6, ed, k, gd);
+        if (RM == RM_DISL && P.dd_first && iterl == 1) {
+            // the first call used the previous call's mobile density; every
+            // later one sees the density that call formed
+#pragma unroll
+            for (int s = 0; s < EVPGPU_NSYSMX; s++)
+                if (s < P.nsys[p]) in.itf[s] = in.tpn[s * in.tps];
+        }
 #pragma unroll
         for (int i = 0; i < 6; i++) edotp[i] = i < 5 ? ed[i] : 0.0;
         double gradf[6], step[6];
@@ -541,6 +571,16 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
                 col[3 * NB_THREADS] = P02 * RSQ2 + P02 * RSQ2;
                 col[4 * NB_THREADS] = P01 * RSQ2 + P01 * RSQ2;
             }
+            if (RM == RM_DISL) {
+                // crss holds the increment's constants: passing stress,
+                // 1/(tau_0 + tau_cut), gam_o, the first call's passing stress
+                const long long S = (long long)EVPGPU_NSYSMX * N;
+                in.tpn = crss + v;
+                in.tps = N;
+#pragma unroll
+                for (int s = 0; s < EVPGPU_NSYSMX; s++)
+                    in.itf[s] = s < ns ? crss[(P.dd_first ? 3 * S : 0) + s * N + v] : 0.0;
+            } else {
 #pragma unroll
             for (int s = 0; s < EVPGPU_NSYSMX; s++) {
                 if (s < ns) {
@@ -549,6 +589,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
                 } else {
                     in.itf[s] = in.itb[s] = 1.0;
                 }
+            }
             }
             double s6[6], e6[6], gd[EVPGPU_NSYSMX], cs, ce;
             int it, ntr;
@@ -694,7 +735,10 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
         else if (rm == RM_LOOP) LAUNCH(RM_LOOP, B, F);                                   \
         else LAUNCH(RM_POW, B, F);                                                       \
     } while (0)
-    if (fast) {
+    if (P.dd) {
+        if (fast) LAUNCH(RM_DISL, false, true);
+        else LAUNCH(RM_DISL, false, false);
+    } else if (fast) {
         if (P.crss_sym) LAUNCH_RM(true, true);
         else LAUNCH_RM(false, true);
     } else {

@@ -17,6 +17,7 @@ LIBDIR = os.environ.get("EVP_LIB_DIR") or os.path.join(PKG, "lib")
 REPO = os.path.dirname(PKG)
 
 FP64, MIXED = 0, 1
+LAW_POWER, LAW_DD = 0, 1  # evp_set_law
 NSYSMX = 12
 
 DP = C.POINTER(C.c_double)
@@ -30,6 +31,15 @@ class Phase(C.Structure):
                 ("hard", (C.c_double * NSYSMX) * NSYSMX),
                 ("schm_xt", (C.c_double * 5) * NSYSMX),
                 ("bn", (C.c_double * 3) * NSYSMX), ("bb", (C.c_double * 3) * NSYSMX)]
+
+
+class DDPhase(C.Structure):
+    """evpgpu_dd_phase: the dislocation-density law of one crystal phase."""
+    _fields_ = [("c", C.c_double * 8), ("q_slip", C.c_double), ("q_bulk", C.c_double),
+                ("shear_G", C.c_double), ("poisson_nu", C.c_double),
+                ("bbB", C.c_double * NSYSMX), ("bp", (C.c_double * 3) * NSYSMX),
+                ("chi", (C.c_double * NSYSMX) * NSYSMX),
+                ("cth", ((C.c_double * NSYSMX) * NSYSMX) * 8)]
 
 
 class Loading(C.Structure):
@@ -81,6 +91,7 @@ EVPGPU_SYMBOLS = [
     "evpgpu_upload_state", "evpgpu_download_state", "evpgpu_set_macro", "evpgpu_get_macro",
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
     "evpgpu_newton_host", "evpgpu_adjoint_record", "evpgpu_adjoint_run", "evpgpu_copy_probe",
+    "evpgpu_set_dd_law", "evpgpu_download_dd_state",
 ]
 EVP_SYMBOLS = [
     "GetNameList", "PrintNameList", "OpenFiles", "CloseFiles", "SetupJob", "Evolution",
@@ -89,7 +100,7 @@ EVP_SYMBOLS = [
     "evp_job_macro", "evp_job_ids", "evp_job_grain_list", "evp_job_setup_info", "evp_job_reset",
     "evp_job_vm_row", "evp_job_set_params", "evp_job_get_params", "evp_write_iout6",
     "evp_write_texture", "evp_texture_to_text", "evp_mat_to_euler", "evp_job_write_field",
-    "evp_job_write_texture",
+    "evp_job_write_texture", "evp_set_law", "evp_job_law",
 ]
 
 _gpu = None
@@ -145,6 +156,7 @@ def gpu():
         L.evpgpu_copy_probe.argtypes = [VP, C.c_longlong, C.c_int, DP]
         L.evpgpu_adjoint_record.argtypes = [VP, C.c_int, C.c_int, DP, C.POINTER(Stats)]
         L.evpgpu_adjoint_run.argtypes = [VP, DP, C.c_double, C.c_int, DP, C.POINTER(AdjointStats)]
+        L.evpgpu_download_dd_state.argtypes = [VP] + [DP] * 7
         _gpu = L
     return _gpu
 
@@ -175,6 +187,8 @@ def evp():
         L.GetNameList.argtypes = [C.POINTER(C.c_char_p)]
         L.evp_set_abort_on_error.argtypes = [C.c_int]
         L.evp_set_precision.argtypes = [C.c_int]
+        L.evp_set_law.argtypes = [C.c_int]
+        L.evp_job_law.argtypes = [VP]
         _evp = L
     return _evp
 
@@ -191,10 +205,14 @@ def _chk(st, what):
 class GpuJob:
     """A drop-in EVPFFT job (input.in + .sx + microstructure) resident on the GPU."""
 
-    def __init__(self, input_path: str, precision: int = FP64):
+    def __init__(self, input_path: str, precision: int = FP64, law: int = LAW_POWER):
         L = evp()
         st = C.c_int(0)
-        self.h = L.evp_job_open(os.fsencode(input_path), precision, C.byref(st))
+        L.evp_set_law(law)
+        try:
+            self.h = L.evp_job_open(os.fsencode(input_path), precision, C.byref(st))
+        finally:
+            L.evp_set_law(LAW_POWER)
         if not self.h:
             raise EvpError(st.value, f"evp_job_open({input_path})")
         d = (C.c_int * 3)()
@@ -331,6 +349,14 @@ class GpuJob:
                     TranMat=a["tran"].reshape(9, n).T.reshape(n, 3, 3),
                     crss=a["crss"].reshape(24, n).T.reshape(n, 12, 2),
                     gamacum=a["gac"].copy(), gamdot=a["gd"].reshape(12, n).T.copy())
+
+
+    def dd_state(self) -> dict:
+        """Densities of a dislocation-density job, each [N][12] (1E12/m^2)."""
+        names = ("rho_s", "rho_m", "rho_g1", "rho_g2", "rho_g3", "rho_F", "rho_P")
+        a = {k: np.zeros((self.n, 12)) for k in names}
+        _chk(gpu().evpgpu_download_dd_state(self.ctx, *[_p(a[k]) for k in names]), "download_dd_state")
+        return a
 
 
 class Context:

@@ -1,0 +1,134 @@
+"""The dislocation-density law on the GPU (StrainRate_Orowan in the Newton
+kernel, the per-increment constants, the shear-rate gradient and
+DislocationEvolution) against the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pyevp import lib, synth
+
+pytestmark = pytest.mark.gpu
+
+DD_FIELDS = (("rho_s", O.RHO_S), ("rho_m", O.RHO_M), ("rho_g1", O.RHO_G1), ("rho_g2", O.RHO_G2),
+             ("rho_g3", O.RHO_G3), ("rho_F", O.RHO_F), ("rho_P", O.RHO_P))
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _job(tmp_path, name, **kw):
+    law = kw.pop("law", synth.DDLaw())
+    cfg = synth.Config(dd=law, **kw)
+    return synth.write_job(str(tmp_path / name), cfg)
+
+
+def _compare(J, g, tol=1e-10, dens_tol=1e-10):
+    s = g.state()
+    assert rel(s["Sig"], J.field(O.SIG)) < tol, rel(s["Sig"], J.field(O.SIG))
+    assert rel(s["Edot"], J.field(O.EDOT)) < tol
+    assert rel(s["Eps"], J.field(O.EPS)) < tol
+    assert rel(s["DisGrad"], J.field(O.DISGRAD)) < tol
+    assert rel(s["TranMat"], J.field(O.TRANMAT)) < tol
+    assert rel(s["gamdot"], J.field(O.GAMDOT)) < tol
+    d = g.dd_state()
+    for k, f in DD_FIELDS:
+        ref = J.field(f)
+        if np.abs(ref).max() == 0.0:
+            assert np.abs(d[k]).max() == 0.0, k
+        else:
+            assert rel(d[k], ref) < dens_tol, (k, rel(d[k], ref))
+    mo, mg = J.macro(), g.macro()
+    assert rel(mg["sig_avg"], mo["SigAvg"]) < tol
+
+
+def test_dd_loop_parity_8(tmp_path):
+    """Three increments, so the third forms the shear-rate gradient (Hard_Flag,
+    istep > 1) and the GND densities move."""
+    inp = _job(tmp_path, "dd8", n=8, grains=8, seed=11, increments=3)
+    J = O.Job(inp, O.LAW_DD)
+    g = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    it_o = J.run(3)
+    it_g, _ = g.run(3)
+    assert list(it_g) == list(it_o), (it_g, it_o)
+    assert np.abs(J.field(O.RHO_G1)).max() > 0
+    _compare(J, g)
+    g.close()
+
+
+def test_dd_loop_parity_16_pores(tmp_path):
+    inp = _job(tmp_path, "dd16", n=16, grains=8, seed=12, increments=3, pore_fraction=0.03,
+               pore_radius=2)
+    J = O.Job(inp, O.LAW_DD)
+    g = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    it_o = J.run(3)
+    it_g, _ = g.run(3)
+    assert list(it_g) == list(it_o), (it_g, it_o)
+    _compare(J, g)
+    g.close()
+
+
+def test_dd_newton_shortcut_is_exact(tmp_path):
+    inp = _job(tmp_path, "dd8s", n=8, grains=8, seed=13, increments=3)
+    a = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    b = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    b.set_newton_shortcut(False)
+    ia, _ = a.run(3)
+    ib, _ = b.run(3)
+    assert list(ia) == list(ib)
+    sa, sb = a.state(), b.state()
+    for k in ("Sig", "Edot", "gamdot", "DisGrad"):
+        assert np.array_equal(sa[k], sb[k]), k
+    da, db = a.dd_state(), b.dd_state()
+    for k in da:
+        assert np.array_equal(da[k], db[k]), k
+    a.close()
+    b.close()
+
+
+def test_dd_mixed_precision(tmp_path):
+    """The mixed loop on the DD law, at the oracle's iteration counts: stress
+    within 1e-6."""
+    inp = _job(tmp_path, "dd16m", n=16, grains=8, seed=14, increments=2)
+    J = O.Job(inp, O.LAW_DD)
+    it = J.run(2)
+    g = lib.GpuJob(inp, lib.MIXED, law=lib.LAW_DD)
+    for k in it:
+        g.run(1, forced=int(k))
+    s = g.state()
+    assert rel(s["Sig"], J.field(O.SIG)) < 1e-6, rel(s["Sig"], J.field(O.SIG))
+    d = g.dd_state()
+    assert rel(d["rho_s"], J.field(O.RHO_S)) < 1e-6
+    g.close()
+
+
+def test_dd_without_tau_drx_freezes_the_first_increment(tmp_path):
+    """tau_DRX absent (0): the DRX term of the passing stress is 0 * NaN at
+    TimeTot = 0 (src/constitutive.c:585), so nothing slips in the first
+    increment, and -0/0 turns into exp(-inf) = 0 afterwards."""
+    law = synth.DDLaw(tau_drx=None)
+    inp = _job(tmp_path, "dd8n", n=8, grains=8, seed=15, increments=2, law=law)
+    J = O.Job(inp, O.LAW_DD)
+    g = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    it_o = J.run(1)
+    it_g, _ = g.run(1)
+    assert list(it_g) == list(it_o)
+    assert (g.state()["gamdot"] == 0).all() and (J.field(O.GAMDOT) == 0).all()
+    it_o = J.run(1)
+    it_g, _ = g.run(1)
+    assert list(it_g) == list(it_o)
+    assert np.abs(J.field(O.GAMDOT)).max() > 0
+    _compare(J, g)
+    g.close()
+
+
+def test_dd_reference_test_run_constants_freeze_slip(tmp_path):
+    """Q_slip = 2e10 (the reference's test_run input): exp(-q_slip) = 0, no slip."""
+    law = synth.DDLaw(q_slip=2.0e10, T_K=30.0)
+    inp = _job(tmp_path, "dd8f", n=8, grains=8, seed=16, increments=2, law=law)
+    g = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    g.run(2)
+    s = g.state()
+    assert (s["gamdot"] == 0).all() and (s["Edot"] == 0).all()
+    g.close()
