@@ -20,6 +20,7 @@ loops run; `value` is the mixed loop's voxel-iterations per second:
 Prints one JSON line (rank 0, N = 1 only).
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -70,8 +71,8 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(inp, precision, steps, warmup):
-    job = lib.GpuJob(inp, precision)
+def run_gpu(inp, precision, steps, warmup, law=lib.LAW_POWER):
+    job = lib.GpuJob(inp, precision, law=law)
     if warmup > 0:
         job.run(warmup)
         job.reset()
@@ -166,6 +167,11 @@ def main():
     ap.add_argument("--adj-n", type=int, default=64, help="grid edge of the adjoint workload")
     ap.add_argument("--adj-increments", type=int, default=20, help="0: skip the adjoint")
     ap.add_argument("--adj-checkpoint", type=int, default=5)
+    ap.add_argument("--dd-steps", type=int, default=2,
+                    help="increments of the same job under the dislocation-density law, "
+                         "reported beside the headline (0: skip)")
+    ap.add_argument("--law", choices=["power", "dd"], default="power",
+                    help="law of the profiling mode's job")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="profiling mode: one increment forced to this many Green iterations "
                          "per precision, then exit (for rocprofv3 counter runs)")
@@ -191,11 +197,19 @@ def main():
     log(f"bench: wrote {cfg.n}^3 job ({cfg.grains} grains, {cfg.pore_fraction:.0%} pores) "
         f"in {time.perf_counter() - t:.1f} s")
     N = cfg.n ** 3
+    # the same job under the dislocation-density law
+    inp_dd = None
+    if args.dd_steps > 0 or args.law == "dd":
+        cfg_dd = dataclasses.replace(cfg, dd=synth.DDLaw())
+        inp_dd = synth.write_job(tempfile.mkdtemp(prefix="evpbench_dd_"), cfg_dd)
     if args.profile_iters > 0:
         for name, prec in (("fp64", lib.FP64), ("mixed", lib.MIXED)):
             if args.precision not in ("both", name):
                 continue
-            job = lib.GpuJob(inp, prec)
+            if args.law == "dd":
+                job = lib.GpuJob(inp_dd, prec, law=lib.LAW_DD)
+            else:
+                job = lib.GpuJob(inp, prec)
             t0 = time.perf_counter()
             job.run(1, forced=args.profile_iters)
             job.synchronize()
@@ -306,6 +320,29 @@ def main():
                                                         szz(res["fp64"]["macro"])) /
                                                     abs(szz(res["fp64"]["macro"]))),
         }
+    if args.dd_steps > 0:
+        log(f"bench: dislocation-density law, {main_key}: timing {args.dd_steps} increments")
+        prec = lib.MIXED if main_key == "mixed" else lib.FP64
+        try:
+            r = run_gpu(inp_dd, prec, args.dd_steps, 0, law=lib.LAW_DD)
+        except lib.EvpError as e:
+            r = None
+            out["dd_law"] = {"error": str(e)}
+            log(f"bench: dd: {e}")
+    if args.dd_steps > 0 and r is not None:
+        vips = N * sum(r["iters"]) / r["wall"]
+        log(f"bench: dd: iterations {r['iters']}, {r['wall']:.2f} s, {vips:.4g} voxel-iters/s")
+        dk = {}
+        for k, nm in enumerate(names):
+            kms, kn = r["timing"][k]
+            dk[nm] = dict(ms_total=round(kms, 3), launches=kn, ms_avg=round(kms / max(kn, 1), 4))
+        out["dd_law"] = {"value": vips, "unit": "voxel-iters/s", "precision": main_key,
+                         "steps": args.dd_steps, "warmup": 0,
+                         "ms_per_step": 1e3 * r["wall"] / args.dd_steps,
+                         "green_iters": r["iters"], "kernels_ms": dk,
+                         "workload": f"the same {cfg.n}^3 job under StrainRate_Orowan + "
+                                     "DislocationEvolution, increments 1.."
+                                     f"{args.dd_steps}"}
     if args.adj_increments > 0:
         log(f"bench: adjoint: {args.adj_n}^3, {args.adj_increments} increments")
         out["adjoint"] = run_adjoint(args.adj_n, args.adj_increments, args.adj_checkpoint)

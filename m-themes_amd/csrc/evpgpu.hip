@@ -832,12 +832,7 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
         tcollect(c);
         if (*c->h_status) return *c->h_status;
     }
-    if (c->dd_on) {
-        // gradient of the shear rates and DislocationEvolution, with the
-        // orientations of the loop (before update_orient)
-        st = dd_evolve_launch(c, dd_params(c));
-        if (st) return st;
-    }
+
     S.seconds = now_s() - t0;
     S.iters = iter;
     S.fp64_iters = it64;
@@ -856,6 +851,12 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
         if (c->creep_flag == 1) m3[0] = m3[1] = m3[2] = 0.0;
     }
     tmark(c, 4, 0);
+    if (c->dd_on) {
+        // gradient of the shear rates and DislocationEvolution, with the
+        // orientations of the loop (before update_orient)
+        st = dd_evolve_launch(c, dd_params(c));
+        if (st) return st;
+    }
     st = fft_inverse_real(c, c->acc, 3, c->rwork, 1.0 / (double)c->N);
     if (st) return st;
     st = antisym_add_launch(c, m3);
