@@ -1,6 +1,7 @@
 /*
  * evp_oracle.c -- CPU restatement of M-THEMES' EVPFFT Green-operator
- * fixed-point loop (power-law + Voce build, FP64).
+ * fixed-point loop (power-law + Voce build, and the dislocation-density build
+ * the reference ships; FP64).
  *
  * TEST INFRASTRUCTURE: the checker for the HIP path.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
@@ -14,6 +15,14 @@
  *   src/evolution.c:1318-1547 the live increment (prologue, Green loop, epilogue)
  *   src/init.c:10-396,444-591,727-900,1135-1205  SetupJob pieces
  *   src/io.c:96-164           namelist parser
+ *   dislocation-density law: src/constitutive.c:507-721 (StrainRate_Orowan),
+ *   875-1108 (Gradient_ShearRate), 1346-1520 (DislocationEvolution);
+ *   src/init.c:179-203, 593-702, 821-846, 1025, 1073-1125 (its set-up)
+ *
+ * Parity pins: tests/test_oracle_pins.py and tests/test_oracle_dd.py.  The
+ * SSD and GND rate expressions of DislocationEvolution (dd_evolution) are
+ * parity unpinned beyond those runs: they are restated term by term and
+ * only checked for the signs and onsets the law implies.
  *
  * Dispositions of the reference's quirks (shared with the HIP path):
  *   Q1  the power-law driver is reconstructed: the live loop of
@@ -33,6 +42,14 @@
  *       never assigns (src/evolution.c:221-246, uninitialised); the restated
  *       algorithm sets f2 = f_new at each backtrack, as in the line search the
  *       code follows (Numerical Recipes lnsrch).
+ *   Q19 StrainRate_Orowan reads rho_m for the passing stress before the same
+ *       call recomputes it: the first call after the densities change uses
+ *       the previous call's mobile density.  Reproduced.
+ *   Q20 Gradient_ExchangeGrainID compares the z-minus side with the -y
+ *       neighbour's grain (src/constitutive.c:818, 838, 858).  Reproduced.
+ *   Q21 DislocationEvolution: the istep > 20000000 climb branch, the
+ *       rho_dipole / rho_forest diagnostics and the second-order GND terms
+ *       (compiled out) are not restated.
  */
 #include "evp_oracle.h"
 
@@ -419,7 +436,7 @@ double orc_dd_kT(double T_K) { return (T_K * BOLZ_REF * J_IN_EV_REF); }
 /* StrainRate_Orowan (src/constitutive.c:507-721), FORWARD_PLUS_BACKWARD
  * branch.  rho_m is read (passing stress) before it is recomputed, system by
  * system, so the first call after the densities change uses the previous
- * call's mobile density (Q22).  Returns ORC_ERR_NAN where the reference's
+ * call's mobile density (Q19).  Returns ORC_ERR_NAN where the reference's
  * assert(isnan(...)==0) aborts.                                         */
 int orc_strain_rate_dd(const double stress[6], double edot[6], double dedot[6][6], int nsys,
                        const double sc[][5], const orc_dd_phase *ph, const double rho_s[],
@@ -1470,7 +1487,7 @@ static void dd_context(orc_job *J, int v, orc_dd_phase *ph, nrctx *dd)
 
 /* Gradient_ExchangeGrainID, one slab (src/constitutive.c:724-873): whether
  * the +x, -x, +y, -y, +z neighbours share the voxel's grain; the sixth flag
- * compares with the -y neighbour, as the reference does (Q23)             */
+ * compares with the -y neighbour, as the reference does (Q20)             */
 static void dd_grain_checks(orc_job *J)
 {
     const int nx = J->CellDim[0], ny = J->CellDim[1], nz = J->CellDim[2];
@@ -1529,7 +1546,7 @@ static void dd_gradient(orc_job *J)
 /* DislocationEvolution (src/constitutive.c:1346-1520): SSD production,
  * annihilation and climb, first-order GND rates from the shear-rate
  * gradient.  The climb branch for istep > 20000000 and the diagnostic
- * rho_dipole / rho_forest sums are not restated (Q24).                    */
+ * rho_dipole / rho_forest sums are not restated (Q21).                    */
 static void dd_evolution(orc_job *J)
 {
     const double kT = orc_dd_kT(J->T_K), dt = J->TimeStep;

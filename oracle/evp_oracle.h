@@ -1,6 +1,7 @@
 /*
  * evp_oracle.h -- CPU restatement of the EVPFFT Green-operator fixed-point
- * loop (power-law / Voce build) of M-THEMES, used ONLY as the checker.
+ * loop (power-law / Voce build and the dislocation-density build) of
+ * M-THEMES, used ONLY as the checker.
  *
  * TEST INFRASTRUCTURE.  Nothing in the product links or calls this code:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
