@@ -162,8 +162,12 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
                     const double itc = in.tpn[(EVPGPU_NSYSMX + k) * in.tps];
                     const double g0k = in.tpn[(2 * EVPGPU_NSYSMX + k) * in.tps];
                     const double x = (a - in.itf[k]) * itc;
-                    r2 = g0k * sinh(x) * (double)(2 * (r > 0) - 1);
-                    if (DD) r1 = g0k * cosh(x) * itc;
+                    // sinh and cosh from one expm1: with m = e^x - 1 and
+                    // q = e^-x, sinh x = m (1 + q) / 2 (no cancellation at
+                    // small x) and cosh x = (1 + m + q) / 2
+                    const double m = expm1(x), q = 1.0 / (1.0 + m);
+                    r2 = g0k * (0.5 * m * (1.0 + q)) * (double)(2 * (r > 0) - 1);
+                    if (DD) r1 = g0k * (0.5 * ((1.0 + m) + q)) * itc;
                 }
             } else {
             double itau = (r < 0) ? in.itb[k] : in.itf[k];
