@@ -6,9 +6,12 @@ Runs the CPU oracle (oracle/) on
   * a 32^3 synthetic job (64 Voronoi grains, seed 0x5EED0001), 3 increments;
 and stores per-increment Green iteration counts, macroscopic stress and the
 vm.out row, plus the final stress / strain-rate / displacement-gradient /
-CRSS fields on every 16th voxel.
+CRSS fields on every 16th voxel.  The test_run inputs are also run under the
+dislocation-density law the reference's build compiles in
+(oracle_test_run_dd.npz, with the densities on the same voxels).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py        # all fixtures
+    python tests/golden/make_golden.py dd     # the dislocation-density one only
 """
 import os
 import sys
@@ -27,8 +30,8 @@ from pyevp import synth  # noqa: E402
 STRIDE = 16
 
 
-def record(inp, nsteps, out):
-    J = O.Job(inp)
+def record(inp, nsteps, out, law=O.LAW_POWER):
+    J = O.Job(inp, law)
     iters, sig_avg, vm = [], [], []
     for _ in range(nsteps):
         iters.append(int(J.run(1)[0]))
@@ -36,15 +39,23 @@ def record(inp, nsteps, out):
         sig_avg.append(m["SigAvg"].ravel())
         vm.append([m[k] for k in ("TimeTot", "e_vm", "evmp", "d_vm", "dvmp", "s_vm", "s_vm1")])
     sel = np.arange(0, J.n, STRIDE)
+    extra = {}
+    if law == O.LAW_DD:
+        extra = dict(rho_s=J.field(O.RHO_S)[sel], rho_m=J.field(O.RHO_M)[sel],
+                     rho_F=J.field(O.RHO_F)[sel], rho_P=J.field(O.RHO_P)[sel])
     np.savez_compressed(
         out, iters=np.array(iters), sig_avg=np.array(sig_avg), vm=np.array(vm), sel=sel,
         sig=J.field(O.SIG)[sel], edot=J.field(O.EDOT)[sel], disgrad=J.field(O.DISGRAD)[sel],
         crss=J.field(O.CRSS)[sel], tranmat=J.field(O.TRANMAT)[sel], grain=J.grain(),
-        phase=J.phase())
+        phase=J.phase(), **extra)
     print(out, iters, flush=True)
 
 
 if __name__ == "__main__":
+    record(os.path.join(HERE, "test_run", "input.in"), 2, os.path.join(HERE, "oracle_test_run_dd.npz"),
+           O.LAW_DD)
+    if sys.argv[1:] == ["dd"]:
+        sys.exit(0)
     record(os.path.join(HERE, "test_run", "input.in"), 2, os.path.join(HERE, "oracle_test_run.npz"))
     with tempfile.TemporaryDirectory() as d:
         inp = synth.write_job(d, synth.config_c1())

@@ -132,3 +132,52 @@ def test_dd_reference_test_run_constants_freeze_slip(tmp_path):
     s = g.state()
     assert (s["gamdot"] == 0).all() and (s["Edot"] == 0).all()
     g.close()
+
+
+def test_dd_reference_test_run(tmp_path):
+    """The reference's own test_run inputs (32^3, 24 % pores) under the law its
+    build compiles in -- copper elastic constants, Q_slip = 2e10 (no slip),
+    the SSD annihilation term alone moving the densities -- against the
+    oracle's fixture (tests/golden/make_golden.py), then through forge with
+    EVP_LAW=dd."""
+    import os
+    import shutil
+    import subprocess
+    from conftest import GOLDEN
+    g = np.load(os.path.join(GOLDEN, "oracle_test_run_dd.npz"))
+    inp = os.path.join(GOLDEN, "test_run", "input.in")
+    job = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    iters, sig_avg, vm = [], [], []
+    for _ in range(2):
+        it, _ = job.run(1)
+        iters.append(int(it[0]))
+        m = job.macro()
+        sig_avg.append(m["sig_avg"].ravel())
+        vm.append([m[k] for k in ("time_tot", "e_vm", "evmp", "d_vm", "dvmp", "s_vm", "s_vm1")])
+    assert iters == list(g["iters"]), (iters, list(g["iters"]))
+    assert rel(np.array(sig_avg), g["sig_avg"]) < 1e-10
+    assert rel(np.array(vm), g["vm"]) < 1e-10
+    s = job.state()
+    sel = g["sel"]
+    assert (s["gamdot"] == 0).all()
+    for k, f in (("Sig", "sig"), ("Edot", "edot"), ("DisGrad", "disgrad"), ("TranMat", "tranmat")):
+        if np.abs(g[f]).max() == 0.0:
+            assert np.abs(s[k][sel]).max() == 0.0, k
+        else:
+            assert rel(s[k][sel], g[f]) < 1e-10, k
+    d = job.dd_state()
+    for k in ("rho_s", "rho_m", "rho_F", "rho_P"):
+        assert rel(d[k][sel], g[k]) < 1e-10, k
+    _, ph = job.ids()
+    assert (d["rho_s"][ph == 1] < 0.05).all()
+    job.close()
+    for f in ("input.in", "Cu_p.sx", "FFT_microstructure_pore_197.txt"):
+        shutil.copy(os.path.join(GOLDEN, "test_run", f), tmp_path / f)
+    env = dict(os.environ, EVP_LAW="dd")
+    r = subprocess.run([os.path.join(lib.LIBDIR, "forge"), "input.in"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rows = open(tmp_path / "vm.out").read().splitlines()
+    vals = np.array([[float(x) for x in ln.split("\t")] for ln in rows[1:]])
+    # vm.out carries %e (7 significant digits)
+    assert np.allclose(vals, g["vm"], rtol=1e-6, atol=0)
