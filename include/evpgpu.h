@@ -35,7 +35,7 @@ enum {
     EVPGPU_ERR_LINESEARCH = 1203, /* slope >= 0, src/evolution.c:210-214          */
     EVPGPU_ERR_ARG = -1,          /* bad argument                                 */
     EVPGPU_ERR_HIP = -2,          /* HIP runtime failure                          */
-    EVPGPU_ERR_SIZE = -3,         /* grid size not supported (powers of 2, 8..256) */
+    EVPGPU_ERR_SIZE = -3,         /* grid size not supported (each length 2..512)  */
     EVPGPU_ERR_NOMEM = -4,        /* device allocation failed                     */
     EVPGPU_ERR_STATE = -5,        /* call out of order                            */
     EVPGPU_ERR_NODEV = -6         /* no HIP device                                */
@@ -120,7 +120,9 @@ typedef struct {
 
 const char *evpgpu_strerror(int status);
 
-/* Create a context for an nx*ny*nz grid (each a power of 2 in [8, 256]). */
+/* Create a context for an nx*ny*nz grid, each length in [2, 512].  Powers of
+ * two in [8, 256] take the Stockham transforms; other lengths a direct DFT
+ * along that axis (O(L) work per element, correct for any length). */
 int evpgpu_create(evpgpu_ctx **ctx, int nx, int ny, int nz, int precision);
 int evpgpu_destroy(evpgpu_ctx *ctx);
 int evpgpu_set_precision(evpgpu_ctx *ctx, int precision);

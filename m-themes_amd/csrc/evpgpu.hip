@@ -121,7 +121,9 @@ double now_s()
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-bool pow2_ok(int n) { return n >= 8 && n <= 256 && (n & (n - 1)) == 0; }
+// grid lengths: any length in [2, 512]; powers of two from 8 take the
+// Stockham kernels, the others the direct-DFT ones (fft.hip)
+bool len_ok(int n) { return n >= 2 && n <= 512; }
 
 template <typename T> int dalloc(T **p, size_t n)
 {
@@ -387,7 +389,7 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
 {
     if (!out) return EVPGPU_ERR_ARG;
     *out = nullptr;
-    if (!pow2_ok(nx) || !pow2_ok(ny) || !pow2_ok(nz)) return EVPGPU_ERR_SIZE;
+    if (!len_ok(nx) || !len_ok(ny) || !len_ok(nz)) return EVPGPU_ERR_SIZE;
     if (precision != EVPGPU_FP64 && precision != EVPGPU_MIXED) return EVPGPU_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EVPGPU_ERR_NODEV;

@@ -219,6 +219,143 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, d
     }
 }
 
+// ---- any length: direct DFTs.  An axis whose length is not a power of two
+// (FFTW, which the reference calls, takes any length) is transformed by
+// summing each output over the line held in LDS: O(L) work per element
+// instead of O(log L), with the same twiddle table exp(-2 pi i m / L), the
+// same output layout and the same modes as the Stockham kernels above.
+constexpr int DFT_LDS = 4096; // complex values per workgroup: 64 KiB in FP64
+
+__host__ __device__ constexpr int dft_lines(int L) { return L >= DFT_LDS ? 1 : (DFT_LDS / L < TB ? DFT_LDS / L : TB); }
+
+// strided C2C along an axis of length L: the lines of one group, as
+// k_fft_strided tiles them (TBL = dft_lines(L) lines per workgroup)
+template <typename R, bool INV>
+__global__ __launch_bounds__(NT) void k_dft_strided(typename CT<R>::T *data, int L, int ninner, int nbx,
+                                                    long long es, int O2, long long S1, long long S2,
+                                                    const typename CT<R>::T *tw)
+{
+    typedef typename CT<R>::T C;
+    extern __shared__ unsigned char dft_smem[];
+    C *lds = reinterpret_cast<C *>(dft_smem);
+    const int TBL = dft_lines(L);
+    const int t0 = (blockIdx.x % nbx) * TBL;
+    const int o = blockIdx.x / nbx;
+    const long long base = (long long)(o / O2) * S1 + (long long)(o % O2) * S2 + t0;
+    const int nt = min(TBL, ninner - t0);
+    for (int e = threadIdx.x; e < L * TBL; e += NT) {
+        const int i = e / TBL, t = e % TBL;
+        if (t < nt) lds[t * L + i] = data[base + i * es + t];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < L * TBL; e += NT) {
+        const int k = e / TBL, t = e % TBL;
+        if (t >= nt) continue;
+        const C *x = lds + t * L;
+        R re = 0, im = 0;
+        int m = 0; // (n k) mod L
+        for (int n = 0; n < L; n++) {
+            const C w = tw[m];
+            const R wy = INV ? -w.y : w.y;
+            re += x[n].x * w.x - x[n].y * wy;
+            im += x[n].x * wy + x[n].y * w.x;
+            m += k;
+            if (m >= L) m -= L;
+        }
+        data[base + k * es + t] = C{re, im};
+    }
+}
+
+// real lines along z -> half spectrum (IN as k_fft_z_r2c); one workgroup
+// holds TBL lines
+template <typename R, int IN>
+__global__ __launch_bounds__(NT) void k_dft_z_r2c(const double *in, const float *inA, typename CT<R>::T *out,
+                                                  int L, long long nlines, long long N, long long NK,
+                                                  const typename CT<R>::T *tw)
+{
+    typedef typename CT<R>::T C;
+    extern __shared__ unsigned char dft_smem[];
+    R *lds = reinterpret_cast<R *>(dft_smem);
+    const int TBL = dft_lines(L), P = L / 2 + 1;
+    const int comp = blockIdx.y;
+    const long long line0 = (long long)blockIdx.x * TBL;
+    const int nt = (int)min((long long)TBL, nlines - line0);
+    const long long off = comp * N + line0 * L;
+    for (int e = threadIdx.x; e < L * TBL; e += NT) {
+        if (e / L >= nt) continue;
+        R a;
+        if (IN == 0) a = (R)in[off + e];
+        else if (IN == 1) a = (R)(in[off + e] - (double)inA[off + e]);
+        else a = (R)inA[off + e];
+        lds[e] = a;
+    }
+    __syncthreads();
+    C *dst = out + comp * NK + line0 * P;
+    for (int e = threadIdx.x; e < P * TBL; e += NT) {
+        const int t = e / P, k = e % P;
+        if (t >= nt) continue;
+        const R *x = lds + t * L;
+        R re = 0, im = 0;
+        int m = 0;
+        for (int n = 0; n < L; n++) {
+            const C w = tw[m];
+            re += x[n] * w.x;
+            im += x[n] * w.y;
+            m += k;
+            if (m >= L) m -= L;
+        }
+        dst[e] = C{re, im};
+    }
+}
+
+// half spectrum -> real lines along z: Re(IFFT) of the Hermitian extension
+// (imaginary parts of k = 0 and, for even L, k = L/2 dropped), then MODE as
+// k_fft_z_c2r
+template <typename R, int MODE>
+__global__ __launch_bounds__(NT) void k_dft_z_c2r(const typename CT<R>::T *in, double *target, const float *fa,
+                                                  float *fo, const double *base, int L, long long nlines,
+                                                  long long N, long long NK, double a0, double a1, double a2,
+                                                  double a3, double a4, double a5, double scale,
+                                                  const typename CT<R>::T *tw)
+{
+    typedef typename CT<R>::T C;
+    extern __shared__ unsigned char dft_smem[];
+    C *lds = reinterpret_cast<C *>(dft_smem);
+    const int TBL = dft_lines(L), P = L / 2 + 1, kmax = (L - 1) / 2;
+    const int comp = blockIdx.y;
+    const long long line0 = (long long)blockIdx.x * TBL;
+    const int nt = (int)min((long long)TBL, nlines - line0);
+    const C *src = in + comp * NK + line0 * P;
+    for (int e = threadIdx.x; e < P * TBL; e += NT)
+        if (e / P < nt) lds[e] = src[e];
+    __syncthreads();
+    const double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
+    const long long off = comp * N + line0 * L;
+    for (int e = threadIdx.x; e < L * TBL; e += NT) {
+        const int t = e / L, n = e % L;
+        if (t >= nt) continue;
+        const C *X = lds + t * P;
+        R s = 0;
+        int m = n; // (n k) mod L at k = 1
+        for (int k = 1; k <= kmax; k++) {
+            const C w = tw[m]; // Re(X w^-nk) = X.x w.x + X.y w.y
+            s += X[k].x * w.x + X[k].y * w.y;
+            m += n;
+            if (m >= L) m -= L;
+        }
+        R v = X[0].x + (R)2 * s;
+        if ((L & 1) == 0) v += (n & 1) ? -X[L / 2].x : X[L / 2].x;
+        const double r = (double)v * scale;
+        if (MODE == 0) target[off + e] += add + r;
+        else if (MODE == 1) target[off + e] = r;
+        else if (MODE == 2) target[off + e] += add + (double)fa[off + e] + r;
+        else if (MODE == 3) fo[off + e] = (float)r;
+        else target[off + e] = base[off + e] + r;
+    }
+}
+
+bool is_pow2_len(int L) { return L >= 8 && L <= 256 && (L & (L - 1)) == 0; }
+
 template <typename R> const typename CT<R>::T *twp(evpgpu_ctx *c, int axis);
 template <> const double2 *twp<double>(evpgpu_ctx *c, int axis) { return c->tw64[axis]; }
 template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[axis]; }
@@ -239,6 +376,13 @@ template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[a
 template <typename R, bool INV>
 int pass_y(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 {
+    if (!is_pow2_len(c->ny)) {
+        const int L = c->ny, tbl = dft_lines(L), nbx = (c->nzh + tbl - 1) / tbl;
+        hipLaunchKernelGGL((k_dft_strided<R, INV>), dim3(nbx * ncomp * c->nx), dim3(NT),
+                           sizeof(typename CT<R>::T) * L * tbl, c->stream, d, L, c->nzh, nbx, (long long)c->nzh, 1,
+                           (long long)c->ny * c->nzh, 0LL, twp<R>(c, 1));
+        return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+    }
     const int nbx = (c->nzh + TB - 1) / TB;
     dim3 g(nbx * ncomp * c->nx);
 #define CALLY(L) hipLaunchKernelGGL((k_fft_strided<R, L, INV>), g, dim3(NT), 0, c->stream, d, c->nzh, nbx, \
@@ -251,6 +395,13 @@ int pass_y(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 template <typename R, bool INV>
 int pass_x(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 {
+    if (!is_pow2_len(c->nx)) {
+        const int L = c->nx, tbl = dft_lines(L), nbx = (c->nzh + tbl - 1) / tbl;
+        hipLaunchKernelGGL((k_dft_strided<R, INV>), dim3(nbx * ncomp * c->ny), dim3(NT),
+                           sizeof(typename CT<R>::T) * L * tbl, c->stream, d, L, c->nzh, nbx,
+                           (long long)c->ny * c->nzh, c->ny, (long long)c->NK, (long long)c->nzh, twp<R>(c, 0));
+        return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+    }
     const int nbx = (c->nzh + TB - 1) / TB;
     dim3 g(nbx * ncomp * c->ny);
 #define CALLX(L) hipLaunchKernelGGL((k_fft_strided<R, L, INV>), g, dim3(NT), 0, c->stream, d, c->nzh, nbx, \
@@ -264,11 +415,19 @@ int pass_x(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 template <typename R, int IN>
 int fwd(evpgpu_ctx *c, const double *real_in, const float *inA, int ncomp, typename CT<R>::T *spec)
 {
+    if (!is_pow2_len(c->nz)) {
+        const int L = c->nz, tbl = dft_lines(L);
+        const long long nl = (long long)c->nx * c->ny;
+        hipLaunchKernelGGL((k_dft_z_r2c<R, IN>), dim3((unsigned)((nl + tbl - 1) / tbl), ncomp), dim3(NT),
+                           sizeof(R) * L * tbl, c->stream, real_in, inA, spec, L, nl, c->N, c->NK, twp<R>(c, 2));
+        if (hipGetLastError() != hipSuccess) return EVPGPU_ERR_HIP;
+    } else {
     dim3 g((unsigned)(c->nx * c->ny / TB), ncomp);
 #define CALLZ(L) hipLaunchKernelGGL((k_fft_z_r2c<R, L, IN>), g, dim3(NT), 0, c->stream, real_in, inA, \
                                     spec, c->N, c->NK, twp<R>(c, 2))
     FFT_DISPATCH(c->nz, CALLZ);
 #undef CALLZ
+    }
     int st = pass_y<R, false>(c, spec, ncomp);
     if (st) return st;
     return pass_x<R, false>(c, spec, ncomp);
@@ -285,6 +444,14 @@ int inv(evpgpu_ctx *c, typename CT<R>::T *spec, int ncomp, double *target, const
     double a[6] = {0, 0, 0, 0, 0, 0};
     if (addc)
         for (int i = 0; i < ncomp && i < 6; i++) a[i] = addc[i];
+    if (!is_pow2_len(c->nz)) {
+        const int L = c->nz, tbl = dft_lines(L);
+        const long long nl = (long long)c->nx * c->ny;
+        hipLaunchKernelGGL((k_dft_z_c2r<R, MODE>), dim3((unsigned)((nl + tbl - 1) / tbl), ncomp), dim3(NT),
+                           sizeof(typename CT<R>::T) * (L / 2 + 1) * tbl, c->stream, spec, target, fa, fo, base,
+                           L, nl, c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5], scale, twp<R>(c, 2));
+        return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+    }
     dim3 g((unsigned)(c->nx * c->ny / TB), ncomp);
 #define CALLZI(L) hipLaunchKernelGGL((k_fft_z_c2r<R, L, MODE>), g, dim3(NT), 0, c->stream, spec, target, \
                                      fa, fo, base, c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5],   \

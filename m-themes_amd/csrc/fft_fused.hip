@@ -314,7 +314,9 @@ int xg(evpgpu_ctx *c)
 // the fused path covers ny = nz in {8..128} (the plane fits in LDS in FP64)
 bool fused_ok(const evpgpu_ctx *c)
 {
-    return c->ny == c->nz && c->ny >= 8 && c->ny <= 128 && c->nx >= 8 && c->nx <= 256;
+    auto p2 = [](int n) { return (n & (n - 1)) == 0; };
+    return c->ny == c->nz && c->ny >= 8 && c->ny <= 128 && c->nx >= 8 && c->nx <= 256 && p2(c->nx) &&
+           p2(c->ny);
 }
 
 // FP64 iteration: spec = R2C(sigma) ; Green (acc += antisym) ; E += add + C2R/N

@@ -49,8 +49,9 @@ def test_libraries_load_without_gpu():
 
 def test_create_reports_status_instead_of_aborting():
     h = C.c_void_p()
-    # bad sizes are rejected before any device call
-    assert lib.gpu().evpgpu_create(C.byref(h), 12, 8, 8, 0) == -3
+    # bad sizes (each length 2..512) are rejected before any device call
+    assert lib.gpu().evpgpu_create(C.byref(h), 1, 8, 8, 0) == -3
+    assert lib.gpu().evpgpu_create(C.byref(h), 8, 513, 8, 0) == -3
     assert lib.gpu().evpgpu_create(C.byref(h), 8, 8, 8, 7) == -1
 
 

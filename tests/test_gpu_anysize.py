@@ -1,0 +1,69 @@
+"""Grids whose lengths are not powers of two (the reference's FFTW takes any
+length): the direct-DFT transform kernels against numpy, and the whole loop
+against the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pyevp import lib
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(12, 10, 9), (5, 6, 7), (24, 20, 15), (3, 8, 100), (16, 16, 12), (9, 32, 16)]
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("dims", SIZES)
+def test_any_length_r2c_c2r_fp64_vs_numpy(dims):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(dims)
+    ctx = lib.Context(*dims)
+    assert rel(ctx.rfftn(x), np.fft.rfftn(x)) < 1e-12
+    nh = dims[2] // 2 + 1
+    # arbitrary half spectrum, imaginary parts on the kz = 0 (and Nyquist) planes included
+    X = rng.standard_normal(dims[:2] + (nh,)) + 1j * rng.standard_normal(dims[:2] + (nh,))
+    ref = np.fft.irfftn(X, s=dims) * np.prod(dims)
+    assert rel(ctx.irfftn_unnormalised(X), ref) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("dims", [(12, 10, 9), (24, 20, 15)])
+def test_any_length_fp32_vs_numpy(dims):
+    x = np.random.default_rng(6).standard_normal(dims)
+    ctx = lib.Context(*dims, precision=lib.MIXED)
+    assert rel(ctx.rfftn(x), np.fft.rfftn(x)) < 2e-6
+    X = np.fft.rfftn(x)
+    assert rel(ctx.irfftn_unnormalised(X), x * np.prod(dims)) < 2e-6
+    ctx.close()
+
+
+@pytest.mark.parametrize("dims", [(12, 10, 9), (10, 12, 12)])
+def test_loop_parity_any_size(job_factory, dims):
+    """The whole loop on odd and even non-power-of-two grids, pores included,
+    against the oracle (natural convergence, identical iteration counts)."""
+    inp = job_factory(f"anysize{dims}", n=0, dims=dims, grains=6, seed=47, increments=3,
+                      pore_fraction=0.02, pore_radius=1)
+    J = O.Job(inp)
+    g = lib.GpuJob(inp)
+    it_o = J.run(3)
+    it_g, _ = g.run(3)
+    assert list(it_g) == list(it_o), (it_g, it_o)
+    s = g.state()
+    for k, f in (("Sig", O.SIG), ("Edot", O.EDOT), ("DisGrad", O.DISGRAD), ("crss", O.CRSS)):
+        assert rel(s[k], J.field(f)) < 1e-10, k
+    g.close()
+
+
+def test_mixed_precision_any_size(job_factory):
+    inp = job_factory("anysizemix", n=0, dims=(20, 18, 15), grains=8, seed=53, increments=2)
+    J = O.Job(inp)
+    it = J.run(2)
+    g = lib.GpuJob(inp, lib.MIXED)
+    for k in it:
+        g.run(1, forced=int(k))
+    assert rel(g.state()["Sig"], J.field(O.SIG)) < 1e-6
+    g.close()
