@@ -219,14 +219,79 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, d
     }
 }
 
-// ---- any length: direct DFTs.  An axis whose length is not a power of two
-// (FFTW, which the reference calls, takes any length) is transformed by
-// summing each output over the line held in LDS: O(L) work per element
-// instead of O(log L), with the same twiddle table exp(-2 pi i m / L), the
-// same output layout and the same modes as the Stockham kernels above.
+// ---- any length: two-level DFTs.  An axis whose length is not a power of
+// two (FFTW, which the reference calls, takes any length) is transformed with
+// L = L1 L2 (L1 the largest divisor <= sqrt L): length-L1 DFTs down the
+// columns of the L1 x L2 view of each line, the twiddles w^(n2 k1), then
+// length-L2 DFTs along the rows (Cooley-Tukey with direct inner DFTs), the
+// lines of a workgroup and the twiddle table exp(-2 pi i m / L) in LDS.
+// O(L1 + L2) work per element; a prime length is one direct DFT.  Same
+// output layout and modes as the Stockham kernels above.
 constexpr int DFT_LDS = 4096; // complex values per workgroup: 64 KiB in FP64
 
-__host__ __device__ constexpr int dft_lines(int L) { return L >= DFT_LDS ? 1 : (DFT_LDS / L < TB ? DFT_LDS / L : TB); }
+__host__ __device__ constexpr int dft_lines(int L)
+{
+    return (DFT_LDS / L - 1) / 2 < TB ? (DFT_LDS / L - 1) / 2 : TB;
+}
+
+__host__ __device__ inline int dft_l1(int L)
+{
+    int l1 = 1;
+    for (int d = 2; d * d <= L; d++)
+        if (L % d == 0) l1 = d;
+    return l1;
+}
+
+// the TBL lines of a (TBL x L, complex) -> out(t, k) for k < KOUT, via b
+// (TBL x L scratch); tw: the length-L table in LDS.  OUT(t, k, value)
+// stores one output.
+template <bool INV, typename C, typename OUT>
+__device__ __forceinline__ void dft2_lines(const C *a, C *b, const C *tw, int L, int TBL, int nt, int KOUT,
+                                           OUT out)
+{
+    const int L1 = dft_l1(L), L2 = L / L1;
+    // stage A: b[t][k1*L2 + n2] = w^(n2 k1) sum_n1 a[t][n1*L2 + n2] w^(n1 k1 L2)
+    for (int e = threadIdx.x; e < L * TBL; e += NT) {
+        const int t = e / L, r = e % L;
+        if (t >= nt) continue;
+        const int k1 = r / L2, n2 = r % L2;
+        const C *x = a + t * L + n2;
+        const int step = (k1 * L2) % L;
+        decltype(C{}.x) re = 0, im = 0;
+        int m = 0;
+        for (int n1 = 0; n1 < L1; n1++) {
+            const C w = tw[m], v = x[n1 * L2];
+            const auto wy = INV ? -w.y : w.y;
+            re += v.x * w.x - v.y * wy;
+            im += v.x * wy + v.y * w.x;
+            m += step;
+            if (m >= L) m -= L;
+        }
+        const C w = tw[(n2 * k1) % L];
+        const auto wy = INV ? -w.y : w.y;
+        b[t * L + r] = C{re * w.x - im * wy, re * wy + im * w.x};
+    }
+    __syncthreads();
+    // stage B: X[k1 + L1 k2] = sum_n2 b[t][k1*L2 + n2] w^(n2 k2 L1)
+    for (int e = threadIdx.x; e < KOUT * TBL; e += NT) {
+        const int t = e / KOUT, k = e % KOUT;
+        if (t >= nt) continue;
+        const int k1 = k % L1, k2 = k / L1;
+        const C *y = b + t * L + k1 * L2;
+        const int step = (k2 * L1) % L;
+        decltype(C{}.x) re = 0, im = 0;
+        int m = 0;
+        for (int n2 = 0; n2 < L2; n2++) {
+            const C w = tw[m], v = y[n2];
+            const auto wy = INV ? -w.y : w.y;
+            re += v.x * w.x - v.y * wy;
+            im += v.x * wy + v.y * w.x;
+            m += step;
+            if (m >= L) m -= L;
+        }
+        out(t, k, C{re, im});
+    }
+}
 
 // strided C2C along an axis of length L: the lines of one group, as
 // k_fft_strided tiles them (TBL = dft_lines(L) lines per workgroup)
@@ -237,37 +302,23 @@ __global__ __launch_bounds__(NT) void k_dft_strided(typename CT<R>::T *data, int
 {
     typedef typename CT<R>::T C;
     extern __shared__ unsigned char dft_smem[];
-    C *lds = reinterpret_cast<C *>(dft_smem);
     const int TBL = dft_lines(L);
+    C *a = reinterpret_cast<C *>(dft_smem), *b = a + TBL * L, *tws = b + TBL * L;
     const int t0 = (blockIdx.x % nbx) * TBL;
     const int o = blockIdx.x / nbx;
     const long long base = (long long)(o / O2) * S1 + (long long)(o % O2) * S2 + t0;
     const int nt = min(TBL, ninner - t0);
+    for (int i = threadIdx.x; i < L; i += NT) tws[i] = tw[i];
     for (int e = threadIdx.x; e < L * TBL; e += NT) {
         const int i = e / TBL, t = e % TBL;
-        if (t < nt) lds[t * L + i] = data[base + i * es + t];
+        if (t < nt) a[t * L + i] = data[base + i * es + t];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < L * TBL; e += NT) {
-        const int k = e / TBL, t = e % TBL;
-        if (t >= nt) continue;
-        const C *x = lds + t * L;
-        R re = 0, im = 0;
-        int m = 0; // (n k) mod L
-        for (int n = 0; n < L; n++) {
-            const C w = tw[m];
-            const R wy = INV ? -w.y : w.y;
-            re += x[n].x * w.x - x[n].y * wy;
-            im += x[n].x * wy + x[n].y * w.x;
-            m += k;
-            if (m >= L) m -= L;
-        }
-        data[base + k * es + t] = C{re, im};
-    }
+    dft2_lines<INV>(a, b, tws, L, TBL, nt, L,
+                    [&](int t, int k, C v) { data[base + k * es + t] = v; });
 }
 
-// real lines along z -> half spectrum (IN as k_fft_z_r2c); one workgroup
-// holds TBL lines
+// real lines along z -> half spectrum (IN as k_fft_z_r2c)
 template <typename R, int IN>
 __global__ __launch_bounds__(NT) void k_dft_z_r2c(const double *in, const float *inA, typename CT<R>::T *out,
                                                   int L, long long nlines, long long N, long long NK,
@@ -275,37 +326,24 @@ __global__ __launch_bounds__(NT) void k_dft_z_r2c(const double *in, const float 
 {
     typedef typename CT<R>::T C;
     extern __shared__ unsigned char dft_smem[];
-    R *lds = reinterpret_cast<R *>(dft_smem);
     const int TBL = dft_lines(L), P = L / 2 + 1;
+    C *a = reinterpret_cast<C *>(dft_smem), *b = a + TBL * L, *tws = b + TBL * L;
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TBL;
     const int nt = (int)min((long long)TBL, nlines - line0);
     const long long off = comp * N + line0 * L;
+    for (int i = threadIdx.x; i < L; i += NT) tws[i] = tw[i];
     for (int e = threadIdx.x; e < L * TBL; e += NT) {
         if (e / L >= nt) continue;
-        R a;
-        if (IN == 0) a = (R)in[off + e];
-        else if (IN == 1) a = (R)(in[off + e] - (double)inA[off + e]);
-        else a = (R)inA[off + e];
-        lds[e] = a;
+        R v;
+        if (IN == 0) v = (R)in[off + e];
+        else if (IN == 1) v = (R)(in[off + e] - (double)inA[off + e]);
+        else v = (R)inA[off + e];
+        a[e] = C{v, (R)0};
     }
     __syncthreads();
     C *dst = out + comp * NK + line0 * P;
-    for (int e = threadIdx.x; e < P * TBL; e += NT) {
-        const int t = e / P, k = e % P;
-        if (t >= nt) continue;
-        const R *x = lds + t * L;
-        R re = 0, im = 0;
-        int m = 0;
-        for (int n = 0; n < L; n++) {
-            const C w = tw[m];
-            re += x[n] * w.x;
-            im += x[n] * w.y;
-            m += k;
-            if (m >= L) m -= L;
-        }
-        dst[e] = C{re, im};
-    }
+    dft2_lines<false>(a, b, tws, L, TBL, nt, P, [&](int t, int k, C v) { dst[t * P + k] = v; });
 }
 
 // half spectrum -> real lines along z: Re(IFFT) of the Hermitian extension
@@ -320,38 +358,38 @@ __global__ __launch_bounds__(NT) void k_dft_z_c2r(const typename CT<R>::T *in, d
 {
     typedef typename CT<R>::T C;
     extern __shared__ unsigned char dft_smem[];
-    C *lds = reinterpret_cast<C *>(dft_smem);
-    const int TBL = dft_lines(L), P = L / 2 + 1, kmax = (L - 1) / 2;
+    const int TBL = dft_lines(L), P = L / 2 + 1;
+    C *a = reinterpret_cast<C *>(dft_smem), *b = a + TBL * L, *tws = b + TBL * L;
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TBL;
     const int nt = (int)min((long long)TBL, nlines - line0);
     const C *src = in + comp * NK + line0 * P;
-    for (int e = threadIdx.x; e < P * TBL; e += NT)
-        if (e / P < nt) lds[e] = src[e];
+    for (int i = threadIdx.x; i < L; i += NT) tws[i] = tw[i];
+    for (int e = threadIdx.x; e < L * TBL; e += NT) {
+        const int t = e / L, k = e % L;
+        if (t >= nt) continue;
+        C v;
+        if (k < P) {
+            v = src[t * P + k];
+            if (k == 0 || 2 * k == L) v.y = (R)0;
+        } else {
+            v = src[t * P + (L - k)];
+            v.y = -v.y;
+        }
+        a[e] = v;
+    }
     __syncthreads();
     const double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
     const long long off = comp * N + line0 * L;
-    for (int e = threadIdx.x; e < L * TBL; e += NT) {
-        const int t = e / L, n = e % L;
-        if (t >= nt) continue;
-        const C *X = lds + t * P;
-        R s = 0;
-        int m = n; // (n k) mod L at k = 1
-        for (int k = 1; k <= kmax; k++) {
-            const C w = tw[m]; // Re(X w^-nk) = X.x w.x + X.y w.y
-            s += X[k].x * w.x + X[k].y * w.y;
-            m += n;
-            if (m >= L) m -= L;
-        }
-        R v = X[0].x + (R)2 * s;
-        if ((L & 1) == 0) v += (n & 1) ? -X[L / 2].x : X[L / 2].x;
-        const double r = (double)v * scale;
-        if (MODE == 0) target[off + e] += add + r;
-        else if (MODE == 1) target[off + e] = r;
-        else if (MODE == 2) target[off + e] += add + (double)fa[off + e] + r;
-        else if (MODE == 3) fo[off + e] = (float)r;
-        else target[off + e] = base[off + e] + r;
-    }
+    dft2_lines<true>(a, b, tws, L, TBL, nt, L, [&](int t, int n, C z) {
+        const long long i = off + (long long)t * L + n;
+        const double r = (double)z.x * scale;
+        if (MODE == 0) target[i] += add + r;
+        else if (MODE == 1) target[i] = r;
+        else if (MODE == 2) target[i] += add + (double)fa[i] + r;
+        else if (MODE == 3) fo[i] = (float)r;
+        else target[i] = base[i] + r;
+    });
 }
 
 bool is_pow2_len(int L) { return L >= 8 && L <= 256 && (L & (L - 1)) == 0; }
@@ -379,7 +417,7 @@ int pass_y(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
     if (!is_pow2_len(c->ny)) {
         const int L = c->ny, tbl = dft_lines(L), nbx = (c->nzh + tbl - 1) / tbl;
         hipLaunchKernelGGL((k_dft_strided<R, INV>), dim3(nbx * ncomp * c->nx), dim3(NT),
-                           sizeof(typename CT<R>::T) * L * tbl, c->stream, d, L, c->nzh, nbx, (long long)c->nzh, 1,
+                           sizeof(typename CT<R>::T) * (2 * tbl + 1) * L, c->stream, d, L, c->nzh, nbx, (long long)c->nzh, 1,
                            (long long)c->ny * c->nzh, 0LL, twp<R>(c, 1));
         return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
     }
@@ -398,7 +436,7 @@ int pass_x(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
     if (!is_pow2_len(c->nx)) {
         const int L = c->nx, tbl = dft_lines(L), nbx = (c->nzh + tbl - 1) / tbl;
         hipLaunchKernelGGL((k_dft_strided<R, INV>), dim3(nbx * ncomp * c->ny), dim3(NT),
-                           sizeof(typename CT<R>::T) * L * tbl, c->stream, d, L, c->nzh, nbx,
+                           sizeof(typename CT<R>::T) * (2 * tbl + 1) * L, c->stream, d, L, c->nzh, nbx,
                            (long long)c->ny * c->nzh, c->ny, (long long)c->NK, (long long)c->nzh, twp<R>(c, 0));
         return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
     }
@@ -419,7 +457,8 @@ int fwd(evpgpu_ctx *c, const double *real_in, const float *inA, int ncomp, typen
         const int L = c->nz, tbl = dft_lines(L);
         const long long nl = (long long)c->nx * c->ny;
         hipLaunchKernelGGL((k_dft_z_r2c<R, IN>), dim3((unsigned)((nl + tbl - 1) / tbl), ncomp), dim3(NT),
-                           sizeof(R) * L * tbl, c->stream, real_in, inA, spec, L, nl, c->N, c->NK, twp<R>(c, 2));
+                           sizeof(typename CT<R>::T) * (2 * tbl + 1) * L, c->stream, real_in, inA, spec, L, nl, c->N,
+                           c->NK, twp<R>(c, 2));
         if (hipGetLastError() != hipSuccess) return EVPGPU_ERR_HIP;
     } else {
     dim3 g((unsigned)(c->nx * c->ny / TB), ncomp);
@@ -448,7 +487,7 @@ int inv(evpgpu_ctx *c, typename CT<R>::T *spec, int ncomp, double *target, const
         const int L = c->nz, tbl = dft_lines(L);
         const long long nl = (long long)c->nx * c->ny;
         hipLaunchKernelGGL((k_dft_z_c2r<R, MODE>), dim3((unsigned)((nl + tbl - 1) / tbl), ncomp), dim3(NT),
-                           sizeof(typename CT<R>::T) * (L / 2 + 1) * tbl, c->stream, spec, target, fa, fo, base,
+                           sizeof(typename CT<R>::T) * (2 * tbl + 1) * L, c->stream, spec, target, fa, fo, base,
                            L, nl, c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5], scale, twp<R>(c, 2));
         return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
     }
