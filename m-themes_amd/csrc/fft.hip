@@ -392,7 +392,16 @@ __global__ __launch_bounds__(NT) void k_dft_z_c2r(const typename CT<R>::T *in, d
     });
 }
 
-bool is_pow2_len(int L) { return L >= 8 && L <= 256 && (L & (L - 1)) == 0; }
+// lengths with a Stockham plan (2^m 3^a 5^b, divisible by 4, in [8, 256];
+// the list of FFT_DISPATCH); the others take the two-level DFT kernels
+bool is_stockham_len(int L)
+{
+    static const int lens[] = {8, 12, 16, 20, 24, 32, 36, 40, 48, 60, 64, 72, 80, 96, 100, 120,
+                               128, 144, 160, 192, 200, 240, 256};
+    for (int l : lens)
+        if (l == L) return true;
+    return false;
+}
 
 template <typename R> const typename CT<R>::T *twp(evpgpu_ctx *c, int axis);
 template <> const double2 *twp<double>(evpgpu_ctx *c, int axis) { return c->tw64[axis]; }
@@ -402,19 +411,35 @@ template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[a
 #define FFT_DISPATCH(L_, CALL) \
     switch (L_) {              \
     case 8: CALL(8); break;    \
+    case 12: CALL(12); break;  \
     case 16: CALL(16); break;  \
+    case 20: CALL(20); break;  \
+    case 24: CALL(24); break;  \
     case 32: CALL(32); break;  \
+    case 36: CALL(36); break;  \
+    case 40: CALL(40); break;  \
+    case 48: CALL(48); break;  \
+    case 60: CALL(60); break;  \
     case 64: CALL(64); break;  \
+    case 72: CALL(72); break;  \
+    case 80: CALL(80); break;  \
+    case 96: CALL(96); break;  \
+    case 100: CALL(100); break; \
+    case 120: CALL(120); break; \
     case 128: CALL(128); break; \
+    case 144: CALL(144); break; \
+    case 160: CALL(160); break; \
+    case 192: CALL(192); break; \
+    case 200: CALL(200); break; \
+    case 240: CALL(240); break; \
     case 256: CALL(256); break; \
-    case 512: CALL(512); break; \
     default: return EVPGPU_ERR_SIZE; \
     }
 
 template <typename R, bool INV>
 int pass_y(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 {
-    if (!is_pow2_len(c->ny)) {
+    if (!is_stockham_len(c->ny)) {
         const int L = c->ny, tbl = dft_lines(L), nbx = (c->nzh + tbl - 1) / tbl;
         hipLaunchKernelGGL((k_dft_strided<R, INV>), dim3(nbx * ncomp * c->nx), dim3(NT),
                            sizeof(typename CT<R>::T) * (2 * tbl + 1) * L, c->stream, d, L, c->nzh, nbx, (long long)c->nzh, 1,
@@ -433,7 +458,7 @@ int pass_y(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 template <typename R, bool INV>
 int pass_x(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 {
-    if (!is_pow2_len(c->nx)) {
+    if (!is_stockham_len(c->nx)) {
         const int L = c->nx, tbl = dft_lines(L), nbx = (c->nzh + tbl - 1) / tbl;
         hipLaunchKernelGGL((k_dft_strided<R, INV>), dim3(nbx * ncomp * c->ny), dim3(NT),
                            sizeof(typename CT<R>::T) * (2 * tbl + 1) * L, c->stream, d, L, c->nzh, nbx,
@@ -453,7 +478,8 @@ int pass_x(evpgpu_ctx *c, typename CT<R>::T *d, int ncomp)
 template <typename R, int IN>
 int fwd(evpgpu_ctx *c, const double *real_in, const float *inA, int ncomp, typename CT<R>::T *spec)
 {
-    if (!is_pow2_len(c->nz)) {
+    // the Stockham z kernels take whole groups of TB lines
+    if (!is_stockham_len(c->nz) || (c->nx * c->ny) % TB != 0) {
         const int L = c->nz, tbl = dft_lines(L);
         const long long nl = (long long)c->nx * c->ny;
         hipLaunchKernelGGL((k_dft_z_r2c<R, IN>), dim3((unsigned)((nl + tbl - 1) / tbl), ncomp), dim3(NT),
@@ -483,7 +509,7 @@ int inv(evpgpu_ctx *c, typename CT<R>::T *spec, int ncomp, double *target, const
     double a[6] = {0, 0, 0, 0, 0, 0};
     if (addc)
         for (int i = 0; i < ncomp && i < 6; i++) a[i] = addc[i];
-    if (!is_pow2_len(c->nz)) {
+    if (!is_stockham_len(c->nz) || (c->nx * c->ny) % TB != 0) {
         const int L = c->nz, tbl = dft_lines(L);
         const long long nl = (long long)c->nx * c->ny;
         hipLaunchKernelGGL((k_dft_z_c2r<R, MODE>), dim3((unsigned)((nl + tbl - 1) / tbl), ncomp), dim3(NT),

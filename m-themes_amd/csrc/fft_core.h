@@ -1,6 +1,6 @@
 // fft_core.h -- Stockham FFT building blocks shared by the FFT passes
 // (fft.hip) and the fused plane / line kernels (fft_fused.hip): complex
-// helpers, radix-2/4/8 butterflies, the radix plan of a power-of-2 length
+// helpers, radix-2/3/4/5/8 butterflies, the radix plan of a 2^m 3^a 5^b length
 // and a generic LDS stage over any set of lines.
 #pragma once
 
@@ -62,24 +62,72 @@ template <bool INV, typename C> __device__ __forceinline__ void bf8(C v[8])
     }
 }
 
+// radix 3: y0 = a + b + c, y1,2 = a - (b + c)/2 -+ i (sqrt 3 / 2)(b - c)
+// (forward; the signs of the i terms swap for the inverse)
+template <bool INV, typename C> __device__ __forceinline__ void bf3(C v[3])
+{
+    typedef decltype(v[0].x) R;
+    const R h = (R)0.86602540378443864676; // sqrt(3)/2
+    C s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    C m = {v[0].x - (R)0.5 * s.x, v[0].y - (R)0.5 * s.y};
+    C r = cmi<INV>(C{h * d.x, h * d.y});
+    v[0] = cadd(v[0], s);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+}
+
+// radix 5 from the real and imaginary parts of w = exp(-+2 pi i / 5):
+// y_k = a + sum_j w^(jk) x_j, pairing x1 + x4, x2 + x3 (cosine terms) and
+// x1 - x4, x2 - x3 (sine terms)
+template <bool INV, typename C> __device__ __forceinline__ void bf5(C v[5])
+{
+    typedef decltype(v[0].x) R;
+    const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410; // cos 2pi/5, cos 4pi/5
+    const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917; // sin 2pi/5, sin 4pi/5
+    C p1 = cadd(v[1], v[4]), m1 = csub(v[1], v[4]);
+    C p2 = cadd(v[2], v[3]), m2 = csub(v[2], v[3]);
+    C a = v[0];
+    C t1 = {a.x + c1 * p1.x + c2 * p2.x, a.y + c1 * p1.y + c2 * p2.y};
+    C t2 = {a.x + c2 * p1.x + c1 * p2.x, a.y + c2 * p1.y + c1 * p2.y};
+    // -i (s1 m1 + s2 m2) and -i (s2 m1 - s1 m2) forward, +i inverse
+    C u1 = cmi<INV>(C{s1 * m1.x + s2 * m2.x, s1 * m1.y + s2 * m2.y});
+    C u2 = cmi<INV>(C{s2 * m1.x - s1 * m2.x, s2 * m1.y - s1 * m2.y});
+    v[0] = cadd(a, cadd(p1, p2));
+    v[1] = cadd(t1, u1);
+    v[4] = csub(t1, u1);
+    v[2] = cadd(t2, u2);
+    v[3] = csub(t2, u2);
+}
+
 template <bool INV, int RAD, typename C> __device__ __forceinline__ void bfly(C v[RAD])
 {
     if constexpr (RAD == 2) bf2<INV>(v[0], v[1]);
+    else if constexpr (RAD == 3) bf3<INV>(v);
     else if constexpr (RAD == 4) bf4<INV>(v);
+    else if constexpr (RAD == 5) bf5<INV>(v);
     else bf8<INV>(v);
 }
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-// radix of stage s for length L: radix 8 first, then the 4 or 2 left over
+// multiplicity of the factor f in L
+constexpr int fcount(int L, int f) { return (L % f) ? 0 : 1 + fcount(L / f, f); }
+constexpr int ipowi(int b, int e) { return e == 0 ? 1 : b * ipowi(b, e - 1); }
+// radix plan of L = 2^m 3^a 5^b: the radix-5 stages, the radix-3 stages,
+// then radix 8 and the 4 or 2 left over
 constexpr int radix_at(int L, int s)
 {
-    int m = ilog2(L), q = m / 3, r = m % 3;
+    const int c5 = fcount(L, 5), c3 = fcount(L, 3);
+    if (s < c5) return 5;
+    if (s < c5 + c3) return 3;
+    s -= c5 + c3;
+    int m = ilog2(L / ipowi(5, c5) / ipowi(3, c3)), q = m / 3, r = m % 3;
     return s < q ? 8 : (r == 2 ? 4 : 2);
 }
 constexpr int nstages(int L)
 {
-    int m = ilog2(L);
-    return m / 3 + (m % 3 ? 1 : 0);
+    const int c5 = fcount(L, 5), c3 = fcount(L, 3);
+    int m = ilog2(L / ipowi(5, c5) / ipowi(3, c3));
+    return c5 + c3 + m / 3 + (m % 3 ? 1 : 0);
 }
 
 // One Stockham stage of length-L FFTs on NL lines held in LDS, element e of

@@ -9,7 +9,11 @@ from pyevp import lib
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(12, 10, 9), (5, 6, 7), (24, 20, 15), (3, 8, 100), (16, 16, 12), (9, 32, 16)]
+# lengths of the form 2^m 3^a 5^b divisible by 4 (12, 20, 24, 36, 60, 96, 100,
+# 240, ...) take the mixed-radix Stockham kernels, the others (5, 6, 7, 9, 10,
+# 15, 3) the two-level DFT ones
+SIZES = [(12, 10, 9), (5, 6, 7), (24, 20, 15), (3, 8, 100), (16, 16, 12), (9, 32, 16),
+         (96, 40, 36), (60, 100, 12), (240, 20, 24), (72, 144, 48)]
 
 
 def rel(a, b):
@@ -31,7 +35,7 @@ def test_any_length_r2c_c2r_fp64_vs_numpy(dims):
     ctx.close()
 
 
-@pytest.mark.parametrize("dims", [(12, 10, 9), (24, 20, 15)])
+@pytest.mark.parametrize("dims", [(12, 10, 9), (24, 20, 15), (96, 40, 36)])
 def test_any_length_fp32_vs_numpy(dims):
     x = np.random.default_rng(6).standard_normal(dims)
     ctx = lib.Context(*dims, precision=lib.MIXED)
@@ -41,7 +45,7 @@ def test_any_length_fp32_vs_numpy(dims):
     ctx.close()
 
 
-@pytest.mark.parametrize("dims", [(12, 10, 9), (10, 12, 12)])
+@pytest.mark.parametrize("dims", [(12, 10, 9), (10, 12, 12), (24, 20, 12)])
 def test_loop_parity_any_size(job_factory, dims):
     """The whole loop on odd and even non-power-of-two grids, pores included,
     against the oracle (natural convergence, identical iteration counts)."""
