@@ -242,21 +242,36 @@ template <typename R, int NX> constexpr int tbx()
     return per * 16 <= cap ? 16 : per * 8 <= cap ? 8 : per * 4 <= cap ? 4 : 2;
 }
 
+// plane (ny = nz) and line (nx) lengths of the fused kernels: powers of two
+// and the common mixed-radix lengths
 #define PLANE_DISPATCH(CALL)                                   \
-    if (c->ny == 128 && c->nz == 128) CALL(128, 128);          \
-    else if (c->ny == 64 && c->nz == 64) CALL(64, 64);         \
-    else if (c->ny == 32 && c->nz == 32) CALL(32, 32);         \
-    else if (c->ny == 16 && c->nz == 16) CALL(16, 16);         \
-    else if (c->ny == 8 && c->nz == 8) CALL(8, 8);             \
-    else return EVPGPU_ERR_SIZE;
+    if (c->ny != c->nz) return EVPGPU_ERR_SIZE;                \
+    switch (c->ny) {                                           \
+    case 8: CALL(8, 8); break;                                 \
+    case 16: CALL(16, 16); break;                              \
+    case 24: CALL(24, 24); break;                              \
+    case 32: CALL(32, 32); break;                              \
+    case 48: CALL(48, 48); break;                              \
+    case 64: CALL(64, 64); break;                              \
+    case 96: CALL(96, 96); break;                              \
+    case 120: CALL(120, 120); break;                           \
+    case 128: CALL(128, 128); break;                           \
+    default: return EVPGPU_ERR_SIZE;                           \
+    }
 
 #define LINE_DISPATCH(CALL)             \
     switch (c->nx) {                    \
     case 8: CALL(8); break;             \
     case 16: CALL(16); break;           \
+    case 24: CALL(24); break;           \
     case 32: CALL(32); break;           \
+    case 48: CALL(48); break;           \
     case 64: CALL(64); break;           \
+    case 96: CALL(96); break;           \
+    case 120: CALL(120); break;         \
     case 128: CALL(128); break;         \
+    case 192: CALL(192); break;         \
+    case 240: CALL(240); break;         \
     case 256: CALL(256); break;         \
     default: return EVPGPU_ERR_SIZE;    \
     }
@@ -311,12 +326,16 @@ int xg(evpgpu_ctx *c)
 
 } // namespace
 
-// the fused path covers ny = nz in {8..128} (the plane fits in LDS in FP64)
+// the fused path covers the plane lengths ny = nz of PLANE_DISPATCH (<= 128:
+// the plane fits in LDS in FP64) and the line lengths nx of LINE_DISPATCH
 bool fused_ok(const evpgpu_ctx *c)
 {
-    auto p2 = [](int n) { return (n & (n - 1)) == 0; };
-    return c->ny == c->nz && c->ny >= 8 && c->ny <= 128 && c->nx >= 8 && c->nx <= 256 && p2(c->nx) &&
-           p2(c->ny);
+    static const int planes[] = {8, 16, 24, 32, 48, 64, 96, 120, 128};
+    static const int lines[] = {8, 16, 24, 32, 48, 64, 96, 120, 128, 192, 240, 256};
+    bool p = false, l = false;
+    for (int n : planes) p = p || (c->ny == n && c->nz == n);
+    for (int n : lines) l = l || c->nx == n;
+    return p && l;
 }
 
 // FP64 iteration: spec = R2C(sigma) ; Green (acc += antisym) ; E += add + C2R/N

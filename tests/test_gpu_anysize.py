@@ -71,3 +71,22 @@ def test_mixed_precision_any_size(job_factory):
         g.run(1, forced=int(k))
     assert rel(g.state()["Sig"], J.field(O.SIG)) < 1e-6
     g.close()
+
+
+def test_fused_mixed_radix_vs_oracle(job_factory):
+    """48 x 24 x 24: the fused plane / line kernels on mixed-radix lengths
+    (24 = 3 x 8, 48 = 3 x 16), two increments forced to 40 Green iterations
+    each in the oracle and on the GPU, FP64 to the loop bar and the mixed
+    loop to 1e-6."""
+    inp = job_factory("fusedmixed", n=0, dims=(48, 24, 24), grains=6, seed=47, increments=2,
+                      pore_fraction=0.02, pore_radius=1)
+    J = O.Job(inp)
+    J.run(2, forced=[40, 40])
+    for prec, tol in ((lib.FP64, 1e-10), (lib.MIXED, 1e-6)):
+        g = lib.GpuJob(inp, prec)
+        g.run(1, forced=40)
+        g.run(1, forced=40)
+        s = g.state()
+        assert rel(s["Sig"], J.field(O.SIG)) < tol, (prec, rel(s["Sig"], J.field(O.SIG)))
+        assert rel(s["DisGrad"], J.field(O.DISGRAD)) < tol
+        g.close()

@@ -267,7 +267,8 @@ def test_loop_parity_two_crystal_phases(job_factory, nrs2):
     _compare_jobs(O.Job(inp), lib.GpuJob(inp), 3)
 
 
-@pytest.mark.parametrize("n,prec", [(16, lib.FP64), (32, lib.FP64), (32, lib.MIXED)])
+@pytest.mark.parametrize("n,prec", [(16, lib.FP64), (32, lib.FP64), (32, lib.MIXED), (24, lib.FP64),
+                                    (48, lib.MIXED)])
 def test_fused_transforms_match_seven_passes(job_factory, n, prec):
     """The three fused FFT/Green passes against the seven-pass path (the one
     the FFT tests pin to numpy): same Green iteration counts, states equal to
