@@ -20,9 +20,9 @@
  *   src/init.c:179-203, 593-702, 821-846, 1025, 1073-1125 (its set-up)
  *
  * Parity pins: tests/test_oracle_pins.py and tests/test_oracle_dd.py.  The
- * SSD and GND rate expressions of DislocationEvolution (dd_evolution) are
- * parity unpinned beyond those runs: they are restated term by term and
- * only checked for the signs and onsets the law implies.
+ * SSD rate expression of DislocationEvolution (dd_evolution) is parity
+ * unpinned: it is restated term by term and only checked for the signs and
+ * growth the law implies (its GND rates are pinned by a closed form).
  *
  * Dispositions of the reference's quirks (shared with the HIP path):
  *   Q1  the power-law driver is reconstructed: the live loop of
@@ -1854,6 +1854,13 @@ double *orc_job_field(orc_job *J, int which)
     case 16: return J->gadot_grad ? &J->gadot_grad[0][0][0] : NULL;
     }
     return NULL;
+}
+
+int orc_job_dd_evolve(orc_job *J)
+{
+    if (J->law != ORC_LAW_DD) return -1;
+    dd_evolution(J);
+    return ORC_OK;
 }
 
 int orc_job_dd_gradient(orc_job *J)

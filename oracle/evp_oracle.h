@@ -120,6 +120,8 @@ void orc_job_dims(const orc_job *job, int dims[3]);
 double *orc_job_field(orc_job *job, int which);
 /* Gradient_ShearRate of the job's current gamdot field into field 16 */
 int orc_job_dd_gradient(orc_job *job);
+/* DislocationEvolution of the job's current fields (one call) */
+int orc_job_dd_evolve(orc_job *job);
 /* the dislocation-density job's tables: shear modulus [2], Burgers lengths
  * [2][12], sense vectors b x n [2][12][3], interaction matrix [2][12][12],
  * projection tables [2][8][12][12]; returns the law                        */

@@ -80,6 +80,7 @@ def lib():
         L.orc_ten4_transform_inv.argtypes = [DP, DP, DP]
         L.orc_set_law.argtypes = [C.c_int]
         L.orc_job_dd_gradient.argtypes = [C.c_void_p]
+        L.orc_job_dd_evolve.argtypes = [C.c_void_p]
         L.orc_job_dd_tables.argtypes = [C.c_void_p, DP, DP, DP, DP, DP]
         L.orc_dd_interaction.argtypes = [DP, DP]
         L.orc_dd_angles.argtypes = [C.c_int, DP, DP, DP]
@@ -166,6 +167,11 @@ class Job:
         if lib().orc_job_dd_gradient(self.h):
             raise ValueError("not a dislocation-density job")
         return self.field(GADOT_GRAD)
+
+    def dd_evolve(self):
+        """One DislocationEvolution call on the job's current fields."""
+        if lib().orc_job_dd_evolve(self.h):
+            raise ValueError("not a dislocation-density job")
 
     def dd_tables(self) -> dict:
         """Shear moduli, Burgers lengths, sense vectors, interaction matrices and

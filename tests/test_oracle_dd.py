@@ -257,3 +257,35 @@ def test_reference_test_run_slip_is_frozen(tmp_path):
     J.run(2)
     assert (J.field(O.GAMDOT) == 0).all()
     assert (J.field(O.EDOT) == 0).all()
+
+
+def test_dd_gnd_rates_from_a_uniform_shear_rate_gradient(tmp_path):
+    """One grain, gamdot of every system a linear field a.x along the grid
+    (periodic wrap excluded by reading interior voxels only): the central
+    differences give the gradient a exactly, and the first-order GND rates
+    are (a . p)/(b L0) for type 1 (p = b x n) and -(a . b)/(b L0) for type 2
+    (src/constitutive.c:1418-1460), integrated over one time step from 0."""
+    law = synth.DDLaw(c=(0.48, 3.0, 3.0, 0.0, 0.0, 0.0, 0.0, 0.24), q_bulk=1e9)
+    cfg = synth.Config(n=8, grains=1, increments=1, dd=law, single_orientation=(0.0, 0.0, 0.0))
+    inp = synth.write_job(str(tmp_path), cfg)
+    J = O.Job(inp, O.LAW_DD)
+    n = 8
+    x, y, z = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    a = np.array([0.3, -0.2, 0.5])
+    f = a[0] * x + a[1] * y + a[2] * z
+    J.set_field(O.GAMDOT, np.repeat(f.reshape(-1, 1), 12, axis=1))
+    J.dd_gradient()
+    J.dd_evolve()
+    inner = ((x > 0) & (x < n - 1) & (y > 0) & (y < n - 1) & (z > 0) & (z < n - 1)).ravel()
+    t = J.dd_tables()
+    b = t["bbB"][0][0]
+    dt, L0 = cfg.dt, law.L0
+    g1 = J.field(O.RHO_G1)[inner]
+    g2 = J.field(O.RHO_G2)[inner]
+    want1 = dt * (np.cross(B_HAT, N_HAT) @ a) / b / L0
+    want2 = -dt * (B_HAT @ a) / b / L0
+    np.testing.assert_allclose(g1, np.broadcast_to(want1, g1.shape), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(g2, np.broadcast_to(want2, g2.shape), rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(J.field(O.RHO_G3), 0.0)
+    # C_4..C_7 = 0 and no climb: the SSD density is left as it was
+    np.testing.assert_array_equal(J.field(O.RHO_S), law.rho_ssd_initial)
