@@ -181,3 +181,19 @@ def test_dd_reference_test_run(tmp_path):
     vals = np.array([[float(x) for x in ln.split("\t")] for ln in rows[1:]])
     # vm.out carries %e (7 significant digits)
     assert np.allclose(vals, g["vm"], rtol=1e-6, atol=0)
+
+
+def test_dd_loop_parity_simple_shear(tmp_path):
+    """Fully strain-controlled simple shear under the DD law: lattice spin
+    from the 12 shear rate, so update_orient turns the orientations between
+    the densities' gradient steps."""
+    inp = _job(tmp_path, "dd8sh", n=8, grains=8, seed=17, increments=3,
+               velgrad_bc_flag=(1, 1, 1, 1, 1, 1), velgrad_bc=(0.0, 0.0, 0.0, 0.0, 0.0, 1.0),
+               stress_bc_flag=(0, 0, 0, 0, 0, 0))
+    J = O.Job(inp, O.LAW_DD)
+    g = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    it_o = J.run(3)
+    it_g, _ = g.run(3)
+    assert list(it_g) == list(it_o), (it_g, it_o)
+    _compare(J, g)
+    g.close()
