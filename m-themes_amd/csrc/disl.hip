@@ -200,6 +200,12 @@ __global__ __launch_bounds__(DD_THREADS) void k_dd_evolve(
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_value(double *p, long long n, double v)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 } // namespace
 
 // kT of src/evp.h:86
@@ -278,25 +284,23 @@ int evpgpu_set_dd_law(evpgpu_ctx *c, const evpgpu_dd_law *law, const evpgpu_dd_p
     c->dd_law = *law;
     memset(c->dd_ph, 0, sizeof(c->dd_ph));
     double *tab = (double *)calloc(2 * TAB, sizeof(double));
-    double *r0 = (double *)malloc(sizeof(double) * NS * N);
-    if (!tab || !r0) {
-        free(tab);
-        free(r0);
-        return EVPGPU_ERR_NOMEM;
-    }
+    if (!tab) return EVPGPU_ERR_NOMEM;
     for (int p = 0; p < nphases; p++) {
         c->dd_ph[p] = phases[p];
         memcpy(tab + p * TAB, phases[p].chi, sizeof(phases[p].chi));
         memcpy(tab + p * TAB + NS * NS, phases[p].cth, sizeof(phases[p].cth));
     }
-    // initial densities (src/init.c:1073-1093): every voxel and system
-    for (long long i = 0; i < NS * N; i++) r0[i] = law->rho_ssd_initial;
     int st = EVPGPU_OK;
-    if (hipMemcpy(c->dd_tab, tab, sizeof(double) * 2 * TAB, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->rho, r0, sizeof(double) * NS * N, hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(c->dd_tab, tab, sizeof(double) * 2 * TAB, hipMemcpyHostToDevice) != hipSuccess)
         st = EVPGPU_ERR_HIP;
     free(tab);
-    free(r0);
+    // initial densities (src/init.c:1073-1093): rho_s of every voxel and
+    // system, the others 0
+    if (!st) {
+        hipLaunchKernelGGL(k_fill_value, dim3((unsigned)((NS * N + 255) / 256)), dim3(256), 0, c->stream, c->rho,
+                           NS * N, law->rho_ssd_initial);
+        if (hipGetLastError() != hipSuccess) st = EVPGPU_ERR_HIP;
+    }
     if (!st) st = fill_zero(c, c->rho + NS * N, 6 * NS * N);
     if (!st) st = fill_zero(c, c->dd_k, 4 * NS * N);
     if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
