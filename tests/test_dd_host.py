@@ -40,3 +40,20 @@ def test_dd_host_tables_match_the_oracle(tmp_path):
     np.testing.assert_array_equal(np.ctypeslib.as_array(out.bbB), t["bbB"][0])
     assert list(out.c) == list(law.c)
     assert (out.q_slip, out.q_bulk, out.poisson_nu) == (law.q_slip, law.q_bulk, 0.35)
+
+
+def test_dd_keys_follow_the_namelist_rules(tmp_path):
+    """The DD keys are read with the namelist's rules: the first definition of
+    a key wins, '#' lines are comments, absent keys read as 0."""
+    inp = tmp_path / "input.in"
+    inp.write_text("# comment\nT__K\t300\nQ_slip\t0.2\t0.0\nT__K\t999\n#C_1 7\nC_1 0.48 0.5\n")
+    L = lib.evp()
+    keys = (C.c_double * 64)()
+    assert L.evp_parse_dd_keys(os.fsencode(str(inp)), keys) == 0
+    k = np.ctypeslib.as_array(keys)
+    # layout of evp_dd_keys: a0[2], L0, GND_ScaleRatio, T__K, Q_slip[2], Q_bulk[2], C[8][2], ...
+    assert k[4] == 300.0                     # T__K: the first definition
+    assert (k[5], k[6]) == (0.2, 0.0)        # Q_slip
+    assert (k[9], k[10]) == (0.48, 0.5)      # C_1 (the commented line skipped)
+    assert k[0] == 0.0 and k[2] == 0.0       # a0, L0 absent
+    assert L.evp_parse_dd_keys(os.fsencode(str(tmp_path / "missing.in")), keys) == -1
