@@ -182,6 +182,10 @@ int evpgpu_set_dd_law(evpgpu_ctx *ctx, const evpgpu_dd_law *law, const evpgpu_dd
  * be NULL. */
 int evpgpu_download_dd_state(evpgpu_ctx *ctx, double *rho_s, double *rho_m, double *rho_g1,
                              double *rho_g2, double *rho_g3, double *rho_f, double *rho_p);
+/* AvgDislocation (src/constitutive.c:1669-1780): the sums over crystal
+ * voxels and systems of rho_F, rho_P, rho_s, rho_m, |rho_g|, rho_dipole and
+ * rho_forest, each divided by the voxel count -- the dd.out columns 3-9. */
+int evpgpu_dd_averages(evpgpu_ctx *ctx, double avg[7]);
 
 /* One loading increment: src/evolution.c:1318-1547 plus hardening.
  * forced_iters > 0 runs exactly that many Green iterations (parity protocol). */

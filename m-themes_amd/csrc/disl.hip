@@ -118,6 +118,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_dd_evolve(
     const int x = (int)(v / ((long long)ny * nz)), y = (int)((v / nz) % ny), z = (int)(v % nz);
     const long long S = NS * N;
     double *rs = rho, *rg1 = rho + 2 * S, *rg2 = rho + 3 * S, *rg3 = rho + 4 * S;
+    double *rdip = rho + 7 * S, *rfor = rho + 8 * S;
     const double *rm = rho + S, *rF = rho + 5 * S;
     // stress in the B-basis (chg_basis opt 2), hydro as the reference forms it
     double s6[6];
@@ -175,6 +176,9 @@ __global__ __launch_bounds__(DD_THREADS) void k_dd_evolve(
         const double gd = gamdot[iv];
         if (D.istep <= 2000000) {
             const double r = rs[iv];
+            // dipole and forest densities formed by slip (diagnostic sums)
+            rdip[iv] = rdip[iv] + fabs(gd) * (c6 * (d_dipole)*rm[iv]) * dt;
+            rfor[iv] = rfor[iv] + fabs(gd) * (c4 * sqrt(rF[iv])) * dt;
             const double dot_ssd =
                 fabs(gd) * (c4 * sqrt(rF[iv]) + c6 * (d_dipole)*rm[iv] - c5 * r) -
                 c7 * 1.6 * 1E12 * 1E-9 * r * r -
@@ -198,6 +202,48 @@ __global__ __launch_bounds__(DD_THREADS) void k_dd_evolve(
             rg3[iv] = rg3[iv] + 0.0 * dt;
         }
     }
+}
+
+// AvgDislocation (src/constitutive.c:1669-1780): per block, the sums over
+// its crystal voxels and their systems of rho_F, rho_P, rho_s, rho_m,
+// |(rho_g1, rho_g2, rho_g3)|, rho_dipole and rho_forest (fixed-order tree)
+constexpr int NAVG = 7;
+__global__ __launch_bounds__(DD_THREADS) void k_dd_avg(long long N, const int *__restrict__ phase,
+                                                       const int *__restrict__ is_gas2,
+                                                       const int *__restrict__ nsys2,
+                                                       const double *__restrict__ rho,
+                                                       double *__restrict__ partials)
+{
+    __shared__ double sh[NAVG][DD_THREADS];
+    const long long v = (long long)blockIdx.x * DD_THREADS + threadIdx.x;
+    double a[NAVG] = {0, 0, 0, 0, 0, 0, 0};
+    if (v < N) {
+        const int p = phase[v] - 1;
+        if (p >= 0 && p <= 1 && !is_gas2[p]) {
+            const long long S = NS * N;
+            for (int i = 0; i < nsys2[p]; i++) {
+                const long long iv = (long long)i * N + v;
+                const double g1 = rho[2 * S + iv], g2 = rho[3 * S + iv], g3 = rho[4 * S + iv];
+                a[0] += rho[5 * S + iv];
+                a[1] += rho[6 * S + iv];
+                a[2] += rho[iv];
+                a[3] += rho[S + iv];
+                a[4] += sqrt(g1 * g1 + g2 * g2 + g3 * g3);
+                a[5] += rho[7 * S + iv];
+                a[6] += rho[8 * S + iv];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NAVG; k++) sh[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int w = DD_THREADS / 2; w > 0; w /= 2) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int k = 0; k < NAVG; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < NAVG) partials[(long long)blockIdx.x * NAVG + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 __global__ __launch_bounds__(256) void k_fill_value(double *p, long long n, double v)
@@ -269,7 +315,7 @@ int evpgpu_set_dd_law(evpgpu_ctx *c, const evpgpu_dd_law *law, const evpgpu_dd_p
     for (int p = 0; p < nphases; p++)
         if (!c->ph[p].is_gas && c->ph[p].nsys != NS) return EVPGPU_ERR_ARG;
     const long long N = c->N;
-    if (!c->rho && hipMalloc((void **)&c->rho, sizeof(double) * 7 * NS * N) != hipSuccess) {
+    if (!c->rho && hipMalloc((void **)&c->rho, sizeof(double) * 9 * NS * N) != hipSuccess) {
         c->rho = nullptr;
         return EVPGPU_ERR_NOMEM;
     }
@@ -301,11 +347,41 @@ int evpgpu_set_dd_law(evpgpu_ctx *c, const evpgpu_dd_law *law, const evpgpu_dd_p
                            NS * N, law->rho_ssd_initial);
         if (hipGetLastError() != hipSuccess) st = EVPGPU_ERR_HIP;
     }
-    if (!st) st = fill_zero(c, c->rho + NS * N, 6 * NS * N);
+    if (!st) st = fill_zero(c, c->rho + NS * N, 8 * NS * N);
     if (!st) st = fill_zero(c, c->dd_k, 4 * NS * N);
     if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
     if (!st) c->dd_on = 1;
     return st;
+}
+
+int evpgpu_dd_averages(evpgpu_ctx *c, double avg[7])
+{
+    if (!c || !avg) return EVPGPU_ERR_ARG;
+    if (!c->dd_on) return EVPGPU_ERR_STATE;
+    int h[4];
+    for (int p = 0; p < 2; p++) {
+        h[p] = p < c->nphases ? c->ph[p].is_gas : 1;
+        h[2 + p] = p < c->nphases ? c->ph[p].nsys : 0;
+    }
+    int *dh = nullptr;
+    if (hipMalloc((void **)&dh, sizeof(h)) != hipSuccess) return EVPGPU_ERR_NOMEM;
+    int st = EVPGPU_OK;
+    const int nb = (int)((c->N + DD_THREADS - 1) / DD_THREADS);
+    if (hipMemcpy(dh, h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
+    if (!st) {
+        hipLaunchKernelGGL(k_dd_avg, dim3(nb), dim3(DD_THREADS), 0, c->stream, c->N, c->phase, dh, dh + 2,
+                           c->rho, c->partials);
+        if (hipGetLastError() != hipSuccess) st = EVPGPU_ERR_HIP;
+    }
+    if (!st) st = reduce_partials(c, c->partials, nb, NAVG, c->red);
+    double s[NAVG];
+    if (!st && hipMemcpyAsync(s, c->red, sizeof(s), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        st = EVPGPU_ERR_HIP;
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
+    (void)hipFree(dh);
+    if (st) return st;
+    for (int k = 0; k < NAVG; k++) avg[k] = s[k] * (1.0 / (double)c->N);
+    return EVPGPU_OK;
 }
 
 int evpgpu_download_dd_state(evpgpu_ctx *c, double *rho_s, double *rho_m, double *rho_g1, double *rho_g2,

@@ -540,7 +540,7 @@ static evp_job *G = NULL;
 static char G_input[1024];
 static int G_abort = 1, G_status = 0, G_precision = EVPGPU_FP64;
 static evp_namelist G_nl;
-static FILE *fp_vm = NULL, *fp_err = NULL;
+static FILE *fp_vm = NULL, *fp_err = NULL, *fp_dd = NULL;
 
 void evp_set_abort_on_error(int on) { G_abort = on; }
 int evp_status(void) { return G_status; }
@@ -570,6 +570,8 @@ void PrintNameList(void) { evp_print_namelist(&G_nl); }
 void OpenFiles(void)
 {
     fp_vm = fopen("vm.out", "w+");
+    /* the dislocation-density build's density averages (src/io.c:220) */
+    if (G_law == EVP_LAW_DD) fp_dd = fopen("dd.out", "w+");
     fp_err = fopen("err.out", "w+");
 }
 
@@ -577,7 +579,8 @@ void CloseFiles(void)
 {
     if (fp_vm) fclose(fp_vm);
     if (fp_err) fclose(fp_err);
-    fp_vm = fp_err = NULL;
+    if (fp_dd) fclose(fp_dd);
+    fp_vm = fp_err = fp_dd = NULL;
 }
 
 void SetupJob(void)
@@ -625,6 +628,17 @@ void Evolution(void)
             fprintf(fp_vm, "%e\t%e\t%e\t%e\t%e\t%e\t%e\n", m.time_tot, m.e_vm, m.evmp, m.d_vm, m.dvmp,
                     m.s_vm, m.s_vm1);
             fflush(fp_vm);
+        }
+        if (fp_dd && G->law == EVP_LAW_DD) {
+            /* src/evolution.c:1565-1570: TimeTot, e_vm and AvgDislocation */
+            double a[7];
+            if ((st = evpgpu_dd_averages(G->ctx, a))) {
+                fail_with(st, "dd.out averages failed");
+                return;
+            }
+            fprintf(fp_dd, "%e\t%e\t%e\t%e\t%e\t%e\t%e\t%e\t%e\n", m.time_tot, m.e_vm, a[0], a[1], a[2],
+                    a[3], a[4], a[5], a[6]);
+            fflush(fp_dd);
         }
         double row[7] = {m.time_tot, m.e_vm, m.evmp, m.d_vm, m.dvmp, m.s_vm, m.s_vm1};
         memcpy(G->vm_row, row, sizeof(row));

@@ -91,7 +91,7 @@ EVPGPU_SYMBOLS = [
     "evpgpu_upload_state", "evpgpu_download_state", "evpgpu_set_macro", "evpgpu_get_macro",
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
     "evpgpu_newton_host", "evpgpu_adjoint_record", "evpgpu_adjoint_run", "evpgpu_copy_probe",
-    "evpgpu_set_dd_law", "evpgpu_download_dd_state",
+    "evpgpu_set_dd_law", "evpgpu_download_dd_state", "evpgpu_dd_averages",
 ]
 EVP_SYMBOLS = [
     "GetNameList", "PrintNameList", "OpenFiles", "CloseFiles", "SetupJob", "Evolution",
@@ -157,6 +157,7 @@ def gpu():
         L.evpgpu_adjoint_record.argtypes = [VP, C.c_int, C.c_int, DP, C.POINTER(Stats)]
         L.evpgpu_adjoint_run.argtypes = [VP, DP, C.c_double, C.c_int, DP, C.POINTER(AdjointStats)]
         L.evpgpu_download_dd_state.argtypes = [VP] + [DP] * 7
+        L.evpgpu_dd_averages.argtypes = [VP, DP]
         _gpu = L
     return _gpu
 
@@ -350,6 +351,12 @@ class GpuJob:
                     crss=a["crss"].reshape(24, n).T.reshape(n, 12, 2),
                     gamacum=a["gac"].copy(), gamdot=a["gd"].reshape(12, n).T.copy())
 
+
+    def dd_averages(self) -> np.ndarray:
+        """AvgDislocation: rho_F, rho_P, rho_s, rho_m, |rho_g|, dipole, forest."""
+        a = np.zeros(7)
+        _chk(gpu().evpgpu_dd_averages(self.ctx, _p(a)), "dd_averages")
+        return a
 
     def dd_state(self) -> dict:
         """Densities of a dislocation-density job, each [N][12] (1E12/m^2)."""

@@ -47,9 +47,8 @@
  *       the previous call's mobile density.  Reproduced.
  *   Q20 Gradient_ExchangeGrainID compares the z-minus side with the -y
  *       neighbour's grain (src/constitutive.c:818, 838, 858).  Reproduced.
- *   Q21 DislocationEvolution: the istep > 20000000 climb branch, the
- *       rho_dipole / rho_forest diagnostics and the second-order GND terms
- *       (compiled out) are not restated.
+ *   Q21 DislocationEvolution: the istep > 20000000 climb branch and the
+ *       second-order GND terms (compiled out) are not restated.
  */
 #include "evp_oracle.h"
 
@@ -901,6 +900,8 @@ struct orc_job {
     double Shear_G[2], bb_B[2][12], bp[2][12][3], Chi[2][12][12], cth[2][8][12][12];
     double (*rho_s)[12], (*rho_m)[12], (*rho_g1)[12], (*rho_g2)[12], (*rho_g3)[12];
     double (*rho_F)[12], (*rho_P)[12], (*gadot_grad)[12][3], *t_last;
+    double (*rho_dipole)[12], (*rho_forest)[12];
+    double dd_avg[7]; /* AvgDislocation: F, P, s, m, |g|, dipole, forest */
     unsigned char (*gbc)[6]; /* GB_checkX/Y/Z [0] and [1] */
 };
 
@@ -1225,7 +1226,7 @@ orc_job *orc_job_setup(const char *input_path, int *status)
          * checks (src/init.c:1025, 1073-1128) */
         ALLOC(J->rho_s, N); ALLOC(J->rho_m, N); ALLOC(J->rho_g1, N); ALLOC(J->rho_g2, N);
         ALLOC(J->rho_g3, N); ALLOC(J->rho_F, N); ALLOC(J->rho_P, N); ALLOC(J->gadot_grad, N);
-        ALLOC(J->t_last, N); ALLOC(J->gbc, N);
+        ALLOC(J->t_last, N); ALLOC(J->gbc, N); ALLOC(J->rho_dipole, N); ALLOC(J->rho_forest, N);
         for (int v = 0; v < N; v++) {
             J->t_last[v] = -1E3 * J->tau_DRX * J->TimeStep;
             for (int i = 0; i < ORC_NSYSMX; i++) J->rho_s[v][i] = J->rho_SSD_initial;
@@ -1329,7 +1330,7 @@ void orc_job_free(orc_job *J)
                     J->DisGrad, J->VelGrad, J->Schm_gr, J->crss, J->gamdot, J->gamacum,
                     J->kbuf, J->ksr, J->ksi, J->g, J->err_s_v, J->err_e_v, J->nr_v,
                     J->rho_s, J->rho_m, J->rho_g1, J->rho_g2, J->rho_g3, J->rho_F, J->rho_P,
-                    J->gadot_grad, J->t_last, J->gbc};
+                    J->gadot_grad, J->t_last, J->gbc, J->rho_dipole, J->rho_forest};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(J);
 }
@@ -1544,9 +1545,9 @@ static void dd_gradient(orc_job *J)
 }
 
 /* DislocationEvolution (src/constitutive.c:1346-1520): SSD production,
- * annihilation and climb, first-order GND rates from the shear-rate
- * gradient.  The climb branch for istep > 20000000 and the diagnostic
- * rho_dipole / rho_forest sums are not restated (Q21).                    */
+ * annihilation and climb, the diagnostic dipole / forest sums, first-order
+ * GND rates from the shear-rate gradient.  The climb branch for
+ * istep > 20000000 is not restated (Q21).                                  */
 static void dd_evolution(orc_job *J)
 {
     const double kT = orc_dd_kT(J->T_K), dt = J->TimeStep;
@@ -1574,6 +1575,8 @@ static void dd_evolution(orc_job *J)
             double dot_SSD = 0.0;
             if (J->istep <= 2000000) {
                 double rs = J->rho_s[v][i];
+                J->rho_dipole[v][i] += fabs(J->gamdot[v][i]) * (c_6 * (d_dipole)*J->rho_m[v][i]) * dt;
+                J->rho_forest[v][i] += fabs(J->gamdot[v][i]) * (c_4 * sqrt(J->rho_F[v][i])) * dt;
                 dot_SSD = fabs(J->gamdot[v][i]) *
                               (c_4 * sqrt(J->rho_F[v][i]) + c_6 * (d_dipole)*J->rho_m[v][i] - c_5 * rs) -
                           c_7 * 1.6 * 1E12 * 1E-9 * rs * rs -
@@ -1597,6 +1600,35 @@ static void dd_evolution(orc_job *J)
             }
         }
     }
+}
+
+/* AvgDislocation (src/constitutive.c:1669-1780): sums over crystal voxels
+ * and systems, divided by the voxel count (gas voxels count as zero)     */
+static void dd_averages(orc_job *J)
+{
+    double a[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 7; k++)
+        for (int v = 0; v < J->Nxyz; v++) {
+            const int jph = J->phase[v];
+            if (J->Type_phases[jph - 1]) continue;
+            for (int i = 0; i < J->nSYS[jph - 1]; i++) {
+                double x;
+                switch (k) {
+                case 0: x = J->rho_F[v][i]; break;
+                case 1: x = J->rho_P[v][i]; break;
+                case 2: x = J->rho_s[v][i]; break;
+                case 3: x = J->rho_m[v][i]; break;
+                case 4:
+                    x = sqrt(J->rho_g1[v][i] * J->rho_g1[v][i] + J->rho_g2[v][i] * J->rho_g2[v][i] +
+                             J->rho_g3[v][i] * J->rho_g3[v][i]);
+                    break;
+                case 5: x = J->rho_dipole[v][i]; break;
+                default: x = J->rho_forest[v][i]; break;
+                }
+                a[k] += x;
+            }
+        }
+    for (int k = 0; k < 7; k++) J->dd_avg[k] = a[k] * (1.0 / J->Nxyz);
 }
 
 static int green_iteration(orc_job *J)
@@ -1786,6 +1818,7 @@ static int increment(orc_job *J, int forced)
          * of the shear rates (Hard_Flag, istep > 1), then the densities */
         if (J->Hard_Flag == 1 && J->istep > 1 && J->istep <= 20000000) dd_gradient(J);
         dd_evolution(J);
+        dd_averages(J);
     } else if (J->Hard_Flag == 1 && J->istep > 1) {
 #pragma omp parallel for schedule(static)
         for (int v = 0; v < N; v++) {
@@ -1852,6 +1885,8 @@ double *orc_job_field(orc_job *J, int which)
     case 14: return J->rho_F ? &J->rho_F[0][0] : NULL;
     case 15: return J->rho_P ? &J->rho_P[0][0] : NULL;
     case 16: return J->gadot_grad ? &J->gadot_grad[0][0][0] : NULL;
+    case 17: return J->rho_dipole ? &J->rho_dipole[0][0] : NULL;
+    case 18: return J->rho_forest ? &J->rho_forest[0][0] : NULL;
     }
     return NULL;
 }
@@ -1884,7 +1919,7 @@ int orc_job_dd_tables(const orc_job *J, double *shear_G, double *bbB, double *bp
 /* layout of the macro vector, shared with oracle/oracle.py */
 void orc_job_macro(const orc_job *J, double *o)
 {
-    memset(o, 0, 64 * sizeof(double));
+    memset(o, 0, 72 * sizeof(double));
     o[0] = J->TimeTot; o[1] = J->e_vm; o[2] = J->evmp; o[3] = J->d_vm; o[4] = J->dvmp;
     o[5] = J->s_vm; o[6] = J->s_vm1; o[7] = J->Err_e; o[8] = J->Err_s; o[9] = J->Wgt_ph1;
     memcpy(o + 10, J->SigAvg, 9 * sizeof(double));
@@ -1894,6 +1929,7 @@ void orc_job_macro(const orc_job *J, double *o)
     memcpy(o + 46, J->EdotAvg, 9 * sizeof(double));
     o[55] = J->green_time; o[56] = J->green_iters; o[57] = J->newton_iters;
     o[58] = J->istep; o[59] = J->status;
+    memcpy(o + 60, J->dd_avg, 7 * sizeof(double));
 }
 
 void orc_job_setup_info(const orc_job *J, double c066[36], double s066[36], double c0[81],

@@ -116,7 +116,8 @@ void orc_job_dims(const orc_job *job, int dims[3]);
 /* field access: 0 Sig, 1 Eps, 2 Edot, 3 DisGrad, 4 VelGrad, 5 TranMat (all N^3 x 9),
  * 6 crss (N^3 x 12 x 2), 7 gamdot (N^3 x 12), 8 gamacum (N^3); dislocation-
  * density jobs: 9 rho_s, 10 rho_m, 11 rho_g1, 12 rho_g2, 13 rho_g3, 14 rho_F,
- * 15 rho_P (N^3 x 12 each), 16 gradient of gamdot (N^3 x 12 x 3)                  */
+ * 15 rho_P (N^3 x 12 each), 16 gradient of gamdot (N^3 x 12 x 3), 17 rho_dipole,
+ * 18 rho_forest (N^3 x 12)                                                         */
 double *orc_job_field(orc_job *job, int which);
 /* Gradient_ShearRate of the job's current gamdot field into field 16 */
 int orc_job_dd_gradient(orc_job *job);
@@ -132,7 +133,7 @@ int *orc_job_phase(orc_job *job);
 /* Newton iterations of each voxel in the last Green iteration */
 int *orc_job_newton_counts(orc_job *job);
 /* macro scalars: see orc_job_macro() in evp_oracle.c for the layout */
-void orc_job_macro(const orc_job *job, double *out /* 64 doubles */);
+void orc_job_macro(const orc_job *job, double *out /* 72 doubles */);
 double orc_job_newton_iters(const orc_job *job); /* total Newton iterations so far */
 /* setup products for tests */
 void orc_job_setup_info(const orc_job *job, double c066[36], double s066[36],

@@ -21,7 +21,7 @@ IP = C.POINTER(C.c_int)
 # fields of orc_job_field()
 SIG, EPS, EDOT, DISGRAD, VELGRAD, TRANMAT, CRSS, GAMDOT, GAMACUM = range(9)
 # dislocation-density jobs
-RHO_S, RHO_M, RHO_G1, RHO_G2, RHO_G3, RHO_F, RHO_P, GADOT_GRAD = range(9, 17)
+RHO_S, RHO_M, RHO_G1, RHO_G2, RHO_G3, RHO_F, RHO_P, GADOT_GRAD, RHO_DIPOLE, RHO_FOREST = range(9, 19)
 LAW_POWER, LAW_DD = 0, 1
 
 # layout of orc_job_macro()
@@ -188,9 +188,11 @@ class Job:
         return np.ctypeslib.as_array(lib().orc_job_phase(self.h), shape=(self.n,)).copy()
 
     def macro(self) -> dict:
-        o = f64(64)
+        o = f64(72)
         lib().orc_job_macro(self.h, ptr(o))
         d = {k: float(o[v]) for k, v in MACRO.items()}
+        # AvgDislocation of a dislocation-density job (src/constitutive.c:1669-1780)
+        d["dd_avg"] = o[60:67].copy()
         d["SigAvg"] = o[10:19].reshape(3, 3).copy()
         d["SigAvg1"] = o[19:28].reshape(3, 3).copy()
         d["DisGradAvg"] = o[28:37].reshape(3, 3).copy()

@@ -42,7 +42,8 @@ def record(inp, nsteps, out, law=O.LAW_POWER):
     extra = {}
     if law == O.LAW_DD:
         extra = dict(rho_s=J.field(O.RHO_S)[sel], rho_m=J.field(O.RHO_M)[sel],
-                     rho_F=J.field(O.RHO_F)[sel], rho_P=J.field(O.RHO_P)[sel])
+                     rho_F=J.field(O.RHO_F)[sel], rho_P=J.field(O.RHO_P)[sel],
+                     dd_avg=J.macro()["dd_avg"])
     np.savez_compressed(
         out, iters=np.array(iters), sig_avg=np.array(sig_avg), vm=np.array(vm), sel=sel,
         sig=J.field(O.SIG)[sel], edot=J.field(O.EDOT)[sel], disgrad=J.field(O.DISGRAD)[sel],

@@ -41,6 +41,8 @@ def _compare(J, g, tol=1e-10, dens_tol=1e-10):
             assert rel(d[k], ref) < dens_tol, (k, rel(d[k], ref))
     mo, mg = J.macro(), g.macro()
     assert rel(mg["sig_avg"], mo["SigAvg"]) < tol
+    # AvgDislocation (the dd.out columns), dipole and forest sums included
+    np.testing.assert_allclose(g.dd_averages(), mo["dd_avg"], rtol=dens_tol, atol=0)
 
 
 def test_dd_loop_parity_8(tmp_path):
@@ -181,6 +183,13 @@ def test_dd_reference_test_run(tmp_path):
     vals = np.array([[float(x) for x in ln.split("\t")] for ln in rows[1:]])
     # vm.out carries %e (7 significant digits)
     assert np.allclose(vals, g["vm"], rtol=1e-6, atol=0)
+    # dd.out (src/evolution.c:1565-1570): TimeTot, e_vm, AvgDislocation, one row
+    # per increment, no header
+    dd = np.array([[float(x) for x in ln.split("\t")] for ln in
+                   open(tmp_path / "dd.out").read().splitlines()])
+    assert dd.shape == (2, 9)
+    np.testing.assert_allclose(dd[:, :2], g["vm"][:, :2], rtol=1e-6, atol=0)
+    np.testing.assert_allclose(dd[-1, 2:], g["dd_avg"], rtol=1e-6, atol=0)
 
 
 def test_dd_loop_parity_simple_shear(tmp_path):

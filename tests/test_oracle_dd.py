@@ -246,6 +246,15 @@ def test_dd_job_evolves_densities(tmp_path):
     # GND rates start once the gradient is formed (Hard_Flag, istep > 1)
     assert np.abs(J.field(O.RHO_G1)).max() > 0
     np.testing.assert_array_equal(J.field(O.RHO_G3), 0.0)
+    # dipole and forest sums grow under slip; AvgDislocation is the voxel mean
+    # of the system sums (no pores here)
+    assert J.field(O.RHO_DIPOLE).min() >= 0 and J.field(O.RHO_FOREST).max() > 0
+    avg = J.macro()["dd_avg"]
+    g = np.sqrt(J.field(O.RHO_G1) ** 2 + J.field(O.RHO_G2) ** 2 + J.field(O.RHO_G3) ** 2)
+    want = [J.field(f).sum(axis=1).mean() for f in (O.RHO_F, O.RHO_P, O.RHO_S, O.RHO_M)]
+    want += [g.sum(axis=1).mean(), J.field(O.RHO_DIPOLE).sum(axis=1).mean(),
+             J.field(O.RHO_FOREST).sum(axis=1).mean()]
+    np.testing.assert_allclose(avg, want, rtol=1e-13)
 
 
 def test_reference_test_run_slip_is_frozen(tmp_path):
