@@ -1,7 +1,9 @@
-/* evp_job.c -- SetupJob / Evolution / DestroyJob of the drop-in (power-law
- * build) on top of the evpgpu device loop.
+/* evp_job.c -- SetupJob / Evolution / DestroyJob of the drop-in on top of
+ * the evpgpu device loop: the power law of the configuration files, or
+ * (evp_set_law) the dislocation-density law the reference's build compiles
+ * in, with its copper constants, densities and dd.out / drx.out.
  *
- * Setup follows src/init.c:727-1205 for the non-DD path:
+ * Setup follows src/init.c:727-1205:
  *   k-space            computed on the fly by the Green kernel (not stored)
  *   phases             ElastStiffnessMatrix + PlasticInit (src/init.c:815-862)
  *   microstructure     InitMicroStruct (src/init.c:233-396); C_gr, its
@@ -540,7 +542,7 @@ static evp_job *G = NULL;
 static char G_input[1024];
 static int G_abort = 1, G_status = 0, G_precision = EVPGPU_FP64;
 static evp_namelist G_nl;
-static FILE *fp_vm = NULL, *fp_err = NULL, *fp_dd = NULL;
+static FILE *fp_vm = NULL, *fp_err = NULL, *fp_dd = NULL, *fp_drx = NULL;
 
 void evp_set_abort_on_error(int on) { G_abort = on; }
 int evp_status(void) { return G_status; }
@@ -570,8 +572,12 @@ void PrintNameList(void) { evp_print_namelist(&G_nl); }
 void OpenFiles(void)
 {
     fp_vm = fopen("vm.out", "w+");
-    /* the dislocation-density build's density averages (src/io.c:220) */
-    if (G_law == EVP_LAW_DD) fp_dd = fopen("dd.out", "w+");
+    /* the dislocation-density build's density averages and its DRX log
+     * (src/io.c:220-223; no DRX event before e_vm > 4 leaves the header) */
+    if (G_law == EVP_LAW_DD) {
+        fp_dd = fopen("dd.out", "w+");
+        fp_drx = fopen("drx.out", "w+");
+    }
     fp_err = fopen("err.out", "w+");
 }
 
@@ -580,7 +586,8 @@ void CloseFiles(void)
     if (fp_vm) fclose(fp_vm);
     if (fp_err) fclose(fp_err);
     if (fp_dd) fclose(fp_dd);
-    fp_vm = fp_err = fp_dd = NULL;
+    if (fp_drx) fclose(fp_drx);
+    fp_vm = fp_err = fp_dd = fp_drx = NULL;
 }
 
 void SetupJob(void)
@@ -594,6 +601,9 @@ void SetupJob(void)
         return;
     }
     if (fp_vm) fprintf(fp_vm, "TIME\tEVM\tEVMP\tDVM\tDVMP\tSVM\tSVMSVM_1\n");
+    /* src/init.c:498-502: the dd.out header names 7 of its 9 columns */
+    if (fp_dd) fprintf(fp_dd, "TIME\tEVM\tDD_F\tDD_P\tDD_s\tDD_m\tDD_g\n");
+    if (fp_drx) fprintf(fp_drx, "TIME\tEVM\tNEW_IDs\n");
     if (fp_err) fprintf(fp_err, "IT\tERRE\tERRS\tSVM\n");
     printf("The initial microstructure contains %d grains.\n", evp_job_grain_list(G, NULL));
     printf("\n-----------Job set up-----------------\n");

@@ -183,10 +183,13 @@ def test_dd_reference_test_run(tmp_path):
     vals = np.array([[float(x) for x in ln.split("\t")] for ln in rows[1:]])
     # vm.out carries %e (7 significant digits)
     assert np.allclose(vals, g["vm"], rtol=1e-6, atol=0)
-    # dd.out (src/evolution.c:1565-1570): TimeTot, e_vm, AvgDislocation, one row
-    # per increment, no header
-    dd = np.array([[float(x) for x in ln.split("\t")] for ln in
-                   open(tmp_path / "dd.out").read().splitlines()])
+    # dd.out (src/init.c:498, src/evolution.c:1565-1570): the header, then
+    # TimeTot, e_vm and AvgDislocation, one row per increment; drx.out keeps
+    # its header only (no DRX event)
+    lines = open(tmp_path / "dd.out").read().splitlines()
+    assert lines[0] == "TIME\tEVM\tDD_F\tDD_P\tDD_s\tDD_m\tDD_g"
+    dd = np.array([[float(x) for x in ln.split("\t")] for ln in lines[1:]])
+    assert open(tmp_path / "drx.out").read() == "TIME\tEVM\tNEW_IDs\n"
     assert dd.shape == (2, 9)
     np.testing.assert_allclose(dd[:, :2], g["vm"][:, :2], rtol=1e-6, atol=0)
     np.testing.assert_allclose(dd[-1, 2:], g["dd_avg"], rtol=1e-6, atol=0)
