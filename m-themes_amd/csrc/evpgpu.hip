@@ -331,13 +331,13 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
         c->sig_alt = t;
     }
     tmark(c, 0, 1);
-    st = reduce_partials(c, c->partials, nblk, NRED, c->red);
+    // the 16 sums and the Newton status in one read-back
+    st = reduce_partials(c, c->partials, nblk, NRED, c->red, c->d_status);
     if (st) return st;
-    EVP_CHECK(hipMemcpyAsync(c->h_red, c->red, NRED * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    EVP_CHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EVP_CHECK(hipMemcpyAsync(c->h_red, c->red, (NRED + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EVP_CHECK(hipStreamSynchronize(c->stream));
     tcollect(c);
-    if (*c->h_status) return *c->h_status;
+    if (c->h_red[NRED] != 0.0) return (int)c->h_red[NRED];
     c->m.err_e = c->h_red[1] * scale;
     c->m.err_s = c->h_red[0] * scale;
     *newton_iters += c->h_red[2];
@@ -431,10 +431,10 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
     c->newton_shortcut = 1;
     c->fused = 1;
     if (!st) st = dalloc(&c->partials, nblk * NRED);
-    if (!st) st = dalloc(&c->red, NRED);
+    if (!st) st = dalloc(&c->red, NRED + 1);
     if (!st) st = dalloc(&c->red2, (long long)REDUCE_G * NRED);
     if (!st) st = dalloc(&c->d_status, 1);
-    if (!st && hipHostMalloc((void **)&c->h_red, NRED * sizeof(double), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
+    if (!st && hipHostMalloc((void **)&c->h_red, (NRED + 1) * sizeof(double), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
     if (!st && hipHostMalloc((void **)&c->h_status, sizeof(int), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
     if (!st) st = fft_twiddles(c);
     if (!st) {

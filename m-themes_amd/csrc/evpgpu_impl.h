@@ -230,7 +230,10 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
                   double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status,
                   double *err_vox, int *niter_vox, int fast);
-int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out);
+// column sums of partials[nblk][nval] into out[0..nval); with status, also
+// out[nval] = *status (out must then hold nval + 1 values)
+int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out,
+                    const int *status = nullptr);
 // fields.hip
 int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *macro_sym6);
 int antisym_add_launch(evpgpu_ctx *c, const double *dW3);

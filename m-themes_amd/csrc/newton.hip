@@ -674,9 +674,13 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
 // Stage 2: one workgroup sums the stage-1 rows the same way.
 constexpr int RED_MAXV = 16;
 
+// status (stage 2 only, may be null): the Newton status word, appended as
+// out[nval] so that one copy brings back the sums and the status
 __global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ partials, int nblk,
-                                                     int nval, int per, double *__restrict__ out)
+                                                     int nval, int per, double *__restrict__ out,
+                                                     const int *__restrict__ status)
 {
+    if (status && blockIdx.x == 0 && threadIdx.x == 0) out[nval] = (double)*status;
     __shared__ double sh[256];
     const int v = threadIdx.x % RED_MAXV, r0 = threadIdx.x / RED_MAXV;
     const long long b0 = (long long)blockIdx.x * per;
@@ -758,14 +762,15 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
 }
 
 // out[nval] = column sums of partials[nblk][nval]; uses c->red2 (REDUCE_G rows)
-int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out)
+int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out, const int *status)
 {
     if (nval > RED_MAXV) return EVPGPU_ERR_ARG;
     int g = nblk < REDUCE_G ? nblk : REDUCE_G;
     int per = (nblk + g - 1) / g;
     g = (nblk + per - 1) / per;
-    hipLaunchKernelGGL(k_reduce_rows, dim3(g), dim3(256), 0, c->stream, partials, nblk, nval, per, c->red2);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(g), dim3(256), 0, c->stream, partials, nblk, nval, per, c->red2,
+                       (const int *)nullptr);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->stream, (const double *)c->red2, g, nval, g,
-                       out);
+                       out, status);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
