@@ -105,6 +105,19 @@ __device__ __forceinline__ void v6_to_sym(const double v[6], double t[6])
     t[5] = v[4] * r2;
 }
 
+// 1/d for a positive normal d: the hardware estimate refined by two Newton
+// steps y += y (1 - d y) (the relative error e becomes e^2 per step)
+__device__ __forceinline__ double rcp_refined(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const double e = fma(-d, y, 1.0);
+        y = fma(y, e, y);
+    }
+    return y;
+}
+
 constexpr int NB_THREADS = 64; // one wave per block: the LDS slab is wave-private
 
 struct RateIn {
@@ -165,7 +178,7 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
                     // sinh and cosh from one expm1: with m = e^x - 1 and
                     // q = e^-x, sinh x = m (1 + q) / 2 (no cancellation at
                     // small x) and cosh x = (1 + m + q) / 2
-                    const double m = expm1(x), q = 1.0 / (1.0 + m);
+                    const double m = expm1(x), q = rcp_refined(1.0 + m);
                     r2 = g0k * (0.5 * m * (1.0 + q)) * (double)(2 * (r > 0) - 1);
                     if (DD) r1 = g0k * (0.5 * ((1.0 + m) + q)) * itc;
                 }
