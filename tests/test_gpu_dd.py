@@ -209,3 +209,20 @@ def test_dd_loop_parity_simple_shear(tmp_path):
     assert list(it_g) == list(it_o), (it_g, it_o)
     _compare(J, g)
     g.close()
+
+
+def test_dd_state_calls_refused_where_they_do_not_apply(tmp_path):
+    """The reverse pass is of the power law: a DD context refuses it; the
+    density calls refuse a power-law context (EVPGPU_ERR_STATE, no crash)."""
+    inp = _job(tmp_path, "dd8r", n=8, grains=4, seed=21, increments=2)
+    g = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    with pytest.raises(lib.EvpError) as e:
+        g.adjoint_record(2, 1)
+    assert e.value.code == -5
+    g.close()
+    p = lib.GpuJob(inp, lib.FP64)
+    for call in (p.dd_averages, p.dd_state):
+        with pytest.raises(lib.EvpError) as e:
+            call()
+        assert e.value.code == -5
+    p.close()
