@@ -226,3 +226,21 @@ def test_dd_state_calls_refused_where_they_do_not_apply(tmp_path):
             call()
         assert e.value.code == -5
     p.close()
+
+
+def test_dd_loop_parity_non_cubic_any_size(tmp_path):
+    """12 x 10 x 9 with pores: the shear-rate gradient's neighbour indexing
+    on a grid whose three lengths differ (the reference's grain check on the
+    z-minus side reads the -y neighbour, src/constitutive.c:818,838,858) and
+    the two-level DFT transforms of lengths 10 and 9; three increments, so
+    the GND densities move."""
+    inp = _job(tmp_path, "ddany", n=0, dims=(12, 10, 9), grains=6, seed=61, increments=3,
+               pore_fraction=0.02, pore_radius=1)
+    J = O.Job(inp, O.LAW_DD)
+    g = lib.GpuJob(inp, lib.FP64, law=lib.LAW_DD)
+    it_o = J.run(3)
+    it_g, _ = g.run(3)
+    assert list(it_g) == list(it_o), (it_g, it_o)
+    assert np.abs(J.field(O.RHO_G1)).max() > 0
+    _compare(J, g)
+    g.close()
