@@ -89,6 +89,16 @@ def run_gpu(inp, precision, steps, warmup, law=lib.LAW_POWER):
     return out
 
 
+def convergence(r, cfg):
+    """Final Err_s / Err_e of every timed increment and whether the loop ended on
+    IterMax instead of the tolerance (src/evolution.c:1354)."""
+    es = [s["err_s"] for s in r["stats"]]
+    ee = [s["err_e"] for s in r["stats"]]
+    hit = [max(abs(a), abs(b)) > cfg.err for a, b in zip(es, ee)]
+    return {"err_s": es, "err_e": ee, "tol": cfg.err, "iter_max": cfg.iter_max,
+            "hit_iter_max": hit, "converged_increments": int(len(hit) - sum(hit))}
+
+
 def run_forced(inp, precision, iters, warmup):
     """The same increments forced to given iteration counts (equal-count comparison)."""
     job = lib.GpuJob(inp, precision)
@@ -138,20 +148,50 @@ def run_adjoint(n, increments, ck):
                 grad=dict(zip(lib.PARAM_NAMES, [float(x) for x in g[0]])))
 
 
-def cpu_baseline(inp, n_iter):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(inp, n_iter_1, n_iter_all):
     """The oracle (C restatement, OpenMP over voxels) on a bounded sample of the
-    same workload: the first `n_iter` Green iterations of increment 1."""
+    same workload: the first Green iterations of increment 1, once on one
+    thread (`value`) and once on every host thread the process may use."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    J = O.Job(inp)
-    J.run(1, forced=[n_iter])
-    m = J.macro()
-    n = int(np.prod(J.dims))
-    return dict(value=n * n_iter / m["green_time"], unit="voxel-iters/s", cores=O.threads(),
-                kind="port",
-                sample=f"128^3 job, increment 1, first {n_iter} Green iterations "
-                       f"({m['green_time']:.1f} s)")
+    n_all = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    n_all = min(n_all, int(os.environ.get("OMP_NUM_THREADS", n_all)))
+    out = {}
+    for threads, n_iter in ((1, n_iter_1), (n_all, n_iter_all)):
+        if n_iter <= 0:
+            continue
+        O.set_threads(threads)
+        J = O.Job(inp)
+        J.run(1, forced=[n_iter])
+        m = J.macro()
+        n = int(np.prod(J.dims))
+        out[threads] = (n * n_iter / m["green_time"], n_iter, m["green_time"])
+        del J
+    O.set_threads(n_all)
+    one = out.get(1)
+    allc = out.get(n_all)
+    base = one if one is not None else allc
+    res = dict(value=base[0], unit="voxel-iters/s", cores=1 if one is not None else n_all, kind="port",
+               cpu_model=cpu_model(),
+               sample=f"128^3 job, increment 1, first {base[1]} Green iterations on "
+                      f"{1 if one is not None else n_all} thread(s) ({base[2]:.1f} s)")
+    if allc is not None and one is not None:
+        res["all_cores"] = dict(value=allc[0], cores=n_all,
+                                sample=f"first {allc[1]} Green iterations ({allc[2]:.1f} s)",
+                                speedup_vs_1=allc[0] / one[0])
+    return res
 
 
 def main():
@@ -162,8 +202,10 @@ def main():
     ap.add_argument("--n", type=int, default=128,
                     help="grid edge (default: the 128^3 job; 256: the 256^3 job of 4096 grains)")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
-    ap.add_argument("--cpu-iters", type=int, default=3,
-                    help="Green iterations of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-iters", type=int, default=1,
+                    help="Green iterations of the one-thread CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-iters-all", type=int, default=3,
+                    help="Green iterations of the all-threads CPU baseline sample (0: skip)")
     ap.add_argument("--adj-n", type=int, default=64, help="grid edge of the adjoint workload")
     ap.add_argument("--adj-increments", type=int, default=20, help="0: skip the adjoint")
     ap.add_argument("--adj-checkpoint", type=int, default=5)
@@ -261,11 +303,11 @@ def main():
                trials_per_voxel_launch=ntr / max(n_cryst * launches, 1),
                flops_per_launch=flops / max(launches, 1), bytes_per_launch=bytes_ / max(launches, 1))
     if tf / PEAK_FP64_TFS >= gbs / PEAK_HBM_GBS:
-        roof = dict(bound="mfma", achieved=tf, peak=PEAK_FP64_TFS, unit="TFLOP/s",
+        roof = dict(bound="fp64-valu", achieved=tf, peak=PEAK_FP64_TFS, unit="TFLOP/s",
                     frac=tf / PEAK_FP64_TFS, traffic=traffic, kernel="k_newton",
                     avg_launch_ms=ms / max(launches, 1), hbm_gbs=gbs, **per,
-                    note="FP64 vector ALU bound; peak is the MI355X FP64 figure (78.6 TFLOP/s), "
-                         "no MFMA instructions are used")
+                    note="bound by the FP64 vector ALU (no MFMA instruction is used); peak is "
+                         "the MI355X FP64 vector figure, 78.6 TFLOP/s")
     else:
         roof = dict(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s",
                     frac=gbs / PEAK_HBM_GBS, traffic=traffic, kernel="k_newton",
@@ -296,12 +338,14 @@ def main():
                    "grid": cfg.n, "grains": cfg.grains, "pores": cfg.pore_fraction,
                    "precision": main_key, "green_tol": cfg.err, "seed": hex(cfg.seed)},
         "green_iters": R["iters"],
+        "convergence": convergence(R, cfg),
         "kernels_ms": kern,
         "roofline": roof,
     }
     if "fp64" in res:
         out["fp64"] = {"value": res["fp64"]["vips"], "ms_per_step": 1e3 * res["fp64"]["wall"] / args.steps,
-                       "green_iters": res["fp64"]["iters"]}
+                       "green_iters": res["fp64"]["iters"],
+                       "convergence": convergence(res["fp64"], cfg)}
     if "fp64" in res and "mixed" in res:
         a, b = res["fp64"]["sig"], res["mixed"]["sig"]
         log("bench: mixed forced to the FP64 iteration counts (precision error alone)")
@@ -347,9 +391,11 @@ def main():
         log(f"bench: adjoint: {args.adj_n}^3, {args.adj_increments} increments")
         out["adjoint"] = run_adjoint(args.adj_n, args.adj_increments, args.adj_checkpoint)
         log(f"bench: adjoint/forward = {out['adjoint']['ratio']:.3f}")
-    if args.cpu_iters > 0:
-        log(f"bench: cpu baseline: oracle, {args.cpu_iters} Green iterations")
-        out["cpu_baseline"] = cpu_baseline(inp, args.cpu_iters)
+    if args.cpu_iters > 0 or args.cpu_iters_all > 0:
+        log(f"bench: cpu baseline: oracle, {args.cpu_iters} Green iteration(s) on one thread, "
+            f"{args.cpu_iters_all} on all")
+        out["cpu_baseline"] = cpu_baseline(inp, args.cpu_iters, args.cpu_iters_all)
+        log(f"bench: cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
 

@@ -38,7 +38,9 @@ enum {
     EVPGPU_ERR_SIZE = -3,         /* grid size not supported (each length 2..512)  */
     EVPGPU_ERR_NOMEM = -4,        /* device allocation failed                     */
     EVPGPU_ERR_STATE = -5,        /* call out of order                            */
-    EVPGPU_ERR_NODEV = -6         /* no HIP device                                */
+    EVPGPU_ERR_NODEV = -6,        /* no HIP device                                */
+    EVPGPU_ERR_NONFINITE = -7     /* non-finite stress/norm (the reference's
+                                     assert(!isnan), src/constitutive.c:617,653,700) */
 };
 
 enum { EVPGPU_FP64 = 0, EVPGPU_MIXED = 1 };
@@ -92,6 +94,9 @@ typedef struct {
     double ls_trials;              /* line-search trial evaluations (NR_quality calls) */
     double seconds;                /* Green-loop wall time */
     int stopped;                   /* 1 if the creep stop (e_vm > 0.3) ended the run */
+    double err_s, err_e;           /* Err_s, Err_e of the increment's last Green iteration
+                                      (src/evolution.c:1449-1455); the loop ended on IterMax
+                                      when max(err_s, err_e) > Err */
 } evpgpu_stats;
 
 /* Dislocation-density law of one crystal phase: the constitutive law of the

@@ -208,10 +208,11 @@ __global__ __launch_bounds__(DD_THREADS) void k_dd_evolve(
 // its crystal voxels and their systems of rho_F, rho_P, rho_s, rho_m,
 // |(rho_g1, rho_g2, rho_g3)|, rho_dipole and rho_forest (fixed-order tree)
 constexpr int NAVG = 7;
+struct DdAvgPhases {
+    int is_gas[2], nsys[2];
+};
 __global__ __launch_bounds__(DD_THREADS) void k_dd_avg(long long N, const int *__restrict__ phase,
-                                                       const int *__restrict__ is_gas2,
-                                                       const int *__restrict__ nsys2,
-                                                       const double *__restrict__ rho,
+                                                       DdAvgPhases ph, const double *__restrict__ rho,
                                                        double *__restrict__ partials)
 {
     __shared__ double sh[NAVG][DD_THREADS];
@@ -219,9 +220,9 @@ __global__ __launch_bounds__(DD_THREADS) void k_dd_avg(long long N, const int *_
     double a[NAVG] = {0, 0, 0, 0, 0, 0, 0};
     if (v < N) {
         const int p = phase[v] - 1;
-        if (p >= 0 && p <= 1 && !is_gas2[p]) {
+        if (p >= 0 && p <= 1 && !ph.is_gas[p]) {
             const long long S = NS * N;
-            for (int i = 0; i < nsys2[p]; i++) {
+            for (int i = 0; i < ph.nsys[p]; i++) {
                 const long long iv = (long long)i * N + v;
                 const double g1 = rho[2 * S + iv], g2 = rho[3 * S + iv], g3 = rho[4 * S + iv];
                 a[0] += rho[5 * S + iv];
@@ -358,27 +359,21 @@ int evpgpu_dd_averages(evpgpu_ctx *c, double avg[7])
 {
     if (!c || !avg) return EVPGPU_ERR_ARG;
     if (!c->dd_on) return EVPGPU_ERR_STATE;
-    int h[4];
+    DdAvgPhases h;
     for (int p = 0; p < 2; p++) {
-        h[p] = p < c->nphases ? c->ph[p].is_gas : 1;
-        h[2 + p] = p < c->nphases ? c->ph[p].nsys : 0;
+        h.is_gas[p] = p < c->nphases ? c->ph[p].is_gas : 1;
+        h.nsys[p] = p < c->nphases ? c->ph[p].nsys : 0;
     }
-    int *dh = nullptr;
-    if (hipMalloc((void **)&dh, sizeof(h)) != hipSuccess) return EVPGPU_ERR_NOMEM;
     int st = EVPGPU_OK;
     const int nb = (int)((c->N + DD_THREADS - 1) / DD_THREADS);
-    if (hipMemcpy(dh, h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) st = EVPGPU_ERR_HIP;
-    if (!st) {
-        hipLaunchKernelGGL(k_dd_avg, dim3(nb), dim3(DD_THREADS), 0, c->stream, c->N, c->phase, dh, dh + 2,
-                           c->rho, c->partials);
-        if (hipGetLastError() != hipSuccess) st = EVPGPU_ERR_HIP;
-    }
+    hipLaunchKernelGGL(k_dd_avg, dim3(nb), dim3(DD_THREADS), 0, c->stream, c->N, c->phase, h, c->rho,
+                       c->partials);
+    if (hipGetLastError() != hipSuccess) st = EVPGPU_ERR_HIP;
     if (!st) st = reduce_partials(c, c->partials, nb, NAVG, c->red);
     double s[NAVG];
     if (!st && hipMemcpyAsync(s, c->red, sizeof(s), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         st = EVPGPU_ERR_HIP;
     if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = EVPGPU_ERR_HIP;
-    (void)hipFree(dh);
     if (st) return st;
     for (int k = 0; k < NAVG; k++) avg[k] = s[k] * (1.0 / (double)c->N);
     return EVPGPU_OK;

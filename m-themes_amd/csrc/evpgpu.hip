@@ -338,6 +338,11 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
     EVP_CHECK(hipStreamSynchronize(c->stream));
     tcollect(c);
     if (c->h_red[NRED] != 0.0) return (int)c->h_red[NRED];
+    // a non-finite rate (|x|^n overflowing) makes the voxel's error norm and
+    // stress non-finite, and with them these block sums; the reference stops
+    // on such values (assert(!isnan), src/constitutive.c:617,653,700)
+    for (int k = 0; k < NRED; k++)
+        if (!isfinite(c->h_red[k])) return EVPGPU_ERR_NONFINITE;
     c->m.err_e = c->h_red[1] * scale;
     c->m.err_s = c->h_red[0] * scale;
     *newton_iters += c->h_red[2];
@@ -377,10 +382,11 @@ const char *evpgpu_strerror(int s)
     case EVPGPU_ERR_LINESEARCH: return "roundoff problem during line searching in NR (1203)";
     case EVPGPU_ERR_ARG: return "invalid argument";
     case EVPGPU_ERR_HIP: return "HIP runtime error";
-    case EVPGPU_ERR_SIZE: return "unsupported grid size (each dimension a power of 2 in [8, 256])";
+    case EVPGPU_ERR_SIZE: return "unsupported grid size (each length 2..512)";
     case EVPGPU_ERR_NOMEM: return "device allocation failed";
     case EVPGPU_ERR_STATE: return "call out of order (phases, loading, microstructure, state)";
     case EVPGPU_ERR_NODEV: return "no HIP device";
+    case EVPGPU_ERR_NONFINITE: return "non-finite stress or error norm in the stress update";
     }
     return "unknown status";
 }
@@ -738,6 +744,9 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     // update_schmid (src/evolution.c:1334-1336) happens inside the Newton
     // kernel, from the orientations the previous epilogue left
     st = fill_zero(c, (double *)c->acc, 6 * c->NK);
+    // a failed Newton launch of an earlier increment leaves its status word
+    // set; every increment starts from a clear one
+    if (!st) st = fill_zero_int(c, c->d_status, 1);
     if (st) return st;
     tmark(c, 4, 1);
     EVP_CHECK(hipStreamSynchronize(c->stream));
@@ -797,7 +806,9 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
             if (st) return st;
         }
     } else if (mixed) {
-        while (more()) {
+        // the FP64 sweep that closes the mixed loop counts against IterMax:
+        // the mixed iterations stop one short of it
+        while (more() && iter < c->ld.iter_max - 1) {
             st = mixed_step();
             if (st) return st;
         }
@@ -836,6 +847,8 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     }
 
     S.seconds = now_s() - t0;
+    S.err_s = m.err_s;
+    S.err_e = m.err_e;
     S.iters = iter;
     S.fp64_iters = it64;
     S.newton_iters = nit;

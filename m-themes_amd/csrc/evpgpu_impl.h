@@ -239,6 +239,7 @@ int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *m
 int antisym_add_launch(evpgpu_ctx *c, const double *dW3);
 int epilogue_launch(evpgpu_ctx *c, const EpilogueParams &P, double *partials, int *nblk);
 int fill_zero(evpgpu_ctx *c, double *p, long long n);
+int fill_zero_int(evpgpu_ctx *c, int *p, long long n);
 int to_f32_launch(evpgpu_ctx *c, const double *src, float *dst, long long n);
 int sub_launch(evpgpu_ctx *c, double *a, const double *b, long long n);
 int copy_launch(evpgpu_ctx *c, const double *src, double *dst, long long n);

@@ -20,6 +20,12 @@ __global__ void k_fill(double *p, long long n)
     if (i < n) p[i] = 0.0;
 }
 
+__global__ void k_fill_int(int *p, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 __global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, long long n)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,6 +233,12 @@ __global__ __launch_bounds__(256) void k_epilogue(
 int fill_zero(evpgpu_ctx *c, double *p, long long n)
 {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p, n);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+int fill_zero_int(evpgpu_ctx *c, int *p, long long n)
+{
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p, n);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 

@@ -70,7 +70,8 @@ class Macro(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("iters", C.c_int), ("fp64_iters", C.c_int), ("newton_iters", C.c_double),
-                ("ls_trials", C.c_double), ("seconds", C.c_double), ("stopped", C.c_int)]
+                ("ls_trials", C.c_double), ("seconds", C.c_double), ("stopped", C.c_int),
+                ("err_s", C.c_double), ("err_e", C.c_double)]
 
 
 class AdjointStats(C.Structure):
@@ -235,7 +236,8 @@ class GpuJob:
         stats = (Stats * n)()
         _chk(evp().evp_job_run(self.h, n, forced, _p(it, IP), stats), "evp_job_run")
         return it, [dict(iters=s.iters, fp64_iters=s.fp64_iters, newton_iters=s.newton_iters,
-                         ls_trials=s.ls_trials, seconds=s.seconds, stopped=s.stopped)
+                         ls_trials=s.ls_trials, seconds=s.seconds, stopped=s.stopped,
+                         err_s=s.err_s, err_e=s.err_e)
                     for s in stats]
 
     def synchronize(self):

@@ -1859,6 +1859,15 @@ int orc_threads(void)
     return 1;
 #endif
 }
+/* threads of the OpenMP loops over voxels (results do not depend on it) */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 void orc_job_dims(const orc_job *J, int d[3]) { memcpy(d, J->CellDim, sizeof(J->CellDim)); }
 int *orc_job_grain(orc_job *J) { return J->grain; }
 int *orc_job_phase(orc_job *J) { return J->phase; }

@@ -111,7 +111,8 @@ void orc_job_free(orc_job *job);
  * forced_iters (may be NULL) forces the count of each increment (parity protocol). */
 int orc_job_run(orc_job *job, int n, int *iters, const int *forced_iters);
 int orc_job_nvox(const orc_job *job);
-int orc_threads(void); /* OpenMP threads the oracle uses */
+int orc_threads(void);
+void orc_set_threads(int n); /* OpenMP threads the oracle uses */
 void orc_job_dims(const orc_job *job, int dims[3]);
 /* field access: 0 Sig, 1 Eps, 2 Edot, 3 DisGrad, 4 VelGrad, 5 TranMat (all N^3 x 9),
  * 6 crss (N^3 x 12 x 2), 7 gamdot (N^3 x 12), 8 gamacum (N^3); dislocation-

@@ -46,6 +46,7 @@ def lib():
         L.orc_job_run.argtypes = [C.c_void_p, C.c_int, IP, IP]
         L.orc_job_nvox.argtypes = [C.c_void_p]
         L.orc_threads.argtypes = []
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_job_dims.argtypes = [C.c_void_p, IP]
         L.orc_job_field.restype = DP
         L.orc_job_field.argtypes = [C.c_void_p, C.c_int]
@@ -338,6 +339,11 @@ def set_newton_tol(tol: float = 1e-10, tolx: float = 1e-7):
 
 def threads() -> int:
     return lib().orc_threads()
+
+
+def set_threads(n: int):
+    """Threads of the oracle's OpenMP loops (its results do not depend on it)."""
+    lib().orc_set_threads(int(n))
 
 
 class DDPhase(C.Structure):

@@ -1,10 +1,11 @@
 // adjoint_check.cpp -- host build of the per-voxel adjoint functions
-// (csrc/adjoint_voxel.h, the same source the device kernels compile) behind
-// plain C entry points, so that each transposed linearisation can be checked
-// against finite differences of its forward function on the CPU.
+// (m-themes_amd/csrc/adjoint_voxel.h, the same source the device kernels
+// compile) behind plain C entry points, so that each transposed
+// linearisation can be checked against finite differences of its forward
+// function on the CPU.  Test infrastructure, not part of the libraries.
 #include <string.h>
 
-#include "../adjoint_voxel.h"
+#include "adjoint_voxel.h"
 
 using namespace adj;
 

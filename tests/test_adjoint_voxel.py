@@ -1,4 +1,5 @@
-"""Per-voxel pieces of the adjoint (m-themes_amd/csrc/adjoint_voxel.h, host build).
+"""Per-voxel pieces of the adjoint (m-themes_amd/csrc/adjoint_voxel.h, host build
+in tests/adjcheck).
 
 Each transposed linearisation is checked with the dot-product identity
 <ybar, J dx> = <J^T ybar, dx>, where J dx comes from central finite
@@ -13,16 +14,17 @@ import os
 import numpy as np
 import pytest
 
-from conftest import PKG
+import subprocess
 
 DP = C.POINTER(C.c_double)
-LIB = os.path.join(PKG, "lib", "libadjcheck.so")
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "adjcheck")
+LIB = os.path.join(HERE, "lib", "libadjcheck.so")
 
 
 @pytest.fixture(scope="module")
 def L():
     if not os.path.exists(LIB):
-        pytest.skip("libadjcheck.so not built")
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
     lib = C.CDLL(LIB)
     lib.adjc_voce.restype = C.c_double
     lib.adjc_voce.argtypes = [C.c_double] * 6
