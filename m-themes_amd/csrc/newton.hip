@@ -28,6 +28,7 @@
 // holds about 420 registers, so the code is shaped for latency (next Schmid
 // vector read ahead, constants hoisted, no scalar loops on the hot path).
 #include "evpgpu_impl.h"
+#include "linesearch.h"
 
 namespace {
 
@@ -429,7 +430,7 @@ __device__ int update_voxel(const NewtonParams &P, int p, RateIn &in, const Vox 
             double t = fabs(step[i]) / (d > 1.0 ? d : 1.0);
             if (t > test) test = t;
         }
-        double alamin = TOLX / test, alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0;
+        double alamin = TOLX / test, alam = 1.0, alam2 = 0.0, f2 = 0.0;
         // The first trial (alam = 1) is rejected when 1 < alamin, whatever
         // its merit value: the step is left undone and only gamdot (from the
         // trial point) is affected, which FAST launches do not form -- so
@@ -451,29 +452,11 @@ __device__ int update_voxel(const NewtonParams &P, int p, RateIn &in, const Vox 
                 break;
             } else if (f_new <= f_old + ALF * alam * slope) {
                 break;
-            } else {
-                if (fabs(alam - 1.0) < 1.E-4) {
-                    tmplam = -slope / (2.0 * (f_new - f_old - slope));
-                } else {
-                    double rhs1 = f_new - f_old - alam * slope;
-                    double rhs2 = f2 - f_old - alam2 * slope;
-                    double ac = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
-                    double bc = (-alam2 * rhs1 / (alam * alam) + alam * rhs2 / (alam2 * alam2)) /
-                                (alam - alam2);
-                    if (fabs(ac) < 1E-4) {
-                        tmplam = -slope / (2.0 * bc);
-                    } else {
-                        double disc = bc * bc - 3.0 * ac * slope;
-                        if (disc < 0.0) tmplam = 0.5 * alam;
-                        else if (bc <= 0.0) tmplam = (-bc + sqrt(disc)) / (3.0 * ac);
-                        else tmplam = -slope / (bc + sqrt(disc));
-                    }
-                    if (tmplam > 0.5 * alam) tmplam = 0.5 * alam;
-                }
             }
+            const double next = ls_backtrack(alam, alam2, f_new, f2, f_old, slope);
             alam2 = alam;
             f2 = f_new; // the backtrack value the cubic step needs (Q18)
-            alam = tmplam > 0.1 * alam ? tmplam : 0.1 * alam;
+            alam = next;
         }
         } // line search
         } // !skip

@@ -618,6 +618,78 @@ void orc_set_newton_tol(double tol, double tolx)
     g_tolx = tolx;
 }
 
+/* Optional record of the line search (tests of its branches): one row of
+ * ORC_LS_ROW doubles per trial point, see evp_oracle.h.  Not thread-safe:
+ * set it only around single-voxel calls.                                  */
+static double *g_ls_trace = NULL;
+static int g_ls_cap = 0, g_ls_n = 0;
+
+void orc_ls_trace(double *buf, int cap_rows)
+{
+    g_ls_trace = buf;
+    g_ls_cap = buf ? cap_rows : 0;
+    g_ls_n = 0;
+}
+
+int orc_ls_trace_count(void) { return g_ls_n; }
+
+static void ls_record(int iterl, double alam, double f_new, double f_old, double slope,
+                      double alamin, int outcome, double tmplam, const double old6[6],
+                      const double step[6], double test, double sum, int scaled)
+{
+    if (!g_ls_trace || g_ls_n >= g_ls_cap) return;
+    double *r = g_ls_trace + (size_t)g_ls_n * ORC_LS_ROW;
+    r[0] = iterl; r[1] = alam; r[2] = f_new; r[3] = f_old; r[4] = slope; r[5] = alamin;
+    r[6] = outcome; r[7] = tmplam;
+    for (int i = 0; i < 6; i++) {
+        r[8 + i] = old6[i];
+        r[14 + i] = step[i];
+    }
+    r[20] = test; r[21] = sum; r[22] = scaled; r[23] = 0.0;
+    g_ls_n++;
+}
+
+/* The backtracking step of the NR_MODIFIED line search
+ * (src/evolution.c:218-249) after a rejected trial at alam (merit f_new),
+ * the previous rejected one at alam2 (f2), f(0) = f_old, f'(0) = slope:
+ * returns tmplam (before the caller's 0.1*alam floor); *outcome is the
+ * branch taken (2..6, the trace codes).                                   */
+double orc_ls_next_lambda(double alam, double alam2, double f_new, double f2, double f_old,
+                          double slope, int *outcome)
+{
+    double tmplam;
+    int oc;
+    if (fabs(alam - 1.0) < 1.E-4) {
+        tmplam = -slope / (2.0 * (f_new - f_old - slope));
+        oc = 2;
+    } else {
+        double rhs1 = f_new - f_old - alam * slope;
+        double rhs2 = f2 - f_old - alam2 * slope;
+        double ac = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
+        double bc = (-alam2 * rhs1 / (alam * alam) + alam * rhs2 / (alam2 * alam2)) /
+                    (alam - alam2);
+        if (fabs(ac) < 1E-4) {
+            tmplam = -slope / (2.0 * bc);
+            oc = 3;
+        } else {
+            double disc = bc * bc - 3.0 * ac * slope;
+            if (disc < 0.0) {
+                tmplam = 0.5 * alam;
+                oc = 4;
+            } else if (bc <= 0.0) {
+                tmplam = (-bc + sqrt(disc)) / (3.0 * ac);
+                oc = 5;
+            } else {
+                tmplam = -slope / (bc + sqrt(disc));
+                oc = 6;
+            }
+        }
+        if (tmplam > 0.5 * alam) tmplam = 0.5 * alam;
+    }
+    if (outcome) *outcome = oc;
+    return tmplam;
+}
+
 /* UpdateStress for one voxel; dd (may be NULL) selects the dislocation-
  * density law and carries its phase constants and the voxel's densities. */
 static int update_stress(const double sg66[6][6], const double c066[6][6], double dt,
@@ -696,6 +768,7 @@ static int update_stress(const double sg66[6][6], const double c066[6][6], doubl
         double sum = 0.0, slope = 0.0, test = 0.0;
         for (int i = 0; i < 6; i++) sum += step[i] * step[i];
         sum = sqrt(sum);
+        const int scaled = sum > stpmax;
         if (sum > stpmax)
             for (int i = 0; i < 6; i++) step[i] *= stpmax / sum;
         for (int i = 0; i < 6; i++) slope += gradf[i] * step[i];
@@ -710,32 +783,19 @@ static int update_stress(const double sg66[6][6], const double c066[6][6], doubl
             double f_new = nr_quality(&c, sig6);
             if (c.status) return c.status;
             if (alam < alamin) {
+                ls_record(iterl, alam, f_new, f_old, slope, alamin, 1, 0.0, old6, step, test, sum,
+                          scaled);
                 for (int i = 0; i < 6; i++) sig6[i] = old6[i];
                 break;
             } else if (f_new <= f_old + ALF * alam * slope) {
+                ls_record(iterl, alam, f_new, f_old, slope, alamin, 0, 0.0, old6, step, test, sum,
+                          scaled);
                 break;
             } else {
-                if (fabs(alam - 1.0) < 1.E-4) {
-                    tmplam = -slope / (2.0 * (f_new - f_old - slope));
-                } else {
-                    double rhs1 = f_new - f_old - alam * slope;
-                    double rhs2 = f2 - f_old - alam2 * slope;
-                    double ac = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
-                    double bc = (-alam2 * rhs1 / (alam * alam) + alam * rhs2 / (alam2 * alam2)) /
-                                (alam - alam2);
-                    if (fabs(ac) < 1E-4) {
-                        tmplam = -slope / (2.0 * bc);
-                    } else {
-                        double disc = bc * bc - 3.0 * ac * slope;
-                        if (disc < 0.0)
-                            tmplam = 0.5 * alam;
-                        else if (bc <= 0.0)
-                            tmplam = (-bc + sqrt(disc)) / (3.0 * ac);
-                        else
-                            tmplam = -slope / (bc + sqrt(disc));
-                    }
-                    if (tmplam > 0.5 * alam) tmplam = 0.5 * alam;
-                }
+                int outcome;
+                tmplam = orc_ls_next_lambda(alam, alam2, f_new, f2, f_old, slope, &outcome);
+                ls_record(iterl, alam, f_new, f_old, slope, alamin, outcome, tmplam, old6, step,
+                          test, sum, scaled);
             }
             alam2 = alam;
             f2 = f_new; /* Q18 */

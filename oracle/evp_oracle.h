@@ -112,7 +112,21 @@ void orc_job_free(orc_job *job);
 int orc_job_run(orc_job *job, int n, int *iters, const int *forced_iters);
 int orc_job_nvox(const orc_job *job);
 int orc_threads(void);
-void orc_set_threads(int n); /* OpenMP threads the oracle uses */
+void orc_set_threads(int n);
+
+/* Line-search record of orc_update_stress_voxel (tests).  Row of ORC_LS_ROW
+ * doubles per trial point: [0] Newton iteration, [1] lambda, [2] f(lambda),
+ * [3] f(0), [4] slope, [5] alamin, [6] outcome (0 sufficient decrease,
+ * 1 lambda < alamin: step rejected, 2 quadratic backtrack, 3 cubic branch
+ * with |a| < 1e-4, 4 cubic with disc < 0, 5 cubic with b <= 0, 6 cubic with
+ * b > 0), [7] next lambda before the 0.1 floor, [8..13] sigma at the start
+ * of the iteration (6-D basis), [14..19] the step (after the stpmax cap),
+ * [20] test, [21] |step| before the cap, [22] 1 if capped.            */
+#define ORC_LS_ROW 24
+void orc_ls_trace(double *buf, int cap_rows);
+double orc_ls_next_lambda(double alam, double alam2, double f_new, double f2, double f_old,
+                          double slope, int *outcome);
+int orc_ls_trace_count(void); /* OpenMP threads the oracle uses */
 void orc_job_dims(const orc_job *job, int dims[3]);
 /* field access: 0 Sig, 1 Eps, 2 Edot, 3 DisGrad, 4 VelGrad, 5 TranMat (all N^3 x 9),
  * 6 crss (N^3 x 12 x 2), 7 gamdot (N^3 x 12), 8 gamacum (N^3); dislocation-

@@ -1,11 +1,17 @@
-// adjoint_check.cpp -- host build of the per-voxel adjoint functions
-// (m-themes_amd/csrc/adjoint_voxel.h, the same source the device kernels
-// compile) behind plain C entry points, so that each transposed
-// linearisation can be checked against finite differences of its forward
-// function on the CPU.  Test infrastructure, not part of the libraries.
+// hostcheck.cpp -- host builds of device code that the tests check on the
+// CPU, behind plain C entry points (test infrastructure, not part of the
+// libraries):
+//   * the per-voxel adjoint functions (m-themes_amd/csrc/adjoint_voxel.h,
+//     the same source the device kernels compile), so that each transposed
+//     linearisation can be checked against finite differences of its
+//     forward function;
+//   * the Newton kernel's line-search backtracking rule
+//     (m-themes_amd/csrc/linesearch.h), checked against the oracle's and
+//     against the models it minimises on every branch.
 #include <string.h>
 
 #include "adjoint_voxel.h"
+#include "linesearch.h"
 
 using namespace adj;
 
@@ -155,3 +161,9 @@ int adjc_tangent(const double *S36, const double *D36, double dt, double *Ct36)
 }
 
 } // extern "C"
+
+extern "C" double lsc_backtrack(double alam, double alam2, double f_new, double f2, double f_old,
+                                double slope)
+{
+    return ls_backtrack(alam, alam2, f_new, f2, f_old, slope);
+}

@@ -1,5 +1,5 @@
 """Per-voxel pieces of the adjoint (m-themes_amd/csrc/adjoint_voxel.h, host build
-in tests/adjcheck).
+in tests/hostcheck).
 
 Each transposed linearisation is checked with the dot-product identity
 <ybar, J dx> = <J^T ybar, dx>, where J dx comes from central finite
@@ -17,8 +17,8 @@ import pytest
 import subprocess
 
 DP = C.POINTER(C.c_double)
-HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "adjcheck")
-LIB = os.path.join(HERE, "lib", "libadjcheck.so")
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hostcheck")
+LIB = os.path.join(HERE, "lib", "libhostcheck.so")
 
 
 @pytest.fixture(scope="module")

@@ -1,0 +1,161 @@
+"""The 256^3 and 64^3 configurations on the GPU path.
+
+256^3, 4096 grains (the large-grid job): increment 1 against the oracle at
+two forced Green iterations (FP64 to 1e-10, the mixed loop to 1e-6; the
+oracle holds ~2 KB of host memory per voxel, ~33 GB here, and takes a few
+minutes on the host's cores), then properties of the converged increment:
+the mixed loop at the FP64 loop's iteration count within 1e-6 of it and the
+stress-controlled macroscopic components at the loop tolerance; grain and
+phase fields bit-exact against the microstructure file as the oracle reads
+it.
+
+64^3, 128 grains, 20 increments with checkpoints every 5 (the calibration
+job): the adjoint gradient of L = sum_k (Sigma_33,k - target_k)^2 in all
+five material parameters against central differences (step 1e-6 relative)
+of the GPU forward, every perturbed run forced to the base run's iteration
+counts, under the validation Newton tolerances; and the reverse pass within
+three forward passes.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pyevp import lib, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c3(tmp_path_factory):
+    d = tmp_path_factory.mktemp("c3")
+    t = time.perf_counter()
+    inp = synth.write_job(str(d), synth.config_c3())
+    say(f"c3: wrote the 256^3 job in {time.perf_counter() - t:.1f} s")
+    return dict(inp=inp)
+
+
+def test_c3_256_oracle_setup(c3):
+    """The oracle reads the 256^3 job (its own test step: the setup and the
+    iterations below each take a minute or more on the host)."""
+    t = time.perf_counter()
+    c3["J"] = O.Job(c3["inp"])
+    say(f"c3: oracle setup {time.perf_counter() - t:.1f} s ({O.threads()} threads)")
+    assert c3["J"].n == 256 ** 3
+
+
+def test_c3_256_forced_iterations_vs_oracle(c3):
+    J = c3.pop("J")
+    J.run(1, forced=[2])
+    say(f"c3: oracle 2 iterations {J.macro()['green_time']:.1f} s")
+    ref = {k: J.field(f) for k, f in (("Sig", O.SIG), ("Edot", O.EDOT), ("DisGrad", O.DISGRAD))}
+    g_o, p_o = J.grain(), J.phase()
+    mo = J.macro()
+    del J
+    for prec, tol, tol_edot in ((lib.FP64, 1e-10, 1e-10), (lib.MIXED, 1e-6, 1e-5)):
+        say(f"c3: GPU precision {prec}, 2 forced iterations")
+        g = lib.GpuJob(c3["inp"], prec)
+        g.run(1, forced=2)
+        s = g.state()
+        e = {k: rel(s[k], ref[k]) for k in ref}
+        say(f"c3: precision {prec}: {e}")
+        assert e["Sig"] < tol and e["DisGrad"] < tol and e["Edot"] < tol_edot, e
+        assert rel(g.macro()["sig_avg"], mo["SigAvg"]) < tol
+        # grain / phase: the file's columns 7 and 8 as the oracle read them
+        gg, pg = g.ids()
+        assert np.array_equal(gg, g_o) and np.array_equal(pg, p_o)
+        g.close()
+
+
+def test_c3_256_converged_increment(c3):
+    cfg = synth.config_c3()
+    a = lib.GpuJob(c3["inp"], lib.FP64)
+    ia, sa = a.run(1)
+    m = a.macro()
+    say(f"c3: FP64 increment 1: {int(ia[0])} iterations, err_s {sa[0]['err_s']:.3e}, "
+        f"err_e {sa[0]['err_e']:.3e}, Sigma11 {m['sig_avg'][0, 0]:.3e}, Sigma22 "
+        f"{m['sig_avg'][1, 1]:.3e}, s_vm {m['s_vm']:.4f}")
+    assert max(sa[0]["err_s"], sa[0]["err_e"]) <= cfg.err and ia[0] < cfg.iter_max
+    # the stress-controlled components sit at the loop tolerance
+    assert abs(m["sig_avg"][0, 0]) <= 1e-4 * m["s_vm"] and abs(m["sig_avg"][1, 1]) <= 1e-4 * m["s_vm"]
+    b = lib.GpuJob(c3["inp"], lib.MIXED)
+    b.run(1, forced=int(ia[0]))
+    sb, sa_ = b.state()["Sig"], a.state()["Sig"]
+    e = rel(sb, sa_)
+    say(f"c3: mixed at the FP64 count: max rel stress diff {e:.3e}")
+    assert e <= 1e-6
+    a.close()
+    b.close()
+
+
+# ------------------------------------------------------------------ 64^3 --
+C4_ERR = 1e-9
+
+
+def c4_job(d, p=None):
+    cfg = synth.config_c4()
+    cfg.err = C4_ERR
+    cfg.iter_max = 40000
+    job = lib.GpuJob(synth.write_job(str(d), cfg))
+    job.set_newton_tolerances(1e-14, 1e-13)
+    if p is not None:
+        job.set_params(0, p)
+    return job, cfg
+
+
+def forced_szz(job, iters):
+    out = []
+    for k in iters:
+        job.run(1, forced=int(k))
+        out.append(job.macro()["sig_avg"][2, 2])
+    return np.array(out)
+
+
+def test_c4_64_adjoint_gradient_vs_gpu_central_differences(tmp_path):
+    job, cfg = c4_job(tmp_path / "base")
+    p0 = job.get_params(0)
+    n = cfg.increments
+    # target: a seeded forward at parameters +5 %
+    job.set_params(0, p0 * 1.05)
+    target, _ = job.adjoint_record(n, 5)
+    target = target[:, 2].copy()
+    job.set_params(0, p0)
+    t0 = time.perf_counter()
+    curve, stats = job.adjoint_record(n, 5)
+    t_fwd = time.perf_counter() - t0
+    iters = [s["iters"] for s in stats]
+    say(f"c4: forward {t_fwd:.2f} s, Green iterations {iters}")
+    seeds = np.zeros((n, 6))
+    seeds[:, 2] = 2.0 * (curve[:, 2] - target)
+    t0 = time.perf_counter()
+    g, st = job.adjoint_run(seeds, cg_tol=1e-12, cg_max=4000)
+    t_adj = time.perf_counter() - t0
+    say(f"c4: adjoint {t_adj:.2f} s (replay {st['replay_seconds']:.2f} s, CG {st['cg_iters']}), "
+        f"ratio {t_adj / t_fwd:.3f}; gradient {g[0]}")
+    assert t_adj <= 3.0 * t_fwd
+    fj, _ = c4_job(tmp_path / "fd")
+    fd = np.zeros(5)
+    for i in range(5):
+        h = 1e-6 * abs(p0[i])
+        lv = []
+        for sgn in (1, -1):
+            p = p0.copy()
+            p[i] += sgn * h
+            fj.set_params(0, p)
+            lv.append(np.sum((forced_szz(fj, iters) - target) ** 2))
+        fd[i] = (lv[0] - lv[1]) / (2 * h)
+        say(f"c4: parameter {lib.PARAM_NAMES[i]}: adjoint {g[0][i]:.10e} FD {fd[i]:.10e} "
+            f"rel {abs(g[0][i] - fd[i]) / abs(fd[i]):.2e}")
+    assert np.all(np.abs(g[0] - fd) <= 1e-5 * np.abs(fd)), (g[0], fd)
+    job.close()
+    fj.close()
