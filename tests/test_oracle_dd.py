@@ -298,3 +298,81 @@ def test_dd_gnd_rates_from_a_uniform_shear_rate_gradient(tmp_path):
     np.testing.assert_array_equal(J.field(O.RHO_G3), 0.0)
     # C_4..C_7 = 0 and no climb: the SSD density is left as it was
     np.testing.assert_array_equal(J.field(O.RHO_S), law.rho_ssd_initial)
+
+
+@pytest.mark.parametrize("term", ["C4", "C5", "C6", "C7", "climb"])
+def test_ssd_rate_terms_one_at_a_time(tmp_path, term):
+    """DislocationEvolution's SSD rate (src/constitutive.c:1395-1407) with all
+    coefficients but one at zero, on a uniform crystal with set stress, shear
+    rates and densities (every system different), after one call, against
+    each term written from its physics in numpy:
+      C4  forest storage      + |gamdot| C4 sqrt(rho_F)
+      C5  dynamic recovery    - |gamdot| C5 rho_s
+      C6  dipole formation    + |gamdot| C6 d rho_m, d = sqrt(3) G b /
+                                (16 pi (1 - nu) (|tau| + eps)) the dipole
+                                capture distance (nm: G, tau in MPa, b in nm)
+      C7  static annihilation - C7 rho_s^2 (the reference's 1.6e12 x 1e-9
+                                density units)
+      climb                   - 4e12 d (D0 / b) rho_s^2 exp(|p| b^3 / (k T)
+                                - 1) exp(-Q_bulk / (k T)), with p b^3 in J
+                                (MPa nm^3 = 1e-21 J), k = 1.38e-23 J/K in the
+                                pressure factor as the reference writes it and
+                                the Boltzmann constant of src/evp.h:84-85 in
+                                the Arrhenius one (Q_bulk in 1e-19 J); p is
+                                the reference's 'hydro', a third of the sum of
+                                the first three basis components (a quirk)
+    The dipole and forest diagnostics grow by their terms alone."""
+    import crystal_np as X
+    coeff = {"C4": (3, 800.0), "C5": (4, 16.0), "C6": (5, 1100.0), "C7": (6, 1.25e-4)}
+    c = [0.48, 3.0, 3.0, 0.0, 0.0, 0.0, 0.0, 0.24]
+    if term in coeff:
+        c[coeff[term][0]] = coeff[term][1]
+    T_K, q_bulk, D0 = 700.0, 1.2, (4e-5 if term == "climb" else 0.0)
+    law = synth.DDLaw(c=tuple(c), T_K=T_K, q_bulk=q_bulk, D_0=D0)
+    euler = (20.0, 35.0, 50.0)
+    cfg = synth.Config(n=0, dims=(2, 2, 2), grains=1, increments=1, dd=law, single_orientation=euler)
+    J = O.Job(synth.write_job(str(tmp_path), cfg), O.LAW_DD)
+    N = 8
+    rng = np.random.default_rng(3)
+    s = rng.standard_normal((3, 3)) * 60.0
+    sig = s + s.T + np.eye(3) * 45.0
+    gd = rng.uniform(-2.0, 2.0, 12)
+    rs, rm, rf = rng.uniform(0.5, 3.0, 12), rng.uniform(0.1, 2.0, 12), rng.uniform(1.0, 9.0, 12)
+    J.set_field(O.SIG, np.broadcast_to(sig, (N, 3, 3)))
+    J.set_field(O.GAMDOT, np.broadcast_to(gd, (N, 12)))
+    J.set_field(O.RHO_S, np.broadcast_to(rs, (N, 12)))
+    J.set_field(O.RHO_M, np.broadcast_to(rm, (N, 12)))
+    J.set_field(O.RHO_F, np.broadcast_to(rf, (N, 12)))
+    dip0, for0 = J.field(O.RHO_DIPOLE), J.field(O.RHO_FOREST)
+    J.dd_evolve()
+    d_rs = J.field(O.RHO_S)[0] - rs
+    d_dip = J.field(O.RHO_DIPOLE)[0] - dip0[0]
+    d_for = J.field(O.RHO_FOREST)[0] - for0[0]
+    # numpy: resolved shear stress from the slip geometry and the rotation
+    T = X.rotation(euler)
+    P = 0.5 * (np.einsum("si,sj->sij", B_HAT @ T.T, N_HAT @ T.T) +
+               np.einsum("si,sj->sij", N_HAT @ T.T, B_HAT @ T.T))
+    tau = np.einsum("sij,ij->s", P, sig)
+    Bv = np.einsum("iab,ab->i", O.basis(), sig)
+    p = (Bv[0] + Bv[1] + Bv[2]) / 3.0
+    tab = J.dd_tables()
+    G, b = tab["shear_G"][0], tab["bbB"][0]
+    nu = synth.Material().nu
+    dcap = np.sqrt(3.0) * G * b / (16.0 * X.PI_REF * (1.0 - nu)) / (np.abs(tau) + 5e-3)
+    dt = cfg.dt
+    ag = np.abs(gd)
+    want = {
+        "C4": ag * 800.0 * np.sqrt(rf),
+        "C5": -ag * 16.0 * rs,
+        "C6": ag * 1100.0 * dcap * rm,
+        "C7": -1.25e-4 * 1.6e3 * rs ** 2,
+        "climb": -4e12 * dcap * (D0 / b) * rs ** 2 *
+                 np.exp(abs(p) * 1e6 * (b * 1e-9) ** 3 / (1.38e-23 * T_K) - 1.0) *
+                 np.exp(-q_bulk * 1e-19 / (1.380658e-23 * T_K)),
+    }[term] * dt
+    assert np.all(np.abs(want) > 1e-10 * rs)  # every system's term is resolved
+    np.testing.assert_allclose(d_rs, want, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(d_dip, ag * (1100.0 if term == "C6" else 0.0) * dcap * rm * dt,
+                               rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(d_for, ag * (800.0 if term == "C4" else 0.0) * np.sqrt(rf) * dt,
+                               rtol=1e-12, atol=1e-300)
