@@ -121,8 +121,12 @@ def forced_szz(job, iters):
     return np.array(out)
 
 
-def test_c4_64_adjoint_gradient_vs_gpu_central_differences(tmp_path):
-    job, cfg = c4_job(tmp_path / "base")
+@pytest.fixture(scope="module")
+def c4(tmp_path_factory):
+    """The recorded base forward, its adjoint gradient and a second job for
+    the finite differences."""
+    d = tmp_path_factory.mktemp("c4")
+    job, cfg = c4_job(d / "base")
     p0 = job.get_params(0)
     n = cfg.increments
     # target: a seeded forward at parameters +5 %
@@ -142,20 +146,27 @@ def test_c4_64_adjoint_gradient_vs_gpu_central_differences(tmp_path):
     t_adj = time.perf_counter() - t0
     say(f"c4: adjoint {t_adj:.2f} s (replay {st['replay_seconds']:.2f} s, CG {st['cg_iters']}), "
         f"ratio {t_adj / t_fwd:.3f}; gradient {g[0]}")
-    assert t_adj <= 3.0 * t_fwd
-    fj, _ = c4_job(tmp_path / "fd")
-    fd = np.zeros(5)
-    for i in range(5):
-        h = 1e-6 * abs(p0[i])
-        lv = []
-        for sgn in (1, -1):
-            p = p0.copy()
-            p[i] += sgn * h
-            fj.set_params(0, p)
-            lv.append(np.sum((forced_szz(fj, iters) - target) ** 2))
-        fd[i] = (lv[0] - lv[1]) / (2 * h)
-        say(f"c4: parameter {lib.PARAM_NAMES[i]}: adjoint {g[0][i]:.10e} FD {fd[i]:.10e} "
-            f"rel {abs(g[0][i] - fd[i]) / abs(fd[i]):.2e}")
-    assert np.all(np.abs(g[0] - fd) <= 1e-5 * np.abs(fd)), (g[0], fd)
     job.close()
+    fj, _ = c4_job(d / "fd")
+    yield dict(p0=p0, target=target, iters=iters, grad=g[0], t_fwd=t_fwd, t_adj=t_adj, fj=fj)
     fj.close()
+
+
+def test_c4_64_adjoint_within_three_forwards(c4):
+    assert c4["t_adj"] <= 3.0 * c4["t_fwd"]
+
+
+@pytest.mark.parametrize("i", range(5), ids=list(lib.PARAM_NAMES))
+def test_c4_64_adjoint_gradient_vs_gpu_central_differences(c4, i):
+    p0, fj = c4["p0"], c4["fj"]
+    h = 1e-6 * abs(p0[i])
+    lv = []
+    for sgn in (1, -1):
+        p = p0.copy()
+        p[i] += sgn * h
+        fj.set_params(0, p)
+        lv.append(np.sum((forced_szz(fj, c4["iters"]) - c4["target"]) ** 2))
+    fd = (lv[0] - lv[1]) / (2 * h)
+    g = c4["grad"][i]
+    say(f"c4: parameter {lib.PARAM_NAMES[i]}: adjoint {g:.10e} FD {fd:.10e} rel {abs(g - fd) / abs(fd):.2e}")
+    assert abs(g - fd) <= 1e-5 * abs(fd)

@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 from pyevp import lib, synth
+import ls_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -478,3 +479,99 @@ def test_job_field_and_texture_dumps(job_factory, tmp_path):
         a = O.euler_to_mat(*(rec["angle"][v] * 3.14159265359 / 180.0))
         if abs(a[2, 2]) < 0.999:
             assert np.abs(a.T - st["TranMat"][v]).max() < 1e-9
+
+
+@pytest.mark.parametrize("nrs", ls_cases.NRS_VALUES)
+def test_newton_line_search_branches_vs_oracle(tmp_path, nrs):
+    """The voxels of the oracle's line-search pins (tests/ls_cases.py:
+    orientations, stresses and strains over several decades, n = 10, 20, 30)
+    through evpgpu_newton_host: the oracle's trace records which branches
+    each solve takes (accepted steps, TOLX rejections, quadratic and cubic
+    backtracks); the GPU must give the same Newton count and the same stress.
+    Solves whose Jacobian is nearly singular at some iterate (condition
+    > 1e6) or whose merit value exceeds 1e12 are compared only where the
+    rounding of the two solves (Cholesky on the GPU, the explicit LU inverse
+    in the oracle) cannot change a step decision: they are counted, not
+    asserted."""
+    import test_oracle_line_search as TL
+    cfg = synth.Config(n=8, grains=4, seed=5, increments=1, material=synth.Material(nrs=nrs))
+    gj = lib.GpuJob(synth.write_job(str(tmp_path), cfg))
+    c066 = gj.setup_info()[0]
+    L = TL._lib()
+    vox, refs = [], []
+    for (T, lam, E, crss) in ls_cases.cases(200, 500 + int(nrs), nrs):
+        vx = TL.Voxel(T, lam, E, crss, nrs, c066)
+        st, s, ed, gd, nit, rows = TL.solve(vx, L)
+        if st:
+            continue
+        good = bool(np.all(rows[:, 3] < 1e12))
+        if good:
+            for r in rows[rows[:, 0] != np.roll(rows[:, 0], 1)]:
+                if np.linalg.cond(vx.jac(r[8:14])) > 1e6:
+                    good = False
+                    break
+        vox.append(vx)
+        refs.append((s, nit, rows, good))
+    m = len(vox)
+    sym = lambda t: np.array([t[0, 0], t[1, 1], t[2, 2], t[1, 2], t[0, 2], t[0, 1]])
+    sgr = np.array([O.lu_inv66(v.Cgr66) for v in vox]).reshape(m, 36)
+    sig = np.array([sym(v.from6(v.lam6)) for v in vox]).T.copy()
+    strain = np.array([sym(v.from6(v.E6)) for v in vox]).T.copy()
+    tran = np.array([v.T.reshape(9) for v in vox]).T.copy()
+    crss = np.array([v.crss.reshape(24) for v in vox]).T.copy()
+    st, s_out, ed, gd, err, nit = lib.Context.newton(
+        gj_ctx(gj), np.ones(m, np.int32), np.arange(m, dtype=np.int32), sgr, sig, strain,
+        np.zeros((6, m)), tran, crss)
+    assert st == 0
+    seen = np.zeros(7, int)
+    n_good = n_same = 0
+    for i, (s, it, rows, good) in enumerate(refs):
+        ref = sym(s)
+        same = nit[i] == it and np.max(np.abs(s_out[:, i] - ref)) <= 1e-10 * np.max(np.abs(ref))
+        n_same += bool(same)
+        if good:
+            n_good += 1
+            assert same, (i, nit[i], it, s_out[:, i], ref)
+            seen += np.bincount(rows[:, 6].astype(int), minlength=7)
+    print(f"n = {nrs}: {m} voxels, {n_good} well conditioned (all equal), {n_same} equal in all; "
+          f"branches of the compared solves {seen}")
+    assert seen[0] > 0 and seen[1] > 0 and seen[2] > 0 and seen[5] + seen[6] > 0
+    gj.close()
+
+
+def test_plastic_laminate_vs_exact_solution(tmp_path):
+    """The GPU loop on a two-crystal laminate along x (4 x 2 x 2 voxels) over
+    10 increments against the exact laminate solution solved in numpy
+    (tests/crystal_np.py; the oracle's own check in test_oracle_plastic.py):
+    stress and displacement gradient of every increment, then plastic strain,
+    orientation and CRSS."""
+    import os
+    import crystal_np as X
+    mat = synth.Material(nrs=12.0, tau1=20.0, thet0=300.0)
+    eulers = [(10.0, 30.0, 40.0), (70.0, 20.0, 110.0)]
+    dims = (4, 2, 2)
+    cfg = synth.Config(n=0, dims=dims, grains=2, increments=10, err=1e-12, iter_max=40000, material=mat)
+    N = 16
+    lay = (np.arange(N) // 4) < 2
+    eul = np.where(lay[:, None], np.array(eulers[0]), np.array(eulers[1]))
+    synth.write_ms(os.path.join(str(tmp_path), "ms.txt"), dims, eul, np.where(lay, 1, 2).astype(np.int32),
+                   np.ones(N, np.int32))
+    synth.write_sx(os.path.join(str(tmp_path), "Cu.sx"), mat)
+    synth.write_input(os.path.join(str(tmp_path), "input.in"), cfg)
+    gj = lib.GpuJob(os.path.join(str(tmp_path), "input.in"))
+    gj.set_newton_tolerances(1e-14, 1e-13)
+    M = X.Laminate(eulers, mat)
+    for k in range(10):
+        gj.run(1)
+        sigs, res = M.increment()
+        st = gj.state()
+        ref = np.where(lay[:, None, None], sigs[0][None], sigs[1][None])
+        assert np.max(np.abs(st["Sig"] - ref)) <= 1e-9 * np.max(np.abs(ref)), k
+        grad = np.where(lay[:, None, None], M.L[0].grad[None], M.L[1].grad[None])
+        assert np.max(np.abs(st["DisGrad"] - grad)) <= 1e-9 * np.max(np.abs(grad)), k
+    for l, Ly in enumerate(M.L):
+        v = 0 if l == 0 else N - 1
+        assert np.max(np.abs(st["TranMat"][v] - Ly.T)) <= 1e-9
+        assert np.max(np.abs(st["crss"][v] - Ly.crss[:, None])) <= 1e-9 * np.max(Ly.crss)
+        assert np.max(np.abs(st["Eps"][v] - Ly.epsp)) <= 1e-9 * np.max(np.abs(Ly.epsp))
+    gj.close()
