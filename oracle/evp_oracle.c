@@ -1761,6 +1761,9 @@ static int green_iteration(orc_job *J)
             J->newton_iters += J->nr_v[v];
         }
     }
+    /* a non-finite stress or rate makes these sums non-finite: stop, as the
+     * reference's isnan asserts do (src/constitutive.c:617,653,700) */
+    if (!isfinite(es) || !isfinite(ee)) return ORC_ERR_NAN;
     J->Err_e = ee * J->WGT;
     J->Err_s = es * J->WGT;
     get_smacro(J);

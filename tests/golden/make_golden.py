@@ -3,7 +3,10 @@
 Runs the CPU oracle (oracle/) on
   * the reference's own test_run/ inputs (input.in, Cu_p.sx, the 32^3
     pore microstructure), all N_steps = 2 increments;
-  * a 32^3 synthetic job (64 Voronoi grains, seed 0x5EED0001), 3 increments;
+  * the 32^3 synthetic job (64 Voronoi grains, seed 0x5EED0001), all 10
+    increments of its configuration;
+  * a 16^3 job with 2 % pores (8 grains), 20 increments: orientation updates
+    and hardening compounding over a longer horizon;
 and stores per-increment Green iteration counts, macroscopic stress and the
 vm.out row, plus the final stress / strain-rate / displacement-gradient /
 CRSS fields on every 16th voxel.  The test_run inputs are also run under the
@@ -12,6 +15,8 @@ dislocation-density law the reference's build compiles in
 
     python tests/golden/make_golden.py        # all fixtures
     python tests/golden/make_golden.py dd     # the dislocation-density one only
+    python tests/golden/make_golden.py c1     # the 32^3 job only
+    python tests/golden/make_golden.py long   # the 16^3, 20-increment job only
 """
 import os
 import sys
@@ -52,12 +57,30 @@ def record(inp, nsteps, out, law=O.LAW_POWER):
     print(out, iters, flush=True)
 
 
-if __name__ == "__main__":
-    record(os.path.join(HERE, "test_run", "input.in"), 2, os.path.join(HERE, "oracle_test_run_dd.npz"),
-           O.LAW_DD)
-    if sys.argv[1:] == ["dd"]:
-        sys.exit(0)
-    record(os.path.join(HERE, "test_run", "input.in"), 2, os.path.join(HERE, "oracle_test_run.npz"))
+def c1():
     with tempfile.TemporaryDirectory() as d:
         inp = synth.write_job(d, synth.config_c1())
-        record(inp, 3, os.path.join(HERE, "oracle_c1_32.npz"))
+        record(inp, synth.config_c1().increments, os.path.join(HERE, "oracle_c1_32.npz"))
+
+
+def long16():
+    cfg = synth.Config(n=16, grains=8, seed=0x5EED0016, increments=20, pore_fraction=0.02, pore_radius=1)
+    with tempfile.TemporaryDirectory() as d:
+        record(synth.write_job(d, cfg), cfg.increments, os.path.join(HERE, "oracle_16_pores_20.npz"))
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:]
+    if which == ["c1"]:
+        c1()
+        sys.exit(0)
+    if which == ["long"]:
+        long16()
+        sys.exit(0)
+    record(os.path.join(HERE, "test_run", "input.in"), 2, os.path.join(HERE, "oracle_test_run_dd.npz"),
+           O.LAW_DD)
+    if which == ["dd"]:
+        sys.exit(0)
+    record(os.path.join(HERE, "test_run", "input.in"), 2, os.path.join(HERE, "oracle_test_run.npz"))
+    c1()
+    long16()

@@ -74,3 +74,21 @@ def test_dropin_abort_exit_code(job, tmp_path):
     r = subprocess.run([os.path.join(lib.LIBDIR, "forge"), "input.in"], cwd=d,
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 114
+
+
+def test_oracle_stops_on_non_finite_stress(tmp_path):
+    """A CRSS so small that |tau/crss|^n overflows: the increment stops with
+    an error status -- the singular-LU exit 35 when the overflow reaches the
+    Newton matrix, the NaN status 9001 (the reference's isnan asserts) when
+    only the error sums turn non-finite -- instead of reporting a converged
+    increment of NaN fields.  (The HIP path returns 35 from its Cholesky
+    pivot test, and checks the block sums after every Newton launch:
+    EVPGPU_ERR_NONFINITE.)"""
+    import oracle as O
+    from pyevp import synth
+    mat = synth.Material(tau0f=1e-35, tau0b=1e-35)
+    inp = synth.write_job(str(tmp_path), synth.Config(n=8, grains=2, seed=3, increments=1, material=mat))
+    J = O.Job(inp)
+    with pytest.raises(O.OracleError) as e:
+        J.run(1)
+    assert e.value.code in (35, 9001)

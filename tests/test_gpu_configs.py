@@ -99,7 +99,7 @@ def test_c3_256_converged_increment(c3):
 
 
 # ------------------------------------------------------------------ 64^3 --
-C4_ERR = 1e-9
+C4_ERR = 1e-7
 
 
 def c4_job(d, p=None):

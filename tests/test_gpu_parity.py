@@ -390,12 +390,23 @@ def test_golden_reference_test_run():
 
 
 def test_golden_c1_32(tmp_path):
-    """32^3, 64 grains (the CPU-runnable configuration), first 3 increments."""
+    """32^3, 64 grains (the CPU-runnable configuration), all 10 increments."""
     import os
     from conftest import GOLDEN
     from pyevp import synth
     inp = synth.write_job(str(tmp_path), synth.config_c1())
-    _golden_compare(inp, os.path.join(GOLDEN, "oracle_c1_32.npz"), 3)
+    _golden_compare(inp, os.path.join(GOLDEN, "oracle_c1_32.npz"), synth.config_c1().increments)
+
+
+def test_golden_16_pores_20_increments(tmp_path):
+    """16^3 with 2 % pores, 20 increments: 18 orientation updates and
+    hardening steps compounding, against the oracle's fixture."""
+    import os
+    from conftest import GOLDEN
+    from pyevp import synth
+    cfg = synth.Config(n=16, grains=8, seed=0x5EED0016, increments=20, pore_fraction=0.02, pore_radius=1)
+    inp = synth.write_job(str(tmp_path), cfg)
+    _golden_compare(inp, os.path.join(GOLDEN, "oracle_16_pores_20.npz"), 20)
 
 
 def test_forge_dropin_writes_vm_out(tmp_path):
@@ -489,10 +500,12 @@ def test_newton_line_search_branches_vs_oracle(tmp_path, nrs):
     each solve takes (accepted steps, TOLX rejections, quadratic and cubic
     backtracks); the GPU must give the same Newton count and the same stress.
     Solves whose Jacobian is nearly singular at some iterate (condition
-    > 1e6) or whose merit value exceeds 1e12 are compared only where the
-    rounding of the two solves (Cholesky on the GPU, the explicit LU inverse
-    in the oracle) cannot change a step decision: they are counted, not
-    asserted."""
+    > 1e6) or whose merit value exceeds 1e12 are left out: there the two
+    solves (Cholesky of the symmetric form on the GPU, the explicit pivoted
+    LU inverse in the oracle) round differently enough to change step
+    decisions, and at rates of |x|^n ~ 1e30 and beyond the Cholesky form can
+    lose positive definiteness to rounding (status 35) where the pivoted LU
+    goes on -- states no loop reaches."""
     import test_oracle_line_search as TL
     cfg = synth.Config(n=8, grains=4, seed=5, increments=1, material=synth.Material(nrs=nrs))
     gj = lib.GpuJob(synth.write_job(str(tmp_path), cfg))
@@ -512,6 +525,10 @@ def test_newton_line_search_branches_vs_oracle(tmp_path, nrs):
                     break
         vox.append(vx)
         refs.append((s, nit, rows, good))
+    n_all = len(vox)
+    keep = [i for i in range(n_all) if refs[i][3]]
+    vox = [vox[i] for i in keep]
+    refs = [refs[i] for i in keep]
     m = len(vox)
     sym = lambda t: np.array([t[0, 0], t[1, 1], t[2, 2], t[1, 2], t[0, 2], t[0, 1]])
     sgr = np.array([O.lu_inv66(v.Cgr66) for v in vox]).reshape(m, 36)
@@ -524,17 +541,12 @@ def test_newton_line_search_branches_vs_oracle(tmp_path, nrs):
         np.zeros((6, m)), tran, crss)
     assert st == 0
     seen = np.zeros(7, int)
-    n_good = n_same = 0
     for i, (s, it, rows, good) in enumerate(refs):
         ref = sym(s)
-        same = nit[i] == it and np.max(np.abs(s_out[:, i] - ref)) <= 1e-10 * np.max(np.abs(ref))
-        n_same += bool(same)
-        if good:
-            n_good += 1
-            assert same, (i, nit[i], it, s_out[:, i], ref)
-            seen += np.bincount(rows[:, 6].astype(int), minlength=7)
-    print(f"n = {nrs}: {m} voxels, {n_good} well conditioned (all equal), {n_same} equal in all; "
-          f"branches of the compared solves {seen}")
+        assert nit[i] == it, (i, nit[i], it)
+        assert np.max(np.abs(s_out[:, i] - ref)) <= 1e-10 * np.max(np.abs(ref)), (i, s_out[:, i], ref)
+        seen += np.bincount(rows[:, 6].astype(int), minlength=7)
+    print(f"n = {nrs}: {m} of {n_all} voxels compared; branches of their solves {seen}")
     assert seen[0] > 0 and seen[1] > 0 and seen[2] > 0 and seen[5] + seen[6] > 0
     gj.close()
 
