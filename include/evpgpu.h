@@ -226,6 +226,22 @@ int evpgpu_newton_host(evpgpu_ctx *ctx, int n, const int *phase, const int *orie
                        const double *eps6, const double *tranmat9, const double *crss24,
                        double *edot6, double *gamdot12, double *err2, int *niter);
 
+/* The per-voxel stress update of evpgpu_newton_host on caller-owned DEVICE
+ * buffers (same layouts, n voxels), allocating nothing: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  The kernel's scratch (block
+ * partial sums and the status word) is a file-scope device array of the
+ * library, one per GPU, held until the process ends; n is at most 2^21
+ * (EVPGPU_ERR_SIZE above).  Phase and orientation indices are checked on the
+ * device first (EVPGPU_ERR_ARG).  Runs on the context's stream and waits for
+ * it.  Calls that overlap on one GPU share the scratch array and race. */
+#define EVPGPU_NOALLOC_MAX_VOXELS (1 << 21)
+int evpgpu_newton_device_noalloc(evpgpu_ctx *ctx, int n, const int *phase, const int *orient,
+                                 const double *sgr, int n_orient, double *sig6,
+                                 const double *strain6, const double *eps6,
+                                 const double *tranmat9, const double *crss24, double *edot6,
+                                 double *gamdot12, double *err2, int *niter);
+
 /* Diagnostic: copy n doubles (n even) device to device reps times with the
  * library's streaming copy kernel (HBM traffic 16 n bytes per launch) and
  * report the achieved GB/s.  No counterpart in the reference; used to

@@ -371,6 +371,12 @@ int ready(const evpgpu_ctx *c)
 
 } // namespace
 
+// the Newton kernel's parameters of a context (for launches from other files)
+NewtonParams context_newton_params(const evpgpu_ctx *c)
+{
+    return newton_params(c);
+}
+
 // ------------------------------------------------------------------- ABI --
 extern "C" {
 

@@ -230,6 +230,8 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
                   double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status,
                   double *err_vox, int *niter_vox, int fast);
+// the Newton kernel's parameters of a context (evpgpu.hip)
+NewtonParams context_newton_params(const evpgpu_ctx *c);
 // column sums of partials[nblk][nval] into out[0..nval); with status, also
 // out[nval] = *status (out must then hold nval + 1 values)
 int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out,

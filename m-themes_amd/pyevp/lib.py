@@ -86,6 +86,7 @@ NPARAM = 5
 # every symbol each header declares (checked by the CPU tests)
 EVPGPU_SYMBOLS = [
     "evpgpu_strerror", "evpgpu_create", "evpgpu_destroy", "evpgpu_set_precision",
+    "evpgpu_newton_device_noalloc",
     "evpgpu_set_anchor_interval", "evpgpu_set_newton_tolerances", "evpgpu_set_newton_shortcut",
     "evpgpu_set_fused_transforms",
     "evpgpu_set_phases", "evpgpu_set_loading", "evpgpu_upload_microstructure",
