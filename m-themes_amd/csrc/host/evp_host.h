@@ -18,6 +18,8 @@
 #define EVP_ERR_THET 230        /* thet0 < thet1 (src/init.c:131-133)              */
 #define EVP_ERR_MSINDEX 1139    /* microstructure order (src/init.c:263-266)       */
 #define EVP_ERR_NPH 747         /* N_phases > NPHMAX (assert, src/init.c:747)      */
+#define EVP_ERR_DRX 850         /* the DRX nucleation gate opened (e_vm > 4,
+                                   src/evolution.c:850): not built here           */
 
 /* namelist values (src/io.c:25-94, power-law build) */
 typedef struct {

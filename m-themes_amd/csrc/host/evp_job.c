@@ -465,10 +465,28 @@ int evp_job_write_texture(evp_job *J, const char *prefix, int step)
     return st;
 }
 
+/* The dislocation-density build checks, at the top of every increment, the
+ * von Mises strain the previous increment left: above 4 it runs the DRX
+ * nucleation check and, from the first nucleus on, the phase-field relaxation
+ * (src/evolution.c:850-1316).  Below it that block changes no field the loop
+ * reads.  The DRX event itself is not built here: an increment that would
+ * start with it is refused instead of run without it. */
+static int drx_gate_open(const evp_job *J)
+{
+    if (J->law != EVP_LAW_DD) return 0;
+    evpgpu_macro m;
+    evpgpu_get_macro(J->ctx, &m);
+    return m.e_vm > 4.0;
+}
+
 int evp_job_run(evp_job *J, int n, int forced, int *iters, evpgpu_stats *stats)
 {
     if (!J) return EVPGPU_ERR_ARG;
     for (int k = 0; k < n; k++) {
+        if (drx_gate_open(J)) {
+            J->last_status = EVP_ERR_DRX;
+            return EVP_ERR_DRX;
+        }
         evpgpu_stats S;
         int st = evpgpu_increment(J->ctx, forced, &S);
         if (st) {
@@ -622,6 +640,10 @@ void Evolution(void)
         printf("\n****************************************\n");
         printf("STEP = %d\n", istep);
         if (G->nl.N_steps != 1 && fp_err) fprintf(fp_err, "STEP = %d\n", istep);
+        if (drx_gate_open(G)) {
+            fail_with(EVP_ERR_DRX, "DRX nucleation (e_vm > 4, src/evolution.c:850) is not built");
+            return;
+        }
         evpgpu_stats S;
         int st = evpgpu_increment(G->ctx, 0, &S);
         if (st) {

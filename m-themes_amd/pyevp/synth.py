@@ -150,6 +150,13 @@ def config_c4() -> Config:
     return Config(n=64, grains=128, seed=0x5EED0004, increments=20)
 
 
+def config_c5() -> Config:
+    """128^3, 256 grains, 30 increments under the dislocation-density law (the
+    reference's DRX configuration; its nucleation gate, e_vm > 4, stays shut
+    at 3 % strain)."""
+    return Config(n=128, grains=256, seed=0x5EED0005, increments=30, dd=DDLaw())
+
+
 def _shape(n) -> tuple:
     return (n, n, n) if np.isscalar(n) else tuple(int(d) for d in n)
 

@@ -376,3 +376,57 @@ def test_ssd_rate_terms_one_at_a_time(tmp_path, term):
                                rtol=1e-12, atol=1e-300)
     np.testing.assert_allclose(d_for, ag * (800.0 if term == "C4" else 0.0) * np.sqrt(rf) * dt,
                                rtol=1e-12, atol=1e-300)
+
+
+def test_orowan_rate_from_its_physics():
+    """StrainRate_Orowan (src/constitutive.c:507-721, FORWARD_PLUS_BACKWARD)
+    evaluated in numpy from its physical statement in SI units, against the
+    oracle for one voxel with every system's densities different:
+      rho_F, rho_P   interaction-weighted projections of the densities;
+      tau_pass = c1 G b sqrt(rho_P + rho_m)   (Taylor passing stress, with the
+                 mobile density of the previous call);
+      tau_cut  = Q' sqrt(rho_F) / (c2 c3 b^2)  with Q' = kT (the energy that
+                 sets the cutting stress of the thermally activated jump);
+      rho_m    = 2 kT sqrt(rho_P rho_F) / (c1 c2 c3 G b^3);
+      gam0     = rho_m nu b c2 exp(-Q_slip / kT) / sqrt(rho_F), nu = 1e10 /s;
+      gamdot   = gam0 sinh((|tau| - tau_pass) / (1000 MPa + tau_cut)) sgn tau
+                 above tau_pass, 0 below.
+    Densities in 1e12/m^2, G and stresses in MPa, b in nm, k the reference's
+    Boltzmann constant (src/evp.h:84-85): the unit conversions are done here
+    in SI, so each of the reference's folded factors (1e-3, 0.1, 160.2177,
+    1e-3) is checked, not copied.  160.2177 is the electron volt in 1e-21 J
+    truncated to 7 digits: the comparison is at 1e-6."""
+    sc, args, rho, rng = _rate_setup(4)
+    s = np.zeros(6)
+    s[:5] = rng.normal(0, 40.0, 5)
+    rm_prev = rng.uniform(0.01, 0.2, 12)
+    st, edot, _, gd, rho_m, rF, rP = _rate(s, sc, args, rho, rm_prev.copy())
+    assert st == 0
+    # SI
+    kB, eV = 1.380658e-23, 1.60217733e-19
+    T = synth.DDLaw().T_K
+    kT = kB * T                                    # J
+    G = args["G"] * 1e6                            # Pa
+    b = args["bbB"] * 1e-9                         # m
+    c1, c2, c3 = args["c1"], args["c2"], args["c3"]
+    chi, cth = args["chi"], args["cth"]
+    rs, g1, g2, g3 = (rho[k] for k in ("rho_s", "rho_g1", "rho_g2", "rho_g3"))
+    F = np.array([np.sum(chi[i] * (rs * cth[0, i] + np.abs(g1 * cth[1, i]) + np.abs(g2 * cth[2, i]) +
+                                   np.abs(g3 * cth[3, i]))) for i in range(12)])
+    P = np.array([np.sum(chi[i] * (rs * cth[4, i] + np.abs(g1 * cth[5, i]) + np.abs(g2 * cth[6, i]) +
+                                   np.abs(g3 * cth[7, i]))) for i in range(12)])
+    np.testing.assert_allclose(rF, F, rtol=1e-14)
+    np.testing.assert_allclose(rP, P, rtol=1e-14)
+    F_si, P_si, m_prev_si = F * 1e12, P * 1e12, rm_prev * 1e12   # 1/m^2
+    tau_pass = c1 * G * b * np.sqrt(P_si + m_prev_si) / 1e6      # MPa
+    tau_cut = kT * np.sqrt(F_si) / (c2 * c3 * b ** 2) / 1e6      # MPa
+    m_si = 2.0 * kT * np.sqrt(P_si * F_si) / (c1 * c2 * c3 * G * b ** 3)
+    np.testing.assert_allclose(rho_m, m_si / 1e12, rtol=1e-6)
+    q = synth.DDLaw().q_slip * 1e-19 / kT
+    gam0 = m_si * 1e10 * b * c2 * np.exp(-q) / np.sqrt(F_si)   # 1/s
+    tau = sc @ s[:5]
+    x = (np.abs(tau) - tau_pass) / (1000.0 + tau_cut)
+    want = np.where(np.abs(tau) > tau_pass, gam0 * np.sinh(x) * np.sign(tau), 0.0)
+    assert np.sum(np.abs(tau) > tau_pass) >= 3 and np.sum(np.abs(tau) <= tau_pass) >= 1
+    np.testing.assert_allclose(gd, want, rtol=1e-6, atol=1e-12 * np.abs(want).max())
+    np.testing.assert_allclose(edot[:5], sc.T @ want, rtol=1e-6, atol=1e-12 * np.abs(want).max())
