@@ -39,8 +39,13 @@ enum {
     EVPGPU_ERR_NOMEM = -4,        /* device allocation failed                     */
     EVPGPU_ERR_STATE = -5,        /* call out of order                            */
     EVPGPU_ERR_NODEV = -6,        /* no HIP device                                */
-    EVPGPU_ERR_NONFINITE = -7     /* non-finite stress/norm (the reference's
+    EVPGPU_ERR_NONFINITE = -7,    /* non-finite stress/norm (the reference's
                                      assert(!isnan), src/constitutive.c:617,653,700) */
+    EVPGPU_ERR_PF_SUM = 1385,     /* a voxel's order parameters sum below 0.5: the
+                                     assert of GetStatistics, src/pf_xrx.c:1383-1386 */
+    EVPGPU_ERR_PF_CAP = -8,       /* an order-parameter list, or the grains of a
+                                     3x3x3 neighbourhood, beyond EVPGPU_PF_CAP */
+    EVPGPU_ERR_PF_GRAIN = -9      /* a grain index outside rho_avg[0 .. n-1]     */
 };
 
 enum { EVPGPU_FP64 = 0, EVPGPU_MIXED = 1 };
@@ -290,6 +295,93 @@ int evpgpu_adjoint_record(evpgpu_ctx *ctx, int n_increments, int checkpoint_ever
  * the recorded increments. */
 int evpgpu_adjoint_run(evpgpu_ctx *ctx, const double *sig_avg_bar, double cg_tol, int cg_max,
                        double *grad, evpgpu_adjoint_stats *stats);
+
+/* ---------------------------------------------------------------------------
+ * Sparse phase-field recrystallisation step: PF_XRX (src/pf_xrx.c:1469-1698)
+ * on one GPU.  The order parameters of a voxel are a list of at most
+ * EVPGPU_PF_CAP (grain, value) entries with the reference's PFdata rules
+ * (src/PFgrid.h:54-94); Update_grid's x-major sweep (src/pf_xrx.c:895-1311)
+ * is computed data-parallel with the same results bit for bit, Update_gID,
+ * boundary_conditions and mesh_extend as the reference runs them on one
+ * rank (periodic halos).  Grain indices g are 1-based; rho_avg[g-1] is the
+ * reference's per-grain stored-energy array (src/evolution.c:555).
+ * Host arrays: voxel v = (x*Ly + y)*Lz + z; lists as n[v], idx/val[v*CAP+k].
+ */
+#define EVPGPU_PF_CAP 16
+
+typedef struct evpgpu_pf evpgpu_pf;
+
+/* the arguments and globals PF_XRX reads (src/pf_xrx.c:1469-1504) */
+typedef struct {
+    double E_gb, M_gb;        /* grain-boundary energy and mobility */
+    double mu, bb;            /* Shear_G[0], bb_B[0][0] */
+    double alpha;             /* Scale_Fdeform */
+    double pf_length_scale;   /* voxel length dx */
+    double pf_time_step;      /* phase-field time step dt */
+    int init_gid;             /* last grain ID of the microstructure (src/init.c:352) */
+} evpgpu_pf_params;
+
+typedef struct {
+    int substeps;             /* substeps of the last call */
+    int activity_passes;      /* passes of the lookup-activity fixpoint, summed */
+    int trigger_passes;       /* passes of the stored-energy trigger fixpoint, summed */
+    long long updated;        /* voxels on the update branch, summed */
+    long long rho_writes;     /* rho_avg writes, summed */
+    double ms;                /* device time of the last call's substeps */
+} evpgpu_pf_stats;
+
+/* One grid of Lx x Ly x Lz voxels (each >= 3) and an n_rho_avg-entry
+ * rho_avg, resident on the current device. */
+int evpgpu_pf_create(evpgpu_pf **pf, int Lx, int Ly, int Lz, int n_rho_avg);
+int evpgpu_pf_destroy(evpgpu_pf *pf);
+
+/* PF_XRX (src/pf_xrx.c:1469) without its MPI arguments: each voxel's list
+ * starts as (gID, 1.0), the stored energy as rho, then `steps` substeps of
+ * Update_grid, Update_gID, mesh_extend and the GetStatistics check.
+ * gID_new receives Update_gID's grain field; rho_avg (n_rho_avg) is read and
+ * written as the reference's array. */
+int evpgpu_pf_xrx(evpgpu_pf *pf, const evpgpu_pf_params *P, const int *gID, const double *rho,
+                  int steps, int *gID_new, double *rho_avg);
+
+/* The substeps from a given state: evpgpu_pf_set_lists loads interior lists
+ * (n, idx, val), the grain field GID of the stored-energy averages and the
+ * stored energy; evpgpu_pf_steps runs `steps` substeps as evpgpu_pf_xrx does;
+ * evpgpu_pf_get_lists reads the lists and stored energy back (any pointer
+ * may be NULL). */
+int evpgpu_pf_set_lists(evpgpu_pf *pf, const int *n, const int *idx, const double *val,
+                        const int *GID, const double *defeng);
+int evpgpu_pf_steps(evpgpu_pf *pf, const evpgpu_pf_params *P, int steps, int *gID_new,
+                    double *rho_avg);
+int evpgpu_pf_get_lists(evpgpu_pf *pf, int *n, int *idx, double *val, double *defeng);
+int evpgpu_pf_get_stats(evpgpu_pf *pf, evpgpu_pf_stats *stats);
+
+/* NucleationCheckDRX (src/integrated.c:161-494) in the branch the shipped
+ * build compiles (GB_BULGING_ONLY and Nucl_Static_Flag 1), on its first call,
+ * one GPU: a voxel at a grain boundary (GB_indicator, src/evolution.c:816-828,
+ * from the neighbour checks of src/constitutive.c:790-853) nucleates when
+ * rho_tot / 12 exceeds its static strength kappa, rho_tot the sum over its
+ * nsys systems of rho_s + rho_m + |(rho_g1, rho_g2, rho_g3)|.  Nuclei take the
+ * grain IDs n_grains + 1.. in sweep order and each paints the voxels within
+ * |r - r_n|^2 <= nucleus_radius (no periodic wrap), later nuclei over
+ * earlier ones (src/integrated.c:318-405).  Densities are component-major
+ * (system i of voxel v at i*N + v).  In/out: gid_rex (set to 1 where a grain
+ * is new), kc (k_c_field: zeta_kc * k_c at the nuclei).  Out: grain_out,
+ * nuclei, rho_tot, the nucleus voxels in sweep order (up to max_new) and
+ * their count n_new (EVPGPU_ERR_ARG when it exceeds max_new). */
+int evpgpu_drx_nucleate(int Lx, int Ly, int Lz, int nsys, const int *grain_in, const double *rho_s,
+                        const double *rho_m, const double *rho_g1, const double *rho_g2,
+                        const double *rho_g3, const double *kappa, int n_grains, int nucleus_radius,
+                        double zeta_kc, double k_c, int *grain_out, int *gid_rex, int *nuclei,
+                        double *kc, double *rho_tot, long long *nucleus_vox, int max_new, int *n_new);
+
+/* The static nucleation strengths kappa_drx[v] = gsl_ran_weibull(r, a[v], b)
+ * for v = 0..n-1 in order (src/init.c:1049-1053, src/evolution.c:862-866),
+ * from GSL's MT19937 seeded with `seed` (the reference seeds it from
+ * time(); seed 0 is GSL's default 4357).  Host code: the draws are one
+ * sequential stream.  evpgpu_drx_uniform_pos gives the stream's
+ * gsl_rng_uniform_pos values themselves. */
+int evpgpu_drx_weibull(unsigned long seed, long long n, const double *a, double b, double *out);
+int evpgpu_drx_uniform_pos(unsigned long seed, long long n, double *out);
 
 #ifdef __cplusplus
 }

@@ -393,6 +393,9 @@ const char *evpgpu_strerror(int s)
     case EVPGPU_ERR_STATE: return "call out of order (phases, loading, microstructure, state)";
     case EVPGPU_ERR_NODEV: return "no HIP device";
     case EVPGPU_ERR_NONFINITE: return "non-finite stress or error norm in the stress update";
+    case EVPGPU_ERR_PF_SUM: return "order parameters of a voxel sum below 0.5 (1385)";
+    case EVPGPU_ERR_PF_CAP: return "order-parameter list beyond EVPGPU_PF_CAP entries";
+    case EVPGPU_ERR_PF_GRAIN: return "grain index outside rho_avg";
     }
     return "unknown status";
 }

@@ -1,0 +1,1012 @@
+// pf.hip -- the sparse phase-field recrystallisation step PF_XRX
+// (src/pf_xrx.c:1469-1698) on one GPU, bit for bit the one-rank reference.
+//
+// The reference keeps each voxel's order parameters in a PFdata list
+// (src/PFgrid.h:54-94) and updates them in one x-major sweep, Update_grid
+// (src/pf_xrx.c:895-1311), whose neighbour reads go through PFdata's
+// non-const lookup: a grain absent from a list re-labels the list's last
+// entry when that entry's value is below 5e-3, else a zero entry is
+// appended.  The sweep is sequential as written; it is computed here in
+// parallel from four facts about it:
+//   1. Lookups never create or re-label an entry above 5e-3, so the set of
+//      grains above 5e-3 around a voxel (its candidate grains G) and every
+//      list's entries above 5e-3 are those of the start of the sweep.
+//   2. The value a lookup of grain g returns from a list L depends on L's
+//      start state only: L's value of g when g is an entry other than a
+//      last entry below 5e-3; otherwise that last entry's value when it is
+//      below 5e-3, else 0.
+//   3. A voxel takes the update branch when its list holds more than one
+//      entry at its turn.  A one-entry list above 5e-3 gains a zero entry at
+//      the first lookup of another grain, and only voxels on the update
+//      branch look into their neighbours, so the branch is the least fixed
+//      point of "an earlier update-branch neighbour has a candidate grain I
+//      lack" -- a monotone iteration over the sweep order.  The list's
+//      re-labelled or appended last entry at that turn carries the last
+//      such grain looked up.
+//   4. rho_avg[g-1] at a voxel's turn was written by the last voxel before
+//      it (or itself) whose list held g: sorted (grain, position) keys and a
+//      binary search.  The stored-energy reset to 100 (a new grain passing
+//      0.5) feeds later voxels' 7x7x7 averages; the triggers are iterated to
+//      their fixed point over the sweep order.
+// Lists are structure of arrays, entry k of cell c at k*NC + c, over the
+// reference's padded grid (three halo planes each side).  All arithmetic is
+// IEEE double in the reference's operation order: this file is built with
+// -ffp-contract=off.
+#include <hipcub/hipcub.hpp>
+
+#include "evpgpu_impl.h"
+
+namespace {
+
+constexpr double PF_EPS = 5.0E-3; // src/PFgrid.h:50
+constexpr int CAP = EVPGPU_PF_CAP;
+constexpr int NT = 256;
+constexpr int MAX_PASSES = 4096;
+
+enum { PFE_CAP = 1, PFE_GRAIN = 2, PFE_SUM = 4 };
+
+struct Dims {
+    int Lx, Ly, Lz, lx, ly, lz;
+    long long NC, NV;
+};
+
+struct Lists {
+    int *cnt;
+    int *idx;
+    double *val;
+};
+
+__host__ __device__ inline long long cix(const Dims &g, int x, int y, int z)
+{
+    return ((long long)x * g.ly + y) * g.lz + z;
+}
+
+// interior voxel d -> padded coordinates
+__device__ inline void vox(const Dims &g, long long d, int &x, int &y, int &z)
+{
+    z = (int)(d % g.Lz) + 3;
+    const long long r = d / g.Lz;
+    y = (int)(r % g.Ly) + 3;
+    x = (int)(r / g.Ly) + 3;
+}
+
+__device__ inline bool interior(const Dims &g, int x, int y, int z)
+{
+    return x >= 3 && x < g.lx - 3 && y >= 3 && y < g.ly - 3 && z >= 3 && z < g.lz - 3;
+}
+
+__device__ inline long long vid(const Dims &g, int x, int y, int z)
+{
+    return ((long long)(x - 3) * g.Ly + (y - 3)) * g.Lz + (z - 3);
+}
+
+// value a PFdata lookup of grain gr returns from cell c's start-of-sweep list
+__device__ inline double lookupv(const Dims &g, const Lists &L, long long c, int gr)
+{
+    const int n = L.cnt[c];
+    if (n == 0) return 0.0;
+    const double back = L.val[(long long)(n - 1) * g.NC + c];
+    const bool relabel = back < PF_EPS;
+    const int nf = relabel ? n - 1 : n;
+    for (int k = 0; k < nf; k++)
+        if (L.idx[(long long)k * g.NC + c] == gr) return L.val[(long long)k * g.NC + c];
+    return relabel ? back : 0.0;
+}
+
+// a list being built by assignments list[gr] = v of distinct grains (the
+// temp lists of Update_grid, the halo copies of boundary_conditions)
+struct Build {
+    int n;
+    int idx[CAP];
+    double val[CAP];
+    __device__ bool put(int gr, double v)
+    {
+        if (n > 0 && val[n - 1] < PF_EPS) {
+            idx[n - 1] = gr;
+            val[n - 1] = v;
+            return true;
+        }
+        if (n >= CAP) return false;
+        idx[n] = gr;
+        val[n] = v;
+        n++;
+        return true;
+    }
+};
+
+// the 9 face/edge neighbours that come before a voxel in the x-major sweep,
+// closest first (linear offsets -1, -Lz+1, -Lz, -Lz-1, -LyLz+Lz, -LyLz+1,
+// -LyLz, -LyLz-1, -LyLz-Lz for lengths >= 3)
+__constant__ int EARLY[9][3] = {{0, 0, -1},  {0, -1, 1}, {0, -1, 0}, {0, -1, -1}, {-1, 1, 0},
+                                {-1, 0, 1},  {-1, 0, 0}, {-1, 0, -1}, {-1, -1, 0}};
+
+// candidate grains: values above EPSILON in the 3x3x3 neighbourhood, in
+// order of first appearance (src/pf_xrx.c:940-975)
+__global__ void k_glist(Dims g, Lists A, int *gcnt, int *gidx, int *err)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        int x, y, z;
+        vox(g, d, x, y, z);
+        int G[CAP];
+        int nG = 0;
+        bool over = false;
+        for (int i = x - 1; i <= x + 1; ++i)
+            for (int j = y - 1; j <= y + 1; ++j)
+                for (int m = z - 1; m <= z + 1; ++m) {
+                    const long long c = cix(g, i, j, m);
+                    const int n = A.cnt[c];
+                    for (int k = 0; k < n; ++k)
+                        if (A.val[(long long)k * g.NC + c] > PF_EPS) {
+                            const int index = A.idx[(long long)k * g.NC + c];
+                            int it = 0;
+                            while (it < nG && G[it] != index) it++;
+                            if (it == nG) {
+                                if (nG == CAP) over = true;
+                                else G[nG++] = index;
+                            }
+                        }
+                }
+        if (over) atomicOr(err, PFE_CAP);
+        gcnt[d] = nG;
+        for (int k = 0; k < nG; k++) gidx[(long long)k * g.NV + d] = G[k];
+    }
+}
+
+__global__ void k_act_init(Dims g, Lists A, int *act)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        int x, y, z;
+        vox(g, d, x, y, z);
+        act[d] = A.cnt[cix(g, x, y, z)] >= 2;
+    }
+}
+
+// does neighbour u (interior voxel) look up a grain outside {own}?
+__device__ inline bool has_other(const Dims &g, const int *gcnt, const int *gidx, long long u, int own)
+{
+    const int n = gcnt[u];
+    for (int k = 0; k < n; k++)
+        if (gidx[(long long)k * g.NV + u] != own) return true;
+    return false;
+}
+
+// one pass of the update-branch fixed point: a one-entry list above EPSILON
+// joins when an earlier update-branch neighbour looks up another grain
+__global__ void k_act_iter(Dims g, Lists A, const int *gcnt, const int *gidx, int *act, int *changed)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        if (act[d]) continue;
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        if (A.cnt[c] != 1 || !(A.val[c] >= PF_EPS)) continue;
+        const int own = A.idx[c];
+        for (int e = 0; e < 9; e++) {
+            const int ux = x + EARLY[e][0], uy = y + EARLY[e][1], uz = z + EARLY[e][2];
+            if (!interior(g, ux, uy, uz)) continue;
+            const long long u = vid(g, ux, uy, uz);
+            if (act[u] && has_other(g, gcnt, gidx, u, own)) {
+                act[d] = 1;
+                *changed = 1;
+                break;
+            }
+        }
+    }
+}
+
+// The list at the voxel's turn: its start entries, with the last entry
+// re-labelled (start value below EPSILON) or a zero entry appended, to the
+// last grain outside the fixed entries that an earlier update-branch
+// neighbour looked up.  nn: entries; spos: position of that entry or -1;
+// sg: its grain; wn: rho_avg writes (nn on the update branch).
+__global__ void k_slot(Dims g, Lists A, const int *gcnt, const int *gidx, const int *act, int *nn, int *spos,
+                       int *sg, int *wn, int *nupd, int *err)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        const int n0 = A.cnt[c];
+        const bool relabel = n0 > 0 && A.val[(long long)(n0 - 1) * g.NC + c] < PF_EPS;
+        const int nf = relabel ? n0 - 1 : n0;
+        int found = 0;
+        for (int e = 0; e < 9 && !found; e++) {
+            const int ux = x + EARLY[e][0], uy = y + EARLY[e][1], uz = z + EARLY[e][2];
+            if (!interior(g, ux, uy, uz)) continue;
+            const long long u = vid(g, ux, uy, uz);
+            if (!act[u]) continue;
+            const int n = gcnt[u];
+            for (int k = n - 1; k >= 0; k--) {
+                const int gr = gidx[(long long)k * g.NV + u];
+                bool fixed = false;
+                for (int f = 0; f < nf; f++) fixed = fixed || A.idx[(long long)f * g.NC + c] == gr;
+                if (!fixed) {
+                    found = gr;
+                    break;
+                }
+            }
+        }
+        int n = n0, p = -1, s = 0;
+        if (relabel) {
+            p = n0 - 1;
+            s = found ? found : A.idx[(long long)(n0 - 1) * g.NC + c];
+        } else if (found) {
+            if (n0 == CAP) {
+                atomicOr(err, PFE_CAP); // the appended entry has no room
+            } else {
+                p = n0;
+                s = found;
+                n = n0 + 1;
+            }
+        }
+        nn[d] = n;
+        spos[d] = p;
+        sg[d] = s;
+        wn[d] = n > 1 ? n : 0;
+        if (n > 1) atomicAdd(nupd, 1);
+    }
+}
+
+// entry k of the list at the voxel's turn
+__device__ inline void entry(const Dims &g, const Lists &A, long long c, int k, int p, int s, int &gr, double &v)
+{
+    if (k == p) {
+        gr = s;
+        v = k < A.cnt[c] ? A.val[(long long)k * g.NC + c] : 0.0;
+    } else {
+        gr = A.idx[(long long)k * g.NC + c];
+        v = A.val[(long long)k * g.NC + c];
+    }
+}
+
+// rho_avg writes as (grain - 1, sweep position) keys
+__global__ void k_emit(Dims g, Lists A, const int *nn, const int *spos, const int *sg, const int *wn,
+                       const long long *woff, unsigned long long *keys, int *refs, int n_rho, int *err)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        const int n = wn[d];
+        if (!n) continue;
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        long long o = woff[d];
+        for (int k = 0; k < n; k++) {
+            int gr;
+            double v;
+            entry(g, A, c, k, spos[d], sg[d], gr, v);
+            if (gr < 1 || gr > n_rho) {
+                atomicOr(err, PFE_GRAIN);
+                gr = 1;
+            }
+            keys[o] = (unsigned long long)(gr - 1) * (unsigned long long)g.NV + (unsigned long long)d;
+            refs[o] = (int)(d * CAP + k);
+            o++;
+        }
+    }
+}
+
+// stored energy of cell p as voxel d's average reads it: the reset to 100 of
+// an earlier voxel of this sweep (trig), else the buffer
+__device__ inline double defeng_at(const Dims &g, const double *defeng, const int *trig, int i, int j, int m,
+                                   long long d)
+{
+    if (interior(g, i, j, m)) {
+        const long long q = vid(g, i, j, m);
+        if (q < d && trig[q]) return 100.0;
+    }
+    return defeng[cix(g, i, j, m)];
+}
+
+// Avg and count_dd of each list entry over the 7x7x7 box (src/pf_xrx.c:1042-1077)
+__global__ void k_avg(Dims g, Lists A, const int *wn, const int *spos, const int *sg, const int *gid,
+                      const double *defeng, const int *trig, double *avg, int *cdd)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        const int n = wn[d];
+        if (!n) continue;
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        int gk[CAP];
+        double a[CAP];
+        int cn[CAP];
+        for (int k = 0; k < n; k++) {
+            double v;
+            entry(g, A, c, k, spos[d], sg[d], gk[k], v);
+            a[k] = 0.0;
+            cn[k] = 0;
+        }
+        for (int i = x - 3; i <= x + 3; ++i)
+            for (int j = y - 3; j <= y + 3; ++j)
+                for (int m = z - 3; m <= z + 3; ++m) {
+                    const int gp = gid[cix(g, i, j, m)];
+                    bool hit = false;
+                    for (int k = 0; k < n; k++) hit = hit || gp == gk[k];
+                    if (!hit) continue;
+                    const double e = defeng_at(g, defeng, trig, i, j, m, d);
+                    for (int k = 0; k < n; k++)
+                        if (gp == gk[k]) {
+                            a[k] += e;
+                            cn[k] += 1;
+                        }
+                }
+        for (int k = 0; k < n; k++) {
+            avg[(long long)k * g.NV + d] = a[k];
+            cdd[(long long)k * g.NV + d] = cn[k];
+        }
+    }
+}
+
+__device__ inline double writer_value(const Dims &g, const double *avg, const int *cdd, int ref)
+{
+    const long long w = ref / CAP;
+    const int k = ref % CAP;
+    const int cn = cdd[(long long)k * g.NV + w];
+    return cn > 0 ? avg[(long long)k * g.NV + w] / cn : 0.0;
+}
+
+// rho_avg[gr-1] at voxel d's turn
+__device__ inline double rho_at(const Dims &g, const unsigned long long *keys, const int *refs, long long nw,
+                                const double *avg, const int *cdd, const double *rho_in, int gr, long long d)
+{
+    const unsigned long long key = (unsigned long long)(gr - 1) * (unsigned long long)g.NV + (unsigned long long)d;
+    long long lo = 0, hi = nw; // first index with keys[i] > key
+    while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (keys[mid] <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo > 0 && keys[lo - 1] / (unsigned long long)g.NV == (unsigned long long)(gr - 1))
+        return writer_value(g, avg, cdd, refs[lo - 1]);
+    return rho_in[gr - 1];
+}
+
+__device__ inline double laplacian(const double a[19])
+{
+    // laplacian() (src/pf_xrx.c:6-9), arguments in the call's order
+    return (0.5 * (a[0] + a[1] + a[2] + a[3] + a[4] + a[5]) +
+            0.125 * (a[10] + a[12] + a[11] + a[13] + a[7] + a[6] + a[8] + a[9] + a[14] + a[17] + a[15] + a[16]) -
+            4.5 * a[18]);
+}
+
+struct Coef {
+    double M_coeff, kappa, m, E_disl;
+    int init_gid, n_rho;
+};
+
+// the neighbour cells of the laplacian call, in its argument order
+__constant__ int LAPN[18][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
+                                {-1, 0, 1}, {-1, 0, -1}, {-1, -1, 0}, {-1, 1, 0}, {0, -1, 1}, {0, 1, 1},
+                                {0, -1, -1}, {0, 1, -1}, {1, 0, 1},  {1, 0, -1}, {1, -1, 0}, {1, 1, 0}};
+
+// the voxel update of src/pf_xrx.c:1034-1246 into the temp lists B
+__global__ void k_update(Dims g, Lists A, Lists B, Coef P, const int *gcnt, const int *gidx, const int *nn,
+                         const int *spos, const int *sg, const double *avg, const int *cdd,
+                         const unsigned long long *keys, const int *refs, long long nw, const double *rho_in,
+                         int *trig_out, int *err)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        const int NN = nn[d];
+        const int nG = gcnt[d];
+        Build t;
+        t.n = 0;
+        bool ok = true;
+        int trig = 0;
+        if (NN > 1) {
+            double sum_rex = 0.0, sum1 = 0.0, h_phi_def = 0.0;
+            for (int k = 0; k < NN; k++) {
+                const int cn = cdd[(long long)k * g.NV + d];
+                const double a = avg[(long long)k * g.NV + d];
+                if (cn > 0 && a > 0.0) {
+                    int gr;
+                    double v;
+                    entry(g, A, c, k, spos[d], sg[d], gr, v);
+                    sum_rex += v * v;
+                    sum1 += v * v * v * (10.0 - 15.0 * v + 6.0 * v * v);
+                    h_phi_def += (v * v * v * (10.0 - 15.0 * v + 6.0 * v * v)) * (a / cn);
+                }
+            }
+            int count = 0;
+            for (int it = 0; it < nG; it++)
+                if (gidx[(long long)it * g.NV + d] > P.init_gid) count++;
+            for (int it = 0; it < nG; it++) {
+                const int index = gidx[(long long)it * g.NV + d];
+                if (index < 1 || index > P.n_rho) {
+                    atomicOr(err, PFE_GRAIN);
+                    break;
+                }
+                const double s = lookupv(g, A, c, index);
+                const double h_phi = s * s * s * (10.0 - 15.0 * s + 6.0 * s * s);
+                const double dh_phi = s * s * (30.0 - 60.0 * s + 30.0 * s * s);
+                const double center_rex = s;
+                double a[19];
+                for (int q = 0; q < 18; q++)
+                    a[q] = lookupv(g, A, cix(g, x + LAPN[q][0], y + LAPN[q][1], z + LAPN[q][2]), index);
+                a[18] = center_rex;
+                const double lap_rex = laplacian(a);
+                const double ra = rho_at(g, keys, refs, nw, avg, cdd, rho_in, index, d);
+                if (ra > 0.0) {
+                    const double cc = center_rex;
+                    const double value_rex =
+                        cc - P.M_coeff * (P.m * (cc * cc * cc - cc + 3.0 * cc * (sum_rex - cc * cc)) -
+                                          P.kappa * lap_rex +
+                                          P.E_disl * dh_phi * ((sum1 - h_phi) * ra - (h_phi_def - h_phi * ra)) /
+                                              (sum1 * sum1));
+                    if (count > 0 && index > P.init_gid && cc < 0.5 && value_rex > 0.5) trig = 1;
+                    if (value_rex > 1E-04) ok = ok && t.put(index, value_rex);
+                }
+            }
+        } else if (NN == 1) {
+            for (int it = 0; it < nG; it++) {
+                const int index = gidx[(long long)it * g.NV + d];
+                ok = ok && t.put(index, lookupv(g, A, c, index));
+            }
+        }
+        if (!ok) atomicOr(err, PFE_CAP);
+        B.cnt[c] = t.n;
+        for (int k = 0; k < t.n; k++) {
+            B.idx[(long long)k * g.NC + c] = t.idx[k];
+            B.val[(long long)k * g.NC + c] = t.val[k];
+        }
+        trig_out[d] = trig;
+    }
+}
+
+__global__ void k_diff(long long n, const int *a, const int *b, int *changed)
+{
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT)
+        if (a[i] != b[i]) *changed = 1;
+}
+
+// after the sweep: the resets to 100 and rho_avg of each written grain (its
+// last writer)
+__global__ void k_finish(Dims g, const int *trig, double *defeng, const unsigned long long *keys, const int *refs,
+                         long long nw, const double *avg, const int *cdd, double *rho)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        if (trig[d]) {
+            int x, y, z;
+            vox(g, d, x, y, z);
+            defeng[cix(g, x, y, z)] = 100.0;
+        }
+    }
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < nw; i += (long long)gridDim.x * NT) {
+        const unsigned long long gr = keys[i] / (unsigned long long)g.NV;
+        if (i + 1 == nw || keys[i + 1] / (unsigned long long)g.NV != gr) rho[gr] = writer_value(g, avg, cdd, refs[i]);
+    }
+}
+
+__device__ inline int wrap3(int p, int L) { return p < 3 ? p + L : p >= L + 3 ? p - L : p; }
+
+// boundary_conditions on one rank: each halo cell is filled from its periodic
+// image entry by entry through the lookup (copies of copies add nothing: a
+// built list has values below EPSILON only in its last entry)
+__global__ void k_halo(Dims g, Lists B, int *err)
+{
+    for (long long c = blockIdx.x * (long long)NT + threadIdx.x; c < g.NC; c += (long long)gridDim.x * NT) {
+        const int z = (int)(c % g.lz);
+        const long long r = c / g.lz;
+        const int y = (int)(r % g.ly), x = (int)(r / g.ly);
+        if (interior(g, x, y, z)) continue;
+        const long long s = cix(g, wrap3(x, g.Lx), wrap3(y, g.Ly), wrap3(z, g.Lz));
+        Build t;
+        t.n = 0;
+        const int n = B.cnt[s];
+        bool ok = true;
+        for (int j = 0; j < n; j++) ok = ok && t.put(B.idx[(long long)j * g.NC + s], B.val[(long long)j * g.NC + s]);
+        if (!ok) atomicOr(err, PFE_CAP);
+        B.cnt[c] = t.n;
+        for (int k = 0; k < t.n; k++) {
+            B.idx[(long long)k * g.NC + c] = t.idx[k];
+            B.val[(long long)k * g.NC + c] = t.val[k];
+        }
+    }
+}
+
+// mesh_extend on one rank: periodic halos of the stored energy and grains
+__global__ void k_mesh(Dims g, double *defeng, int *gid)
+{
+    for (long long c = blockIdx.x * (long long)NT + threadIdx.x; c < g.NC; c += (long long)gridDim.x * NT) {
+        const int z = (int)(c % g.lz);
+        const long long r = c / g.lz;
+        const int y = (int)(r % g.ly), x = (int)(r / g.ly);
+        if (interior(g, x, y, z)) continue;
+        const long long s = cix(g, wrap3(x, g.Lx), wrap3(y, g.Ly), wrap3(z, g.Lz));
+        defeng[c] = defeng[s];
+        gid[c] = gid[s];
+    }
+}
+
+// Update_gID (src/pf_xrx.c:11-66) and the sum check of GetStatistics
+// (src/pf_xrx.c:1364-1386)
+__global__ void k_gid(Dims g, Lists A, int *gid_new, int *err)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        const int n = A.cnt[c];
+        int id = 0;
+        double mx = 0.0, sum = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double v = A.val[(long long)i * g.NC + c];
+            if (mx < v) {
+                mx = v;
+                id = A.idx[(long long)i * g.NC + c];
+            }
+            sum += v;
+        }
+        gid_new[d] = id;
+        if (sum < 0.5) atomicOr(err, PFE_SUM);
+    }
+}
+
+__global__ void k_clear(long long n, int *cnt)
+{
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) cnt[i] = 0;
+}
+
+// PF_XRX's initial state (src/pf_xrx.c:1538-1560): one entry (gID, 1.0) per
+// voxel, the stored energy and the grain field
+__global__ void k_init(Dims g, Lists A, const int *gID, const double *rho, double *defeng, int *gid)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        A.cnt[c] = 1;
+        A.idx[c] = gID[d];
+        A.val[c] = 1.0;
+        defeng[c] = rho[d];
+        gid[c] = gID[d];
+    }
+}
+
+// interior lists from host layout [v*CAP + k]
+__global__ void k_load(Dims g, Lists A, const int *n, const int *idx, const double *val, const int *GID,
+                       const double *de, double *defeng, int *gid, int *err)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        int m = n[d];
+        if (m < 0 || m > CAP) {
+            atomicOr(err, PFE_CAP);
+            m = 0;
+        }
+        A.cnt[c] = m;
+        for (int k = 0; k < m; k++) {
+            A.idx[(long long)k * g.NC + c] = idx[d * CAP + k];
+            A.val[(long long)k * g.NC + c] = val[d * CAP + k];
+        }
+        defeng[c] = de[d];
+        gid[c] = GID[d];
+    }
+}
+
+__global__ void k_store(Dims g, Lists A, const double *defeng, int *n, int *idx, double *val, double *de)
+{
+    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
+        int x, y, z;
+        vox(g, d, x, y, z);
+        const long long c = cix(g, x, y, z);
+        const int m = A.cnt[c];
+        n[d] = m;
+        for (int k = 0; k < CAP; k++) {
+            idx[d * CAP + k] = k < m ? A.idx[(long long)k * g.NC + c] : 0;
+            val[d * CAP + k] = k < m ? A.val[(long long)k * g.NC + c] : 0.0;
+        }
+        de[d] = defeng[c];
+    }
+}
+
+int grid_for(long long n)
+{
+    long long b = (n + NT - 1) / NT;
+    if (b > 65536) b = 65536;
+    return (int)(b < 1 ? 1 : b);
+}
+
+} // namespace
+
+struct evpgpu_pf {
+    Dims g;
+    int n_rho;
+    hipStream_t stream;
+    Lists A, B;
+    double *defeng;
+    int *gid;
+    int *gcnt, *gidx, *act, *nn, *spos, *sg, *wn, *trig0, *trig1, *gid_new;
+    long long *woff;
+    double *avg;
+    int *cdd;
+    double *rho_in, *rho_out;
+    unsigned long long *keys[2];
+    int *refs[2];
+    long long wcap;
+    void *tmp;
+    size_t tmp_bytes;
+    int *flags; // [0] error bits, [1] changed, [2] voxels on the update branch
+    int *h_flags;
+    long long *h_total;
+    int *h_last;
+    hipEvent_t ev0, ev1;
+    int loaded;
+    evpgpu_pf_stats st;
+};
+
+namespace {
+
+int pf_err(int bits)
+{
+    if (bits & PFE_CAP) return EVPGPU_ERR_PF_CAP;
+    if (bits & PFE_GRAIN) return EVPGPU_ERR_PF_GRAIN;
+    if (bits & PFE_SUM) return EVPGPU_ERR_PF_SUM;
+    return EVPGPU_OK;
+}
+
+template <typename T> int dalloc(T **p, long long n)
+{
+    return hipMalloc((void **)p, (size_t)(n > 0 ? n : 1) * sizeof(T)) == hipSuccess ? 0 : 1;
+}
+
+int ensure_writers(evpgpu_pf *p, long long nw)
+{
+    if (nw <= p->wcap) return EVPGPU_OK;
+    long long cap = p->wcap * 2;
+    while (cap < nw) cap *= 2;
+    for (int i = 0; i < 2; i++) {
+        (void)hipFree(p->keys[i]);
+        (void)hipFree(p->refs[i]);
+        p->keys[i] = nullptr;
+        p->refs[i] = nullptr;
+        if (dalloc(&p->keys[i], cap) || dalloc(&p->refs[i], cap)) return EVPGPU_ERR_NOMEM;
+    }
+    p->wcap = cap;
+    return EVPGPU_OK;
+}
+
+int ensure_tmp(evpgpu_pf *p, size_t bytes)
+{
+    if (bytes <= p->tmp_bytes) return EVPGPU_OK;
+    (void)hipFree(p->tmp);
+    p->tmp = nullptr;
+    if (hipMalloc(&p->tmp, bytes) != hipSuccess) return EVPGPU_ERR_NOMEM;
+    p->tmp_bytes = bytes;
+    return EVPGPU_OK;
+}
+
+int read_flags(evpgpu_pf *p)
+{
+    EVP_CHECK(hipMemcpyAsync(p->h_flags, p->flags, 3 * sizeof(int), hipMemcpyDeviceToHost, p->stream));
+    EVP_CHECK(hipStreamSynchronize(p->stream));
+    return EVPGPU_OK;
+}
+
+int clear_flag(evpgpu_pf *p, int which)
+{
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(NT), 0, p->stream, 1LL, p->flags + which);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+// one substep: Update_grid, boundary_conditions of the new lists, swap,
+// Update_gID, mesh_extend, the sum check
+int substep(evpgpu_pf *p, const Coef &P)
+{
+    const Dims &g = p->g;
+    hipStream_t s = p->stream;
+    const int gv = grid_for(g.NV), gc = grid_for(g.NC);
+    int st;
+    hipLaunchKernelGGL(k_glist, dim3(gv), dim3(NT), 0, s, g, p->A, p->gcnt, p->gidx, p->flags);
+    hipLaunchKernelGGL(k_act_init, dim3(gv), dim3(NT), 0, s, g, p->A, p->act);
+    for (int pass = 0;; pass++) {
+        if (pass == MAX_PASSES) return EVPGPU_ERR_HIP;
+        if ((st = clear_flag(p, 1))) return st;
+        hipLaunchKernelGGL(k_act_iter, dim3(gv), dim3(NT), 0, s, g, p->A, p->gcnt, p->gidx, p->act, p->flags + 1);
+        if ((st = read_flags(p))) return st;
+        p->st.activity_passes++;
+        if (p->h_flags[0]) return pf_err(p->h_flags[0]);
+        if (!p->h_flags[1]) break;
+    }
+    if ((st = clear_flag(p, 2))) return st;
+    hipLaunchKernelGGL(k_slot, dim3(gv), dim3(NT), 0, s, g, p->A, p->gcnt, p->gidx, p->act, p->nn, p->spos, p->sg,
+                       p->wn, p->flags + 2, p->flags);
+    // writer offsets
+    size_t need = 0;
+    EVP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, p->wn, p->woff, g.NV, s));
+    if ((st = ensure_tmp(p, need))) return st;
+    EVP_CHECK(hipcub::DeviceScan::ExclusiveSum(p->tmp, need, p->wn, p->woff, g.NV, s));
+    EVP_CHECK(hipMemcpyAsync(p->h_total, p->woff + g.NV - 1, sizeof(long long), hipMemcpyDeviceToHost, s));
+    EVP_CHECK(hipMemcpyAsync(p->h_last, p->wn + g.NV - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    EVP_CHECK(hipStreamSynchronize(s));
+    const long long nw = p->h_total[0] + p->h_last[0];
+    if ((st = ensure_writers(p, nw))) return st;
+    hipLaunchKernelGGL(k_emit, dim3(gv), dim3(NT), 0, s, g, p->A, p->nn, p->spos, p->sg, p->wn, p->woff,
+                       p->keys[0], p->refs[0], p->n_rho, p->flags);
+    int end_bit = 1;
+    while (end_bit < 64 && ((unsigned long long)p->n_rho * (unsigned long long)g.NV) >> end_bit) end_bit++;
+    if (nw > 0) {
+        need = 0;
+        EVP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, p->keys[0], p->keys[1], p->refs[0], p->refs[1],
+                                                     (int)nw, 0, end_bit, s));
+        if ((st = ensure_tmp(p, need))) return st;
+        EVP_CHECK(hipcub::DeviceRadixSort::SortPairs(p->tmp, need, p->keys[0], p->keys[1], p->refs[0], p->refs[1],
+                                                     (int)nw, 0, end_bit, s));
+    }
+    p->st.rho_writes += nw;
+    // trigger fixed point
+    int *tin = p->trig0, *tout = p->trig1;
+    hipLaunchKernelGGL(k_clear, dim3(gv), dim3(NT), 0, s, g.NV, tin);
+    for (int pass = 0;; pass++) {
+        if (pass == MAX_PASSES) return EVPGPU_ERR_HIP;
+        hipLaunchKernelGGL(k_avg, dim3(gv), dim3(NT), 0, s, g, p->A, p->wn, p->spos, p->sg, p->gid, p->defeng, tin,
+                           p->avg, p->cdd);
+        hipLaunchKernelGGL(k_update, dim3(gv), dim3(NT), 0, s, g, p->A, p->B, P, p->gcnt, p->gidx, p->nn, p->spos,
+                           p->sg, p->avg, p->cdd, p->keys[1], p->refs[1], nw, p->rho_in, tout, p->flags);
+        if ((st = clear_flag(p, 1))) return st;
+        hipLaunchKernelGGL(k_diff, dim3(gv), dim3(NT), 0, s, g.NV, tin, tout, p->flags + 1);
+        if ((st = read_flags(p))) return st;
+        p->st.trigger_passes++;
+        if (p->h_flags[0]) return pf_err(p->h_flags[0]);
+        int *t = tin;
+        tin = tout;
+        tout = t;
+        if (!p->h_flags[1]) break;
+    }
+    hipLaunchKernelGGL(k_finish, dim3(gv), dim3(NT), 0, s, g, tin, p->defeng, p->keys[1], p->refs[1], nw, p->avg,
+                       p->cdd, p->rho_in);
+    hipLaunchKernelGGL(k_halo, dim3(gc), dim3(NT), 0, s, g, p->B, p->flags);
+    Lists t = p->A;
+    p->A = p->B;
+    p->B = t;
+    hipLaunchKernelGGL(k_clear, dim3(gc), dim3(NT), 0, s, g.NC, p->B.cnt);
+    hipLaunchKernelGGL(k_gid, dim3(gv), dim3(NT), 0, s, g, p->A, p->gid_new, p->flags);
+    hipLaunchKernelGGL(k_mesh, dim3(gc), dim3(NT), 0, s, g, p->defeng, p->gid);
+    if ((st = read_flags(p))) return st;
+    if (p->h_flags[0]) return pf_err(p->h_flags[0]);
+    p->st.updated += p->h_flags[2];
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+Coef coef_of(const evpgpu_pf_params *P, int n_rho)
+{
+    // src/pf_xrx.c:1497-1504
+    const double dx = P->pf_length_scale;
+    const double L_gb = 3.0 * dx;
+    const double dt = P->pf_time_step;
+    Coef c;
+    c.M_coeff = dt * 4.0 / 3.0 * P->M_gb / L_gb;
+    c.kappa = 3.0 / 4.0 * P->E_gb * L_gb / dx / dx;
+    c.m = 6 * P->E_gb / L_gb;
+    c.E_disl = P->alpha * P->mu * P->bb * P->bb;
+    c.init_gid = P->init_gid;
+    c.n_rho = n_rho;
+    return c;
+}
+
+} // namespace
+
+extern "C" int evpgpu_pf_create(evpgpu_pf **out, int Lx, int Ly, int Lz, int n_rho_avg)
+{
+    if (!out) return EVPGPU_ERR_ARG;
+    *out = nullptr;
+    if (Lx < 3 || Ly < 3 || Lz < 3 || Lx > 1024 || Ly > 1024 || Lz > 1024 || n_rho_avg < 1)
+        return EVPGPU_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EVPGPU_ERR_NODEV;
+    evpgpu_pf *p = (evpgpu_pf *)calloc(1, sizeof(evpgpu_pf));
+    if (!p) return EVPGPU_ERR_NOMEM;
+    Dims &g = p->g;
+    g.Lx = Lx;
+    g.Ly = Ly;
+    g.Lz = Lz;
+    g.lx = Lx + 6;
+    g.ly = Ly + 6;
+    g.lz = Lz + 6;
+    g.NC = (long long)g.lx * g.ly * g.lz;
+    g.NV = (long long)Lx * Ly * Lz;
+    if ((unsigned long long)g.NV * CAP > 0x7fffffffULL) {
+        free(p);
+        return EVPGPU_ERR_ARG;
+    }
+    p->n_rho = n_rho_avg;
+    int bad = 0;
+    for (Lists *L : {&p->A, &p->B}) {
+        bad |= dalloc(&L->cnt, g.NC);
+        bad |= dalloc(&L->idx, g.NC * CAP);
+        bad |= dalloc(&L->val, g.NC * CAP);
+    }
+    bad |= dalloc(&p->defeng, g.NC) | dalloc(&p->gid, g.NC);
+    bad |= dalloc(&p->gcnt, g.NV) | dalloc(&p->gidx, g.NV * CAP) | dalloc(&p->act, g.NV) | dalloc(&p->nn, g.NV);
+    bad |= dalloc(&p->spos, g.NV) | dalloc(&p->sg, g.NV) | dalloc(&p->wn, g.NV) | dalloc(&p->woff, g.NV);
+    bad |= dalloc(&p->trig0, g.NV) | dalloc(&p->trig1, g.NV) | dalloc(&p->gid_new, g.NV);
+    bad |= dalloc(&p->avg, g.NV * CAP) | dalloc(&p->cdd, g.NV * CAP);
+    bad |= dalloc(&p->rho_in, n_rho_avg) | dalloc(&p->flags, 3);
+    bad |= hipHostMalloc((void **)&p->h_flags, 3 * sizeof(int), 0) != hipSuccess;
+    bad |= hipHostMalloc((void **)&p->h_total, sizeof(long long), 0) != hipSuccess;
+    bad |= hipHostMalloc((void **)&p->h_last, sizeof(int), 0) != hipSuccess;
+    p->wcap = 0;
+    if (!bad) {
+        p->wcap = g.NV;
+        for (int i = 0; i < 2; i++) bad |= dalloc(&p->keys[i], p->wcap) | dalloc(&p->refs[i], p->wcap);
+    }
+    bad |= hipStreamCreate(&p->stream) != hipSuccess;
+    bad |= hipEventCreate(&p->ev0) != hipSuccess;
+    bad |= hipEventCreate(&p->ev1) != hipSuccess;
+    if (!bad) bad |= hipMemset(p->flags, 0, 3 * sizeof(int)) != hipSuccess;
+    if (!bad) bad |= hipMemset(p->A.cnt, 0, g.NC * sizeof(int)) != hipSuccess;
+    if (!bad) bad |= hipMemset(p->B.cnt, 0, g.NC * sizeof(int)) != hipSuccess;
+    if (bad) {
+        evpgpu_pf_destroy(p);
+        return EVPGPU_ERR_NOMEM;
+    }
+    *out = p;
+    return EVPGPU_OK;
+}
+
+extern "C" int evpgpu_pf_destroy(evpgpu_pf *p)
+{
+    if (!p) return EVPGPU_ERR_ARG;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (Lists *L : {&p->A, &p->B}) {
+        (void)hipFree(L->cnt);
+        (void)hipFree(L->idx);
+        (void)hipFree(L->val);
+    }
+    void *dev[] = {p->defeng, p->gid, p->gcnt, p->gidx, p->act, p->nn,    p->spos,    p->sg,      p->wn,
+                   p->woff,   p->trig0, p->trig1, p->gid_new, p->avg, p->cdd, p->rho_in, p->flags, p->keys[0],
+                   p->keys[1], p->refs[0], p->refs[1], p->tmp};
+    for (void *q : dev) (void)hipFree(q);
+    if (p->h_flags) (void)hipHostFree(p->h_flags);
+    if (p->h_total) (void)hipHostFree(p->h_total);
+    if (p->h_last) (void)hipHostFree(p->h_last);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    free(p);
+    return EVPGPU_OK;
+}
+
+extern "C" int evpgpu_pf_set_lists(evpgpu_pf *p, const int *n, const int *idx, const double *val, const int *GID,
+                                   const double *defeng)
+{
+    if (!p || !n || !idx || !val || !GID || !defeng) return EVPGPU_ERR_ARG;
+    const Dims &g = p->g;
+    int *dn = nullptr, *di = nullptr, *dg = nullptr;
+    double *dv = nullptr, *dd = nullptr;
+    int st = EVPGPU_OK;
+    if (dalloc(&dn, g.NV) || dalloc(&di, g.NV * CAP) || dalloc(&dg, g.NV) || dalloc(&dv, g.NV * CAP) ||
+        dalloc(&dd, g.NV))
+        st = EVPGPU_ERR_NOMEM;
+    hipStream_t s = p->stream;
+    if (!st) {
+        bool ok = hipMemcpyAsync(dn, n, g.NV * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(di, idx, g.NV * CAP * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(dv, val, g.NV * CAP * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(dg, GID, g.NV * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(dd, defeng, g.NV * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess;
+        if (!ok) st = EVPGPU_ERR_HIP;
+    }
+    if (!st) {
+        const int gv = grid_for(g.NV), gc = grid_for(g.NC);
+        hipLaunchKernelGGL(k_clear, dim3(gc), dim3(NT), 0, s, g.NC, p->A.cnt);
+        hipLaunchKernelGGL(k_clear, dim3(gc), dim3(NT), 0, s, g.NC, p->B.cnt);
+        hipLaunchKernelGGL(k_load, dim3(gv), dim3(NT), 0, s, g, p->A, dn, di, dv, dg, dd, p->defeng, p->gid,
+                           p->flags);
+        hipLaunchKernelGGL(k_halo, dim3(gc), dim3(NT), 0, s, g, p->A, p->flags);
+        hipLaunchKernelGGL(k_mesh, dim3(gc), dim3(NT), 0, s, g, p->defeng, p->gid);
+        st = read_flags(p);
+        if (!st && p->h_flags[0]) st = pf_err(p->h_flags[0]);
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dn);
+    (void)hipFree(di);
+    (void)hipFree(dg);
+    (void)hipFree(dv);
+    (void)hipFree(dd);
+    p->loaded = st == EVPGPU_OK;
+    return st;
+}
+
+extern "C" int evpgpu_pf_steps(evpgpu_pf *p, const evpgpu_pf_params *P, int steps, int *gID_new, double *rho_avg)
+{
+    if (!p || !P || steps < 0 || !rho_avg) return EVPGPU_ERR_ARG;
+    if (!p->loaded) return EVPGPU_ERR_STATE;
+    if (!(P->pf_length_scale > 0.0)) return EVPGPU_ERR_ARG;
+    hipStream_t s = p->stream;
+    EVP_CHECK(hipMemcpyAsync(p->rho_in, rho_avg, p->n_rho * sizeof(double), hipMemcpyHostToDevice, s));
+    const Coef C = coef_of(P, p->n_rho);
+    memset(&p->st, 0, sizeof(p->st));
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(NT), 0, s, 3LL, p->flags);
+    EVP_CHECK(hipEventRecord(p->ev0, s));
+    int st = EVPGPU_OK;
+    for (int it = 0; it < steps && !st; it++) {
+        st = substep(p, C);
+        p->st.substeps++;
+    }
+    EVP_CHECK(hipEventRecord(p->ev1, s));
+    EVP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
+    p->st.ms = ms;
+    if (st) return st;
+    EVP_CHECK(hipMemcpyAsync(rho_avg, p->rho_in, p->n_rho * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (gID_new && steps > 0)
+        EVP_CHECK(hipMemcpyAsync(gID_new, p->gid_new, p->g.NV * sizeof(int), hipMemcpyDeviceToHost, s));
+    EVP_CHECK(hipStreamSynchronize(s));
+    return EVPGPU_OK;
+}
+
+extern "C" int evpgpu_pf_xrx(evpgpu_pf *p, const evpgpu_pf_params *P, const int *gID, const double *rho, int steps,
+                             int *gID_new, double *rho_avg)
+{
+    if (!p || !P || !gID || !rho || !gID_new || !rho_avg || steps < 0) return EVPGPU_ERR_ARG;
+    const Dims &g = p->g;
+    hipStream_t s = p->stream;
+    int *dg = nullptr;
+    double *dr = nullptr;
+    if (dalloc(&dg, g.NV) || dalloc(&dr, g.NV)) {
+        (void)hipFree(dg);
+        return EVPGPU_ERR_NOMEM;
+    }
+    int st = EVPGPU_OK;
+    if (hipMemcpyAsync(dg, gID, g.NV * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dr, rho, g.NV * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+        st = EVPGPU_ERR_HIP;
+    if (!st) {
+        const int gv = grid_for(g.NV), gc = grid_for(g.NC);
+        hipLaunchKernelGGL(k_clear, dim3(gc), dim3(NT), 0, s, g.NC, p->A.cnt);
+        hipLaunchKernelGGL(k_clear, dim3(gc), dim3(NT), 0, s, g.NC, p->B.cnt);
+        hipLaunchKernelGGL(k_init, dim3(gv), dim3(NT), 0, s, g, p->A, dg, dr, p->defeng, p->gid);
+        hipLaunchKernelGGL(k_halo, dim3(gc), dim3(NT), 0, s, g, p->A, p->flags);
+        hipLaunchKernelGGL(k_mesh, dim3(gc), dim3(NT), 0, s, g, p->defeng, p->gid);
+        if (hipGetLastError() != hipSuccess) st = EVPGPU_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dg);
+    (void)hipFree(dr);
+    if (st) return st;
+    p->loaded = 1;
+    // gID_new starts as gID (src/pf_xrx.c:1551) and is rewritten by each substep
+    memcpy(gID_new, gID, g.NV * sizeof(int));
+    return evpgpu_pf_steps(p, P, steps, gID_new, rho_avg);
+}
+
+extern "C" int evpgpu_pf_get_lists(evpgpu_pf *p, int *n, int *idx, double *val, double *defeng)
+{
+    if (!p) return EVPGPU_ERR_ARG;
+    if (!p->loaded) return EVPGPU_ERR_STATE;
+    const Dims &g = p->g;
+    hipStream_t s = p->stream;
+    int *dn = nullptr, *di = nullptr;
+    double *dv = nullptr, *dd = nullptr;
+    if (dalloc(&dn, g.NV) || dalloc(&di, g.NV * CAP) || dalloc(&dv, g.NV * CAP) || dalloc(&dd, g.NV)) {
+        (void)hipFree(dn);
+        (void)hipFree(di);
+        (void)hipFree(dv);
+        return EVPGPU_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_store, dim3(grid_for(g.NV)), dim3(NT), 0, s, g, p->A, p->defeng, dn, di, dv, dd);
+    bool ok = hipGetLastError() == hipSuccess;
+    if (ok && n) ok = hipMemcpyAsync(n, dn, g.NV * sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (ok && idx) ok = hipMemcpyAsync(idx, di, g.NV * CAP * sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (ok && val) ok = hipMemcpyAsync(val, dv, g.NV * CAP * sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (ok && defeng) ok = hipMemcpyAsync(defeng, dd, g.NV * sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess;
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dn);
+    (void)hipFree(di);
+    (void)hipFree(dv);
+    (void)hipFree(dd);
+    return ok ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+extern "C" int evpgpu_pf_get_stats(evpgpu_pf *p, evpgpu_pf_stats *st)
+{
+    if (!p || !st) return EVPGPU_ERR_ARG;
+    *st = p->st;
+    return EVPGPU_OK;
+}

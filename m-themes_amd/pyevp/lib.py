@@ -94,6 +94,9 @@ EVPGPU_SYMBOLS = [
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
     "evpgpu_newton_host", "evpgpu_adjoint_record", "evpgpu_adjoint_run", "evpgpu_copy_probe",
     "evpgpu_set_dd_law", "evpgpu_download_dd_state", "evpgpu_dd_averages",
+    "evpgpu_pf_create", "evpgpu_pf_destroy", "evpgpu_pf_xrx", "evpgpu_pf_set_lists",
+    "evpgpu_pf_steps", "evpgpu_pf_get_lists", "evpgpu_pf_get_stats",
+    "evpgpu_drx_nucleate", "evpgpu_drx_weibull", "evpgpu_drx_uniform_pos",
 ]
 EVP_SYMBOLS = [
     "GetNameList", "PrintNameList", "OpenFiles", "CloseFiles", "SetupJob", "Evolution",

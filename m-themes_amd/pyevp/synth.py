@@ -387,3 +387,47 @@ def write_job(dirpath: str, cfg: Config) -> str:
     inp = os.path.join(dirpath, "input.in")
     write_input(inp, cfg)
     return inp
+
+
+# ------------------------------------------------------------ phase field --
+# Inputs of the phase-field recrystallisation step (PF_XRX, src/pf_xrx.c:1469):
+# the grain field and stored energy it starts from.  Parameters follow the
+# reference's test_run/input.in (E_gb 0.625, M_gb 7.4e-11, Scale_Fdeform 0.5,
+# pf_length_scale 2.5e-6, pf_time_step 0.004); mu and bb set the stored-energy
+# scale E_disl = alpha mu bb^2 (bb = 1 makes nuclei grow within a few substeps).
+PF_PARAMS = dict(E_gb=0.625, M_gb=7.4e-11, mu=4.0e4, bb=1.0, alpha=0.5, pf_length_scale=2.5e-6,
+                 pf_time_step=0.004)
+
+
+def pf_params(init_gid: int, **kw) -> dict:
+    p = dict(PF_PARAMS)
+    p.update(kw)
+    p["init_gid"] = int(init_gid)
+    return p
+
+
+def pf_inputs(shape, grains: int, nuclei: int, seed: int, radius2: int = 2):
+    """Periodic Voronoi grains 1..grains (uniform seeds), stored energy
+    U(200, 250), and `nuclei` spheres (squared radius radius2, periodic) of new
+    grains grains+1.. with the stored energy 8 per slip system x 12 = 96 that
+    the reference gives a nucleus (src/evolution.c:977-983).  Returns (grain
+    field int32, stored energy, parameters with init_gid = grains)."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(s) for s in shape)
+    L = np.array(shape, float)
+    seeds = rng.uniform(0, 1, (grains, 3)) * L
+    pts = np.indices(shape).reshape(3, -1).T + 0.5
+    _, near = cKDTree(seeds, boxsize=L).query(pts)
+    gid = (near + 1).astype(np.int32).reshape(shape)
+    rho = 200.0 + rng.uniform(0.0, 50.0, shape)
+    idx = np.indices(shape).reshape(3, -1).T
+    Li = np.array(shape)
+    for k in range(nuclei):
+        c = rng.integers(0, Li)
+        d = np.abs(idx - c)
+        d = np.minimum(d, Li - d)
+        m = ((d * d).sum(1) <= radius2).reshape(shape)
+        gid[m] = grains + 1 + k
+        rho[m] = 96.0
+    return gid, rho, pf_params(grains)

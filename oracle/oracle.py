@@ -1,4 +1,4 @@
-"""ctypes front end of the CPU oracle (oracle/evp_oracle.c).
+"""ctypes front end of the CPU oracle (oracle/evp_oracle.c, oracle/pf_oracle.c).
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, as the checker of the HIP path.
@@ -90,6 +90,17 @@ def lib():
         L.orc_dd_kT.restype = C.c_double
         L.orc_dd_kT.argtypes = [C.c_double]
         L.orc_strain_rate_dd.argtypes = [DP, DP, DP, C.c_int, DP, C.c_void_p] + [DP] * 8
+        PP = C.POINTER(PfParams)
+        L.orc_pf_coeffs.argtypes = [PP, DP]
+        L.orc_pf_xrx.argtypes = [C.c_int] * 3 + [PP, IP, DP, C.c_int, DP, C.c_int, IP, IP, IP, DP, DP]
+        L.orc_pf_run_lists.argtypes = ([C.c_int] * 3 + [PP, IP, IP, DP, IP, DP, C.c_int, DP, C.c_int,
+                                       IP, IP, IP, DP, DP])
+        L.orc_pf_lookup.argtypes = [IP, C.c_int, IP, DP, C.c_int]
+        L.orc_pf_laplacian.restype = C.c_double
+        L.orc_pf_laplacian.argtypes = [DP]
+        L.orc_gb_indicator.argtypes = [C.c_int] * 3 + [IP, IP]
+        L.orc_drx_nucleate.argtypes = ([C.c_int] * 4 + [IP] + [DP] * 6 + [C.c_int, C.c_int, C.c_double,
+                                       C.c_double, IP, IP, IP, DP, DP, C.POINTER(C.c_longlong), C.c_int, IP])
         _lib = L
     return _lib
 
@@ -389,3 +400,118 @@ def strain_rate_dd(stress6, sc, c1, c2, c3, G, q_slip, kT, drx, bbB, chi, cth, r
         *[ptr(np.ascontiguousarray(a, dtype=np.float64)) for a in (rho_s, rho_g1, rho_g2, rho_g3)],
         ptr(rm), ptr(rF), ptr(rP), ptr(gd))
     return st, edot, dedot, gd[:nsys], rm, rF[:nsys], rP[:nsys]
+
+
+# ---------------------------------------------------------------- phase field
+PF_CAP = 24  # ORC_PF_CAP
+PF_ERR_SUM, PF_ERR_CAP, PF_ERR_GRAIN = 1385, 2001, 2002
+PF_PARAM_NAMES = ("E_gb", "M_gb", "mu", "bb", "alpha", "pf_length_scale", "pf_time_step")
+
+
+class PfParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in PF_PARAM_NAMES] + [("init_gid", C.c_int)]
+
+
+def pf_params(d: dict) -> PfParams:
+    return PfParams(*[float(d[n]) for n in PF_PARAM_NAMES], int(d["init_gid"]))
+
+
+def pf_coeffs(d: dict) -> np.ndarray:
+    out = np.zeros(4)
+    lib().orc_pf_coeffs(C.byref(pf_params(d)), ptr(out))
+    return out
+
+
+def pf_laplacian(a) -> float:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.shape == (19,)
+    return lib().orc_pf_laplacian(ptr(a))
+
+
+def pf_lookup(idx: list, val: list, g: int, cap: int = PF_CAP):
+    """PFdata's non-const lookup on one list: (new idx, new val, position)."""
+    n = C.c_int(len(idx))
+    ia = np.zeros(cap, np.int32)
+    va = np.zeros(cap)
+    ia[:len(idx)] = idx
+    va[:len(val)] = val
+    k = lib().orc_pf_lookup(C.byref(n), cap, ptr(ia, IP), ptr(va), int(g))
+    return list(ia[:n.value]), list(va[:n.value]), k
+
+
+def _pf_out(shape, status, gid_new, n, idx, val, defeng, rho_avg):
+    return dict(status=status, gid=gid_new.reshape(shape), n=n.reshape(shape),
+                idx=idx.reshape(shape + (PF_CAP,)), val=val.reshape(shape + (PF_CAP,)),
+                defeng=defeng.reshape(shape), rho_avg=rho_avg)
+
+
+def pf_xrx(gid: np.ndarray, rho: np.ndarray, steps: int, params: dict, rho_avg: np.ndarray) -> dict:
+    """PF_XRX (src/pf_xrx.c:1469-1698) on one rank: lists from the grain
+    field, then `steps` substeps.  rho_avg is copied, not changed."""
+    shape = tuple(gid.shape)
+    g = np.ascontiguousarray(gid, dtype=np.int32)
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    ra = np.array(rho_avg, dtype=np.float64)
+    nv = g.size
+    gid_new = np.zeros(nv, np.int32)
+    n = np.zeros(nv, np.int32)
+    idx = np.zeros(nv * PF_CAP, np.int32)
+    val = np.zeros(nv * PF_CAP)
+    de = np.zeros(nv)
+    st = lib().orc_pf_xrx(shape[0], shape[1], shape[2], C.byref(pf_params(params)), ptr(g, IP), ptr(r),
+                          int(steps), ptr(ra), ra.size, ptr(gid_new, IP), ptr(n, IP), ptr(idx, IP),
+                          ptr(val), ptr(de))
+    return _pf_out(shape, st, gid_new, n, idx, val, de, ra)
+
+
+def pf_run_lists(n_in, idx_in, val_in, GID, defeng, steps: int, params: dict, rho_avg) -> dict:
+    """The substeps from given interior lists (n_in[v], idx_in/val_in[v, k])."""
+    shape = tuple(GID.shape)
+    nv = GID.size
+    ni = np.ascontiguousarray(n_in, dtype=np.int32).ravel()
+    ii = np.zeros((nv, PF_CAP), np.int32)
+    vi = np.zeros((nv, PF_CAP))
+    k = np.asarray(idx_in).reshape(nv, -1).shape[1]
+    ii[:, :k] = np.asarray(idx_in).reshape(nv, k)
+    vi[:, :k] = np.asarray(val_in).reshape(nv, k)
+    g = np.ascontiguousarray(GID, dtype=np.int32)
+    de_in = np.ascontiguousarray(defeng, dtype=np.float64)
+    ra = np.array(rho_avg, dtype=np.float64)
+    gid_new = np.zeros(nv, np.int32)
+    n = np.zeros(nv, np.int32)
+    idx = np.zeros(nv * PF_CAP, np.int32)
+    val = np.zeros(nv * PF_CAP)
+    de = np.zeros(nv)
+    st = lib().orc_pf_run_lists(shape[0], shape[1], shape[2], C.byref(pf_params(params)), ptr(ni, IP),
+                                ptr(ii, IP), ptr(vi), ptr(g, IP), ptr(de_in), int(steps), ptr(ra), ra.size,
+                                ptr(gid_new, IP), ptr(n, IP), ptr(idx, IP), ptr(val), ptr(de))
+    return _pf_out(shape, st, gid_new, n, idx, val, de, ra)
+
+
+def gb_indicator(grain: np.ndarray) -> np.ndarray:
+    g = np.ascontiguousarray(grain, dtype=np.int32)
+    out = np.zeros(g.shape, np.int32)
+    lib().orc_gb_indicator(*g.shape, ptr(g, IP), ptr(out, IP))
+    return out
+
+
+def drx_nucleate(grain, rho_s, rho_m, g1, g2, g3, kappa, n_grains, nucleus_radius, zeta_kc, k_c,
+                 gid_rex, kc, max_new=100000) -> dict:
+    """NucleationCheckDRX (shipped branch, first call); densities (nsys,) + grain.shape"""
+    g = np.ascontiguousarray(grain, dtype=np.int32)
+    arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho_s, rho_m, g1, g2, g3)]
+    nsys = arr[0].shape[0]
+    ka = np.ascontiguousarray(kappa, dtype=np.float64)
+    rex = np.array(gid_rex, dtype=np.int32)
+    kcv = np.array(kc, dtype=np.float64)
+    out = np.zeros(g.shape, np.int32)
+    nuc = np.zeros(g.shape, np.int32)
+    rt = np.zeros(g.shape)
+    vox = np.zeros(max(max_new, 1), np.int64)
+    n_new = C.c_int(0)
+    st = lib().orc_drx_nucleate(*g.shape, nsys, ptr(g, IP), *[ptr(a) for a in arr], ptr(ka), int(n_grains),
+                                int(nucleus_radius), float(zeta_kc), float(k_c), ptr(out, IP), ptr(rex, IP),
+                                ptr(nuc, IP), ptr(kcv), ptr(rt), vox.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                int(max_new), C.byref(n_new))
+    return dict(status=st, grain=out, gid_rex=rex, nuclei=nuc, kc=kcv, rho_tot=rt,
+                vox=vox[:min(n_new.value, max_new)], n_new=n_new.value)

@@ -115,9 +115,11 @@ def c4_job(d, p=None):
 
 def forced_szz(job, iters):
     out = []
-    for k in iters:
+    for j, k in enumerate(iters):
         job.run(1, forced=int(k))
         out.append(job.macro()["sig_avg"][2, 2])
+        if j % 5 == 4:
+            say(f"c4: {j + 1} forced increments")
     return np.array(out)
 
 
@@ -165,7 +167,9 @@ def test_c4_64_adjoint_gradient_vs_gpu_central_differences(c4, i):
         p = p0.copy()
         p[i] += sgn * h
         fj.set_params(0, p)
+        t0 = time.perf_counter()
         lv.append(np.sum((forced_szz(fj, c4["iters"]) - c4["target"]) ** 2))
+        say(f"c4: parameter {lib.PARAM_NAMES[i]} {'+-'[sgn < 0]}h forward {time.perf_counter() - t0:.1f} s")
     fd = (lv[0] - lv[1]) / (2 * h)
     g = c4["grad"][i]
     say(f"c4: parameter {lib.PARAM_NAMES[i]}: adjoint {g:.10e} FD {fd:.10e} rel {abs(g - fd) / abs(fd):.2e}")
