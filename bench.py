@@ -157,6 +157,68 @@ def run_adjoint(n, increments, ck):
                 grad=dict(zip(lib.PARAM_NAMES, [float(x) for x in g[0]])))
 
 
+def run_pf(steps, cpu_steps):
+    """The DRX configuration's phase-field step: PF_XRX on 128^3 voxels with
+    256 grains and 40 nuclei (squared radius 5), `steps` substeps from the
+    one-entry start; device time of the substeps (HIP events on the step's
+    stream)."""
+    from pyevp import pf as gpf
+    shape = (128, 128, 128)
+    gid, rho, prm = synth.pf_inputs(shape, 256, 40, 0x5EED0005, radius2=5)
+    n = int(np.prod(shape))
+    p = gpf.PhaseField(shape, 60000)
+    try:
+        p.pf_xrx(gid, rho, 1, prm, np.zeros(60000))  # warm-up
+        t0 = time.perf_counter()
+        gn, _ = p.pf_xrx(gid, rho, steps, prm, np.zeros(60000))
+        wall = time.perf_counter() - t0
+        st = p.stats()
+    finally:
+        p.close()
+    ms = st["ms"]
+    # roofline of the voxel-update kernel: algorithmic bytes per launch = per
+    # voxel its own list (4 B count + 12 B per entry), its candidate grains
+    # (4 B + 4 B per grain), its turn record (12 B), the averages of its
+    # entries on the update branch (12 B per entry), its new list (4 B + 12 B
+    # per entry) and its reset flag (4 B); list lengths and candidates at
+    # their means after the run, the update-branch share from the counts
+    lists = None
+    p = gpf.PhaseField(shape, 60000)
+    try:
+        p.pf_xrx(gid, rho, steps, prm, np.zeros(60000))
+        lists = p.lists()
+    finally:
+        p.close()
+    nbar = float(lists["n"].mean())
+    f_upd = st["updated"] / (n * steps)
+    per_voxel = 4 + 12 * nbar + 4 + 4 * nbar + 12 + 12 * nbar * f_upd + 4 + 12 * nbar + 4
+    launch_ms = st["ms_update"] / max(st["update_launches"], 1)
+    gbs = n * per_voxel / (launch_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_update", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": gbs / PEAK_HBM_GBS, "traffic": None, "avg_launch_ms": launch_ms,
+            "bytes_per_voxel": per_voxel, "mean_list_length": nbar, "update_branch_share": f_upd,
+            "note": "integer and branch work on lists of a few entries: bound by the latency of the "
+                    "neighbour-list gathers, not by HBM bandwidth; k_avg (7x7x7 averages through LDS) "
+                    f"{st['ms_avg'] / max(st['update_launches'], 1):.3f} ms per launch"}
+    rec = {"workload": f"PF_XRX 128^3, 256 Voronoi grains, 40 nuclei, {steps} substeps",
+           "value": n * steps / (ms * 1e-3), "unit": "voxel-substeps/s", "ms_per_substep": ms / steps,
+           "wall_s_with_transfers": wall, "stats": st, "roofline": roof,
+           "grains_changed": int((gn != gid).sum())}
+    log(f"bench: pf_xrx: {steps} substeps, {ms / steps:.3f} ms per substep, {rec['value']:.4g} voxel-substeps/s")
+    if cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        r = O.pf_xrx(gid, rho, cpu_steps, prm, np.zeros(60000))
+        dt = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": n * cpu_steps / dt, "unit": "voxel-substeps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"the same grid, first {cpu_steps} substeps ({dt:.1f} s)",
+                               "status": r["status"]}
+        log(f"bench: pf_xrx cpu: {rec['cpu_baseline']}")
+    return rec
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -221,6 +283,13 @@ def main():
     ap.add_argument("--dd-steps", type=int, default=2,
                     help="increments of the same job under the dislocation-density law, "
                          "reported beside the headline (0: skip)")
+    ap.add_argument("--pf-steps", type=int, default=10,
+                    help="phase-field substeps of the recrystallisation step at 128^3, 256 grains, "
+                         "40 nuclei, reported beside the headline (0: skip)")
+    ap.add_argument("--pf-cpu-steps", type=int, default=2,
+                    help="substeps of the oracle's phase-field CPU sample (0: skip)")
+    ap.add_argument("--pf-only", action="store_true",
+                    help="profiling mode: only the phase-field step, then exit")
     ap.add_argument("--law", choices=["power", "dd"], default="power",
                     help="law of the profiling mode's job")
     ap.add_argument("--profile-iters", type=int, default=0,
@@ -235,6 +304,9 @@ def main():
               "on one GPU (no multi-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.pf_only:
+        print(json.dumps({"pf_xrx": run_pf(max(args.pf_steps, 1), args.pf_cpu_steps)}), flush=True)
+        return
     if args.n == 256:
         cfg = synth.config_c3()     # 256^3, 4096 grains, no pores
     else:
@@ -402,6 +474,8 @@ def main():
                          "workload": f"the same {cfg.n}^3 job under StrainRate_Orowan + "
                                      "DislocationEvolution, increments 1.."
                                      f"{args.dd_steps}"}
+    if args.pf_steps > 0:
+        out["pf_xrx"] = run_pf(args.pf_steps, args.pf_cpu_steps)
     if args.adj_increments > 0:
         log(f"bench: adjoint: {args.adj_n}^3, {args.adj_increments} increments")
         out["adjoint"] = run_adjoint(args.adj_n, args.adj_increments, args.adj_checkpoint)

@@ -328,6 +328,9 @@ typedef struct {
     long long updated;        /* voxels on the update branch, summed */
     long long rho_writes;     /* rho_avg writes, summed */
     double ms;                /* device time of the last call's substeps */
+    double ms_update;         /* of which the voxel-update kernel (k_update) */
+    double ms_avg;            /* and the stored-energy averages (k_avg) */
+    int update_launches;      /* launches of each of those two */
 } evpgpu_pf_stats;
 
 /* One grid of Lx x Ly x Lz voxels (each >= 3) and an n_rho_avg-entry
@@ -335,7 +338,7 @@ typedef struct {
 int evpgpu_pf_create(evpgpu_pf **pf, int Lx, int Ly, int Lz, int n_rho_avg);
 int evpgpu_pf_destroy(evpgpu_pf *pf);
 
-/* PF_XRX (src/pf_xrx.c:1469) without its MPI arguments: each voxel's list
+/* PF_XRX (src/pf_xrx.h:49, src/pf_xrx.c:1469) without its MPI arguments: each voxel's list
  * starts as (gID, 1.0), the stored energy as rho, then `steps` substeps of
  * Update_grid, Update_gID, mesh_extend and the GetStatistics check.
  * gID_new receives Update_gID's grain field; rho_avg (n_rho_avg) is read and
@@ -355,7 +358,7 @@ int evpgpu_pf_steps(evpgpu_pf *pf, const evpgpu_pf_params *P, int steps, int *gI
 int evpgpu_pf_get_lists(evpgpu_pf *pf, int *n, int *idx, double *val, double *defeng);
 int evpgpu_pf_get_stats(evpgpu_pf *pf, evpgpu_pf_stats *stats);
 
-/* NucleationCheckDRX (src/integrated.c:161-494) in the branch the shipped
+/* NucleationCheckDRX (src/evp.h:649, src/integrated.c:161-494) in the branch the shipped
  * build compiles (GB_BULGING_ONLY and Nucl_Static_Flag 1), on its first call,
  * one GPU: a voxel at a grain boundary (GB_indicator, src/evolution.c:816-828,
  * from the neighbour checks of src/constitutive.c:790-853) nucleates when

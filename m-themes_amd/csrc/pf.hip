@@ -33,6 +33,7 @@
 // IEEE double in the reference's operation order: this file is built with
 // -ffp-contract=off.
 #include <hipcub/hipcub.hpp>
+#include <limits.h>
 
 #include "evpgpu_impl.h"
 
@@ -80,37 +81,46 @@ __device__ inline long long vid(const Dims &g, int x, int y, int z)
     return ((long long)(x - 3) * g.Ly + (y - 3)) * g.Lz + (z - 3);
 }
 
-// value a PFdata lookup of grain gr returns from cell c's start-of-sweep list
-__device__ inline double lookupv(const Dims &g, const Lists &L, long long c, int gr)
-{
-    const int n = L.cnt[c];
-    if (n == 0) return 0.0;
-    const double back = L.val[(long long)(n - 1) * g.NC + c];
-    const bool relabel = back < PF_EPS;
-    const int nf = relabel ? n - 1 : n;
-    for (int k = 0; k < nf; k++)
-        if (L.idx[(long long)k * g.NC + c] == gr) return L.val[(long long)k * g.NC + c];
-    return relabel ? back : 0.0;
-}
-
-// a list being built by assignments list[gr] = v of distinct grains (the
-// temp lists of Update_grid, the halo copies of boundary_conditions)
+// A list being built by assignments list[gr] = v of distinct grains (the
+// temp lists of Update_grid, the halo copies of boundary_conditions),
+// written straight into the structure-of-arrays lists at cell c: an
+// assignment only ever changes the last entry (a re-label when it is below
+// EPSILON) or appends one, so every entry but the last is final and only
+// the last is held in registers.
 struct Build {
-    int n;
-    int idx[CAP];
-    double val[CAP];
+    int *cnt, *gidx;
+    double *gval;
+    long long NC, c;
+    int n, last_idx;
+    double last_val;
+    __device__ Build(const Lists &L, long long NC_, long long c_)
+        : cnt(L.cnt), gidx(L.idx), gval(L.val), NC(NC_), c(c_), n(0), last_idx(0), last_val(0.0)
+    {
+    }
     __device__ bool put(int gr, double v)
     {
-        if (n > 0 && val[n - 1] < PF_EPS) {
-            idx[n - 1] = gr;
-            val[n - 1] = v;
+        if (n > 0 && last_val < PF_EPS) {
+            last_idx = gr;
+            last_val = v;
             return true;
         }
         if (n >= CAP) return false;
-        idx[n] = gr;
-        val[n] = v;
+        if (n > 0) {
+            gidx[(long long)(n - 1) * NC + c] = last_idx;
+            gval[(long long)(n - 1) * NC + c] = last_val;
+        }
+        last_idx = gr;
+        last_val = v;
         n++;
         return true;
+    }
+    __device__ void finish()
+    {
+        if (n > 0) {
+            gidx[(long long)(n - 1) * NC + c] = last_idx;
+            gval[(long long)(n - 1) * NC + c] = last_val;
+        }
+        cnt[c] = n;
     }
 };
 
@@ -119,47 +129,6 @@ struct Build {
 // -LyLz, -LyLz-1, -LyLz-Lz for lengths >= 3)
 __constant__ int EARLY[9][3] = {{0, 0, -1},  {0, -1, 1}, {0, -1, 0}, {0, -1, -1}, {-1, 1, 0},
                                 {-1, 0, 1},  {-1, 0, 0}, {-1, 0, -1}, {-1, -1, 0}};
-
-// candidate grains: values above EPSILON in the 3x3x3 neighbourhood, in
-// order of first appearance (src/pf_xrx.c:940-975)
-__global__ void k_glist(Dims g, Lists A, int *gcnt, int *gidx, int *err)
-{
-    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
-        int x, y, z;
-        vox(g, d, x, y, z);
-        int G[CAP];
-        int nG = 0;
-        bool over = false;
-        for (int i = x - 1; i <= x + 1; ++i)
-            for (int j = y - 1; j <= y + 1; ++j)
-                for (int m = z - 1; m <= z + 1; ++m) {
-                    const long long c = cix(g, i, j, m);
-                    const int n = A.cnt[c];
-                    for (int k = 0; k < n; ++k)
-                        if (A.val[(long long)k * g.NC + c] > PF_EPS) {
-                            const int index = A.idx[(long long)k * g.NC + c];
-                            int it = 0;
-                            while (it < nG && G[it] != index) it++;
-                            if (it == nG) {
-                                if (nG == CAP) over = true;
-                                else G[nG++] = index;
-                            }
-                        }
-                }
-        if (over) atomicOr(err, PFE_CAP);
-        gcnt[d] = nG;
-        for (int k = 0; k < nG; k++) gidx[(long long)k * g.NV + d] = G[k];
-    }
-}
-
-__global__ void k_act_init(Dims g, Lists A, int *act)
-{
-    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
-        int x, y, z;
-        vox(g, d, x, y, z);
-        act[d] = A.cnt[cix(g, x, y, z)] >= 2;
-    }
-}
 
 // does neighbour u (interior voxel) look up a grain outside {own}?
 __device__ inline bool has_other(const Dims &g, const int *gcnt, const int *gidx, long long u, int own)
@@ -170,28 +139,90 @@ __device__ inline bool has_other(const Dims &g, const int *gcnt, const int *gidx
     return false;
 }
 
-// one pass of the update-branch fixed point: a one-entry list above EPSILON
-// joins when an earlier update-branch neighbour looks up another grain
-__global__ void k_act_iter(Dims g, Lists A, const int *gcnt, const int *gidx, int *act, int *changed)
+// Start of the update-branch fixed point: the voxels whose list holds two
+// entries or more, and for each one-entry list above EPSILON the mask of its
+// nine earlier neighbours (EARLY order) whose candidate grains include
+// another grain -- the neighbours whose joining makes it join.
+__global__ __launch_bounds__(NT) void k_act_init(Dims g, Lists A, const int *gcnt, const int *gidx, int *act,
+                                                 int *emask)
 {
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
-        if (act[d]) continue;
         int x, y, z;
         vox(g, d, x, y, z);
         const long long c = cix(g, x, y, z);
-        if (A.cnt[c] != 1 || !(A.val[c] >= PF_EPS)) continue;
-        const int own = A.idx[c];
-        for (int e = 0; e < 9; e++) {
-            const int ux = x + EARLY[e][0], uy = y + EARLY[e][1], uz = z + EARLY[e][2];
-            if (!interior(g, ux, uy, uz)) continue;
-            const long long u = vid(g, ux, uy, uz);
-            if (act[u] && has_other(g, gcnt, gidx, u, own)) {
-                act[d] = 1;
-                *changed = 1;
-                break;
+        const int n = A.cnt[c];
+        act[d] = n >= 2;
+        int mask = 0;
+        if (n == 1 && A.val[c] >= PF_EPS) {
+            const int own = A.idx[c];
+            for (int e = 0; e < 9; e++) {
+                const int ux = x + EARLY[e][0], uy = y + EARLY[e][1], uz = z + EARLY[e][2];
+                if (interior(g, ux, uy, uz) && has_other(g, gcnt, gidx, vid(g, ux, uy, uz), own)) mask |= 1 << e;
             }
         }
+        emask[d] = mask;
     }
+}
+
+// One pass of the update-branch fixed point: a one-entry list above
+// EPSILON joins when an earlier update-branch neighbour looks up another
+// grain.  A workgroup of 16 waves owns a 4 x 4 tile of z rows, one wave per
+// row in chunks of 64 voxels: the eight earlier neighbours in other rows give
+// each voxel a "generate" bit, the row predecessor a "propagate" bit, and the
+// chain along the row (a_z = gen_z | prop_z a_(z-1)) is resolved by a
+// doubling scan over the wave's ballots; the workgroup repeats this until its
+// tile stops changing, so only chains that leave a tile take further passes.
+constexpr int ACT_NT = 1024, ACT_T = 4;
+
+__global__ __launch_bounds__(ACT_NT) void k_act_iter(Dims g, const int *emask, int *act, int *changed)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntx = (g.Lx + ACT_T - 1) / ACT_T;
+    const int bx = blockIdx.x % ntx, by = blockIdx.x / ntx;
+    const int x = 3 + bx * ACT_T + w / ACT_T, y = 3 + by * ACT_T + w % ACT_T;
+    const bool row_ok = x < g.lx - 3 && y < g.ly - 3; // wave-uniform
+    bool any = false;
+    for (;;) {
+        bool moved = false;
+        if (row_ok) {
+            unsigned long long carry = 0;
+            for (int z0 = 0; z0 < g.Lz; z0 += 64) {
+                const int z = z0 + lane + 3;
+                const bool in = z0 + lane < g.Lz;
+                bool cur = false, gen = false, prop = false;
+                long long d = 0;
+                if (in) {
+                    d = vid(g, x, y, z);
+                    cur = __atomic_load_n(&act[d], __ATOMIC_RELAXED) != 0;
+                    gen = cur;
+                    const int mask = emask[d];
+                    if (!cur && mask) {
+                        for (int e = 1; e < 9 && !gen; e++)
+                            if ((mask >> e) & 1)
+                                gen = __atomic_load_n(&act[vid(g, x + EARLY[e][0], y + EARLY[e][1], z + EARLY[e][2])],
+                                                      __ATOMIC_RELAXED) != 0;
+                        prop = !gen && (mask & 1);
+                    }
+                }
+                unsigned long long G = __ballot(gen), P = __ballot(prop);
+#pragma unroll
+                for (int sft = 1; sft < 64; sft <<= 1) {
+                    G |= P & (G << sft);
+                    P &= (P << sft) | ((1ull << sft) - 1ull);
+                }
+                const unsigned long long res = G | (carry ? P : 0ull);
+                if (in && !cur && ((res >> lane) & 1ull)) {
+                    __atomic_store_n(&act[d], 1, __ATOMIC_RELAXED);
+                    moved = true;
+                }
+                const int last = g.Lz - 1 - z0 < 63 ? g.Lz - 1 - z0 : 63;
+                carry = (res >> last) & 1ull;
+            }
+        }
+        any = any || moved;
+        if (!__syncthreads_or(moved)) break;
+    }
+    if (any) *changed = 1;
 }
 
 // The list at the voxel's turn: its start entries, with the last entry
@@ -199,7 +230,7 @@ __global__ void k_act_iter(Dims g, Lists A, const int *gcnt, const int *gidx, in
 // last grain outside the fixed entries that an earlier update-branch
 // neighbour looked up.  nn: entries; spos: position of that entry or -1;
 // sg: its grain; wn: rho_avg writes (nn on the update branch).
-__global__ void k_slot(Dims g, Lists A, const int *gcnt, const int *gidx, const int *act, int *nn, int *spos,
+__global__ __launch_bounds__(NT) void k_slot(Dims g, Lists A, const int *gcnt, const int *gidx, const int *act, int *nn, int *spos,
                        int *sg, int *wn, int *nupd, int *err)
 {
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
@@ -260,7 +291,7 @@ __device__ inline void entry(const Dims &g, const Lists &A, long long c, int k, 
 }
 
 // rho_avg writes as (grain - 1, sweep position) keys
-__global__ void k_emit(Dims g, Lists A, const int *nn, const int *spos, const int *sg, const int *wn,
+__global__ __launch_bounds__(NT) void k_emit(Dims g, Lists A, const int *nn, const int *spos, const int *sg, const int *wn,
                        const long long *woff, unsigned long long *keys, int *refs, int n_rho, int *err)
 {
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
@@ -285,56 +316,222 @@ __global__ void k_emit(Dims g, Lists A, const int *nn, const int *spos, const in
     }
 }
 
-// stored energy of cell p as voxel d's average reads it: the reset to 100 of
-// an earlier voxel of this sweep (trig), else the buffer
-__device__ inline double defeng_at(const Dims &g, const double *defeng, const int *trig, int i, int j, int m,
-                                   long long d)
+// Avg and count_dd of each list entry over the 7x7x7 box (src/pf_xrx.c:1042-1077).
+// A workgroup takes a 4 x 8 x 8 tile of voxels and stages the grain field,
+// the stored energy and the reset flags of the tile's box (10 x 14 x 14 cells)
+// in LDS.  The stored energy a voxel reads at a cell is 100 where an earlier
+// voxel of this sweep reset it (trig), else the buffer's.  With tprev (the
+// resets the previous pass used) a tile whose box saw no change keeps its
+// averages.
+constexpr int TX = 4, TY = 8, TZ = 8, HX = TX + 6, HY = TY + 6, HZ = TZ + 6, HC = HX * HY * HZ;
+static_assert(TX * TY * TZ == NT, "one voxel per thread");
+
+// The lists of a 4 x 8 x 8 voxel tile and of its face/edge neighbours (a
+// 6 x 10 x 10 cell box) staged in LDS: counts and the first LE entries;
+// entries beyond LE (rare) are read from memory.  31 KB.
+constexpr int LX = TX + 2, LY = TY + 2, LZ = TZ + 2, LC = LX * LY * LZ, LE = 4;
+struct ListTile {
+    int cnt[LC];
+    int idx[LE * LC];
+    double val[LE * LC];
+};
+
+// tile b's first voxel in padded coordinates
+__device__ inline void tile_origin(const Dims &g, int b, int &x0, int &y0, int &z0)
 {
-    if (interior(g, i, j, m)) {
-        const long long q = vid(g, i, j, m);
-        if (q < d && trig[q]) return 100.0;
-    }
-    return defeng[cix(g, i, j, m)];
+    const int ntx = (g.Lx + TX - 1) / TX, nty = (g.Ly + TY - 1) / TY;
+    x0 = 3 + (b % ntx) * TX;
+    y0 = 3 + ((b / ntx) % nty) * TY;
+    z0 = 3 + (b / (ntx * nty)) * TZ;
 }
 
-// Avg and count_dd of each list entry over the 7x7x7 box (src/pf_xrx.c:1042-1077)
-__global__ void k_avg(Dims g, Lists A, const int *wn, const int *spos, const int *sg, const int *gid,
-                      const double *defeng, const int *trig, double *avg, int *cdd)
+__device__ inline void load_lists(ListTile &T, const Dims &g, const Lists &A, int x0, int y0, int z0)
 {
-    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
-        const int n = wn[d];
-        if (!n) continue;
-        int x, y, z;
-        vox(g, d, x, y, z);
-        const long long c = cix(g, x, y, z);
-        int gk[CAP];
-        double a[CAP];
-        int cn[CAP];
-        for (int k = 0; k < n; k++) {
-            double v;
-            entry(g, A, c, k, spos[d], sg[d], gk[k], v);
-            a[k] = 0.0;
-            cn[k] = 0;
-        }
-        for (int i = x - 3; i <= x + 3; ++i)
-            for (int j = y - 3; j <= y + 3; ++j)
-                for (int m = z - 3; m <= z + 3; ++m) {
-                    const int gp = gid[cix(g, i, j, m)];
-                    bool hit = false;
-                    for (int k = 0; k < n; k++) hit = hit || gp == gk[k];
-                    if (!hit) continue;
-                    const double e = defeng_at(g, defeng, trig, i, j, m, d);
-                    for (int k = 0; k < n; k++)
-                        if (gp == gk[k]) {
-                            a[k] += e;
-                            cn[k] += 1;
-                        }
+    for (int q = threadIdx.x; q < LC; q += NT) {
+        const int hz = q % LZ, hy = (q / LZ) % LY, hx = q / (LZ * LY);
+        const int i = x0 - 1 + hx, j = y0 - 1 + hy, m = z0 - 1 + hz;
+        int n = 0;
+        if (i < g.lx && j < g.ly && m < g.lz) {
+            const long long c = cix(g, i, j, m);
+            n = A.cnt[c];
+#pragma unroll
+            for (int k = 0; k < LE; k++)
+                if (k < n) {
+                    T.idx[k * LC + q] = A.idx[(long long)k * g.NC + c];
+                    T.val[k * LC + q] = A.val[(long long)k * g.NC + c];
                 }
-        for (int k = 0; k < n; k++) {
+        }
+        T.cnt[q] = n;
+    }
+}
+
+// tile cell of the voxel at tile offset (tx, ty, tz) plus (dx, dy, dz)
+__device__ inline int tq(int tx, int ty, int tz, int dx, int dy, int dz)
+{
+    return ((tx + 1 + dx) * LY + (ty + 1 + dy)) * LZ + (tz + 1 + dz);
+}
+
+__device__ inline int tile_idx(const ListTile &T, const Dims &g, const Lists &A, int q, long long c, int k)
+{
+    return k < LE ? T.idx[k * LC + q] : A.idx[(long long)k * g.NC + c];
+}
+
+__device__ inline double tile_val(const ListTile &T, const Dims &g, const Lists &A, int q, long long c, int k)
+{
+    return k < LE ? T.val[k * LC + q] : A.val[(long long)k * g.NC + c];
+}
+
+// the value a PFdata lookup of grain gr returns from a start-of-sweep list
+// (tile cell q, memory cell c): its value when gr is an entry other than a
+// last entry below EPSILON, else that last entry's value when below, else 0
+__device__ inline double lookup_t(const ListTile &T, const Dims &g, const Lists &A, int q, long long c, int gr)
+{
+    const int n = T.cnt[q];
+    if (n == 0) return 0.0;
+    const double back = tile_val(T, g, A, q, c, n - 1);
+    const bool relabel = back < PF_EPS;
+    const int nf = relabel ? n - 1 : n;
+    for (int k = 0; k < nf; k++)
+        if (tile_idx(T, g, A, q, c, k) == gr) return tile_val(T, g, A, q, c, k);
+    return relabel ? back : 0.0;
+}
+
+// candidate grains: values above EPSILON in the 3x3x3 neighbourhood, in
+// order of first appearance (src/pf_xrx.c:940-975); lists from the tile
+__global__ __launch_bounds__(NT) void k_glist(Dims g, Lists A, int *gcnt, int *gidx, int *err)
+{
+    __shared__ ListTile T;
+    int x0, y0, z0;
+    tile_origin(g, blockIdx.x, x0, y0, z0);
+    load_lists(T, g, A, x0, y0, z0);
+    __syncthreads();
+    const int tz = threadIdx.x % TZ, ty = (threadIdx.x / TZ) % TY, tx = threadIdx.x / (TZ * TY);
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    if (!interior(g, x, y, z)) return;
+    const long long d = vid(g, x, y, z);
+    int G[CAP];
+    int nG = 0;
+    bool over = false;
+    for (int i = -1; i <= 1; ++i)
+        for (int j = -1; j <= 1; ++j)
+            for (int m = -1; m <= 1; ++m) {
+                const int q = tq(tx, ty, tz, i, j, m);
+                const long long c = cix(g, x + i, y + j, z + m);
+                const int n = T.cnt[q];
+                for (int k = 0; k < n; ++k)
+                    if (tile_val(T, g, A, q, c, k) > PF_EPS) {
+                        const int index = tile_idx(T, g, A, q, c, k);
+                        bool seen = false;
+#pragma unroll
+                        for (int it = 0; it < CAP; it++) seen = seen || (it < nG && G[it] == index);
+                        if (!seen) {
+                            if (nG == CAP) {
+                                over = true;
+                            } else {
+#pragma unroll
+                                for (int it = 0; it < CAP; it++)
+                                    if (it == nG) G[it] = index;
+                                nG++;
+                            }
+                        }
+                    }
+            }
+    if (over) atomicOr(err, PFE_CAP);
+    gcnt[d] = nG;
+#pragma unroll
+    for (int k = 0; k < CAP; k++)
+        if (k < nG) gidx[(long long)k * g.NV + d] = G[k];
+}
+
+
+// the sums of one voxel over its box, for lists of at most M entries (the
+// entries' grains, sums and counts stay in registers)
+template <int M>
+__device__ inline void avg_box(const Dims &g, const Lists &A, long long c, long long d, int n, int p, int s, int tx,
+                               int ty, int tz, int x, int y, int z, const int *s_gid, const double *s_de,
+                               const unsigned char *s_tf, double *avg, int *cdd)
+{
+    int gk[M];
+    double a[M];
+    int cn[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        gk[k] = INT_MIN; // matches no grain, nor the -1 of cells outside the grid
+        if (k < n) {
+            double v;
+            entry(g, A, c, k, p, s, gk[k], v);
+        }
+        a[k] = 0.0;
+        cn[k] = 0;
+    }
+    for (int i = 0; i < 7; ++i)
+        for (int j = 0; j < 7; ++j)
+            for (int m = 0; m < 7; ++m) {
+                const int q = ((tx + i) * HY + (ty + j)) * HZ + (tz + m);
+                const int gp = s_gid[q];
+                bool hit = false;
+#pragma unroll
+                for (int k = 0; k < M; k++) hit = hit || gp == gk[k];
+                if (!hit) continue;
+                double e = s_de[q];
+                if (s_tf[q] && vid(g, x - 3 + i, y - 3 + j, z - 3 + m) < d) e = 100.0;
+#pragma unroll
+                for (int k = 0; k < M; k++)
+                    if (gp == gk[k]) {
+                        a[k] += e;
+                        cn[k] += 1;
+                    }
+            }
+#pragma unroll
+    for (int k = 0; k < M; k++)
+        if (k < n) {
             avg[(long long)k * g.NV + d] = a[k];
             cdd[(long long)k * g.NV + d] = cn[k];
         }
+}
+
+__global__ __launch_bounds__(NT) void k_avg(Dims g, Lists A, const int *wn, const int *spos, const int *sg,
+                                            const int *gid, const double *defeng, const int *trig,
+                                            const int *tprev, double *avg, int *cdd)
+{
+    __shared__ int s_gid[HC];
+    __shared__ double s_de[HC];
+    __shared__ unsigned char s_tf[HC];
+    const int ntx = (g.Lx + TX - 1) / TX, nty = (g.Ly + TY - 1) / TY;
+    const int b = blockIdx.x;
+    const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+    // padded coordinates of the tile's first voxel and of its box
+    const int x0 = 3 + bx * TX, y0 = 3 + by * TY, z0 = 3 + bz * TZ;
+    int changed = tprev ? 0 : 1;
+    for (int q = threadIdx.x; q < HC; q += NT) {
+        const int hz = q % HZ, hy = (q / HZ) % HY, hx = q / (HZ * HY);
+        const int i = x0 - 3 + hx, j = y0 - 3 + hy, m = z0 - 3 + hz;
+        int gg = -1, tf = 0;
+        double e = 0.0;
+        if (i < g.lx && j < g.ly && m < g.lz) {
+            const long long c = cix(g, i, j, m);
+            gg = gid[c];
+            e = defeng[c];
+            if (interior(g, i, j, m)) {
+                const long long v = vid(g, i, j, m);
+                tf = trig[v];
+                if (tprev && tprev[v] != tf) changed = 1;
+            }
+        }
+        s_gid[q] = gg;
+        s_de[q] = e;
+        s_tf[q] = (unsigned char)tf;
     }
+    if (!__syncthreads_or(changed)) return;
+    const int tz = threadIdx.x % TZ, ty = (threadIdx.x / TZ) % TY, tx = threadIdx.x / (TZ * TY);
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    if (!interior(g, x, y, z)) return;
+    const long long d = vid(g, x, y, z);
+    const int n = wn[d];
+    if (!n) return;
+    const long long c = cix(g, x, y, z);
+    if (n <= 4) avg_box<4>(g, A, c, d, n, spos[d], sg[d], tx, ty, tz, x, y, z, s_gid, s_de, s_tf, avg, cdd);
+    else avg_box<CAP>(g, A, c, d, n, spos[d], sg[d], tx, ty, tz, x, y, z, s_gid, s_de, s_tf, avg, cdd);
 }
 
 __device__ inline double writer_value(const Dims &g, const double *avg, const int *cdd, int ref)
@@ -361,38 +558,42 @@ __device__ inline double rho_at(const Dims &g, const unsigned long long *keys, c
     return rho_in[gr - 1];
 }
 
-__device__ inline double laplacian(const double a[19])
-{
-    // laplacian() (src/pf_xrx.c:6-9), arguments in the call's order
-    return (0.5 * (a[0] + a[1] + a[2] + a[3] + a[4] + a[5]) +
-            0.125 * (a[10] + a[12] + a[11] + a[13] + a[7] + a[6] + a[8] + a[9] + a[14] + a[17] + a[15] + a[16]) -
-            4.5 * a[18]);
-}
 
 struct Coef {
     double M_coeff, kappa, m, E_disl;
     int init_gid, n_rho;
 };
 
-// the neighbour cells of the laplacian call, in its argument order
-__constant__ int LAPN[18][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
-                                {-1, 0, 1}, {-1, 0, -1}, {-1, -1, 0}, {-1, 1, 0}, {0, -1, 1}, {0, 1, 1},
-                                {0, -1, -1}, {0, 1, -1}, {1, 0, 1},  {1, 0, -1}, {1, -1, 0}, {1, 1, 0}};
+// The cells of the laplacian call (src/pf_xrx.c:1150-1160) in the order its
+// body sums them: the faces right, left, front, back, up, down; then the
+// edges in the order laplacian() adds its parameters, whose names 7-18 do
+// not match the arguments: center_up_front, center_down_front,
+// center_up_back, center_down_back, left_down, left_up, left_front,
+// left_back, right_up, right_back, right_down, right_front.
+__constant__ int LAPQ[18][3] = {{1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0}, {0, 0, 1},   {0, 0, -1},
+                                {0, -1, 1},  {0, -1, -1}, {0, 1, 1},   {0, 1, -1}, {-1, 0, -1}, {-1, 0, 1},
+                                {-1, -1, 0}, {-1, 1, 0},  {1, 0, 1},   {1, 1, 0},  {1, 0, -1},  {1, -1, 0}};
 
 // the voxel update of src/pf_xrx.c:1034-1246 into the temp lists B
-__global__ void k_update(Dims g, Lists A, Lists B, Coef P, const int *gcnt, const int *gidx, const int *nn,
+__global__ __launch_bounds__(NT) void k_update(Dims g, Lists A, Lists B, Coef P, const int *gcnt, const int *gidx, const int *nn,
                          const int *spos, const int *sg, const double *avg, const int *cdd,
                          const unsigned long long *keys, const int *refs, long long nw, const double *rho_in,
                          int *trig_out, int *err)
 {
-    for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
-        int x, y, z;
-        vox(g, d, x, y, z);
+    __shared__ ListTile T;
+    int x0, y0, z0;
+    tile_origin(g, blockIdx.x, x0, y0, z0);
+    load_lists(T, g, A, x0, y0, z0);
+    __syncthreads();
+    const int tz = threadIdx.x % TZ, ty = (threadIdx.x / TZ) % TY, tx = threadIdx.x / (TZ * TY);
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    if (interior(g, x, y, z)) {
+        const long long d = vid(g, x, y, z);
         const long long c = cix(g, x, y, z);
+        const int q0 = tq(tx, ty, tz, 0, 0, 0);
         const int NN = nn[d];
         const int nG = gcnt[d];
-        Build t;
-        t.n = 0;
+        Build t(B, g.NC, c);
         bool ok = true;
         int trig = 0;
         if (NN > 1) {
@@ -418,15 +619,23 @@ __global__ void k_update(Dims g, Lists A, Lists B, Coef P, const int *gcnt, cons
                     atomicOr(err, PFE_GRAIN);
                     break;
                 }
-                const double s = lookupv(g, A, c, index);
+                const double s = lookup_t(T, g, A, q0, c, index);
                 const double h_phi = s * s * s * (10.0 - 15.0 * s + 6.0 * s * s);
                 const double dh_phi = s * s * (30.0 - 60.0 * s + 30.0 * s * s);
                 const double center_rex = s;
-                double a[19];
-                for (int q = 0; q < 18; q++)
-                    a[q] = lookupv(g, A, cix(g, x + LAPN[q][0], y + LAPN[q][1], z + LAPN[q][2]), index);
-                a[18] = center_rex;
-                const double lap_rex = laplacian(a);
+                // laplacian() (src/pf_xrx.c:6-9): faces summed in the call's
+                // order, edges in the order its permuted parameter names sum
+                // them (LAPQ)
+                double f = 0.0, e = 0.0;
+#pragma unroll 1
+                for (int q = 0; q < 18; q++) {
+                    const int dx = LAPQ[q][0], dy = LAPQ[q][1], dz = LAPQ[q][2];
+                    const double v =
+                        lookup_t(T, g, A, tq(tx, ty, tz, dx, dy, dz), cix(g, x + dx, y + dy, z + dz), index);
+                    if (q < 6) f = q == 0 ? v : f + v;
+                    else e = q == 6 ? v : e + v;
+                }
+                const double lap_rex = (0.5 * f + 0.125 * e) - 4.5 * center_rex;
                 const double ra = rho_at(g, keys, refs, nw, avg, cdd, rho_in, index, d);
                 if (ra > 0.0) {
                     const double cc = center_rex;
@@ -442,20 +651,16 @@ __global__ void k_update(Dims g, Lists A, Lists B, Coef P, const int *gcnt, cons
         } else if (NN == 1) {
             for (int it = 0; it < nG; it++) {
                 const int index = gidx[(long long)it * g.NV + d];
-                ok = ok && t.put(index, lookupv(g, A, c, index));
+                ok = ok && t.put(index, lookup_t(T, g, A, q0, c, index));
             }
         }
         if (!ok) atomicOr(err, PFE_CAP);
-        B.cnt[c] = t.n;
-        for (int k = 0; k < t.n; k++) {
-            B.idx[(long long)k * g.NC + c] = t.idx[k];
-            B.val[(long long)k * g.NC + c] = t.val[k];
-        }
+        t.finish();
         trig_out[d] = trig;
     }
 }
 
-__global__ void k_diff(long long n, const int *a, const int *b, int *changed)
+__global__ __launch_bounds__(NT) void k_diff(long long n, const int *a, const int *b, int *changed)
 {
     for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT)
         if (a[i] != b[i]) *changed = 1;
@@ -463,7 +668,7 @@ __global__ void k_diff(long long n, const int *a, const int *b, int *changed)
 
 // after the sweep: the resets to 100 and rho_avg of each written grain (its
 // last writer)
-__global__ void k_finish(Dims g, const int *trig, double *defeng, const unsigned long long *keys, const int *refs,
+__global__ __launch_bounds__(NT) void k_finish(Dims g, const int *trig, double *defeng, const unsigned long long *keys, const int *refs,
                          long long nw, const double *avg, const int *cdd, double *rho)
 {
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
@@ -484,7 +689,7 @@ __device__ inline int wrap3(int p, int L) { return p < 3 ? p + L : p >= L + 3 ? 
 // boundary_conditions on one rank: each halo cell is filled from its periodic
 // image entry by entry through the lookup (copies of copies add nothing: a
 // built list has values below EPSILON only in its last entry)
-__global__ void k_halo(Dims g, Lists B, int *err)
+__global__ __launch_bounds__(NT) void k_halo(Dims g, Lists B, int *err)
 {
     for (long long c = blockIdx.x * (long long)NT + threadIdx.x; c < g.NC; c += (long long)gridDim.x * NT) {
         const int z = (int)(c % g.lz);
@@ -492,22 +697,17 @@ __global__ void k_halo(Dims g, Lists B, int *err)
         const int y = (int)(r % g.ly), x = (int)(r / g.ly);
         if (interior(g, x, y, z)) continue;
         const long long s = cix(g, wrap3(x, g.Lx), wrap3(y, g.Ly), wrap3(z, g.Lz));
-        Build t;
-        t.n = 0;
+        Build t(B, g.NC, c);
         const int n = B.cnt[s];
         bool ok = true;
         for (int j = 0; j < n; j++) ok = ok && t.put(B.idx[(long long)j * g.NC + s], B.val[(long long)j * g.NC + s]);
         if (!ok) atomicOr(err, PFE_CAP);
-        B.cnt[c] = t.n;
-        for (int k = 0; k < t.n; k++) {
-            B.idx[(long long)k * g.NC + c] = t.idx[k];
-            B.val[(long long)k * g.NC + c] = t.val[k];
-        }
+        t.finish();
     }
 }
 
 // mesh_extend on one rank: periodic halos of the stored energy and grains
-__global__ void k_mesh(Dims g, double *defeng, int *gid)
+__global__ __launch_bounds__(NT) void k_mesh(Dims g, double *defeng, int *gid)
 {
     for (long long c = blockIdx.x * (long long)NT + threadIdx.x; c < g.NC; c += (long long)gridDim.x * NT) {
         const int z = (int)(c % g.lz);
@@ -522,7 +722,7 @@ __global__ void k_mesh(Dims g, double *defeng, int *gid)
 
 // Update_gID (src/pf_xrx.c:11-66) and the sum check of GetStatistics
 // (src/pf_xrx.c:1364-1386)
-__global__ void k_gid(Dims g, Lists A, int *gid_new, int *err)
+__global__ __launch_bounds__(NT) void k_gid(Dims g, Lists A, int *gid_new, int *err)
 {
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
         int x, y, z;
@@ -544,14 +744,14 @@ __global__ void k_gid(Dims g, Lists A, int *gid_new, int *err)
     }
 }
 
-__global__ void k_clear(long long n, int *cnt)
+__global__ __launch_bounds__(NT) void k_clear(long long n, int *cnt)
 {
     for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) cnt[i] = 0;
 }
 
 // PF_XRX's initial state (src/pf_xrx.c:1538-1560): one entry (gID, 1.0) per
 // voxel, the stored energy and the grain field
-__global__ void k_init(Dims g, Lists A, const int *gID, const double *rho, double *defeng, int *gid)
+__global__ __launch_bounds__(NT) void k_init(Dims g, Lists A, const int *gID, const double *rho, double *defeng, int *gid)
 {
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
         int x, y, z;
@@ -566,7 +766,7 @@ __global__ void k_init(Dims g, Lists A, const int *gID, const double *rho, doubl
 }
 
 // interior lists from host layout [v*CAP + k]
-__global__ void k_load(Dims g, Lists A, const int *n, const int *idx, const double *val, const int *GID,
+__global__ __launch_bounds__(NT) void k_load(Dims g, Lists A, const int *n, const int *idx, const double *val, const int *GID,
                        const double *de, double *defeng, int *gid, int *err)
 {
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
@@ -588,7 +788,7 @@ __global__ void k_load(Dims g, Lists A, const int *n, const int *idx, const doub
     }
 }
 
-__global__ void k_store(Dims g, Lists A, const double *defeng, int *n, int *idx, double *val, double *de)
+__global__ __launch_bounds__(NT) void k_store(Dims g, Lists A, const double *defeng, int *n, int *idx, double *val, double *de)
 {
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
         int x, y, z;
@@ -620,7 +820,7 @@ struct evpgpu_pf {
     Lists A, B;
     double *defeng;
     int *gid;
-    int *gcnt, *gidx, *act, *nn, *spos, *sg, *wn, *trig0, *trig1, *gid_new;
+    int *gcnt, *gidx, *act, *emask, *nn, *spos, *sg, *wn, *trig0, *trig1, *gid_new;
     long long *woff;
     double *avg;
     int *cdd;
@@ -634,7 +834,7 @@ struct evpgpu_pf {
     int *h_flags;
     long long *h_total;
     int *h_last;
-    hipEvent_t ev0, ev1;
+    hipEvent_t ev0, ev1, eu[3];
     int loaded;
     evpgpu_pf_stats st;
 };
@@ -701,12 +901,15 @@ int substep(evpgpu_pf *p, const Coef &P)
     hipStream_t s = p->stream;
     const int gv = grid_for(g.NV), gc = grid_for(g.NC);
     int st;
-    hipLaunchKernelGGL(k_glist, dim3(gv), dim3(NT), 0, s, g, p->A, p->gcnt, p->gidx, p->flags);
-    hipLaunchKernelGGL(k_act_init, dim3(gv), dim3(NT), 0, s, g, p->A, p->act);
+    // tiles of TX x TY x TZ voxels (k_glist, k_avg, k_update)
+    const int tiles = ((g.Lx + TX - 1) / TX) * ((g.Ly + TY - 1) / TY) * ((g.Lz + TZ - 1) / TZ);
+    hipLaunchKernelGGL(k_glist, dim3(tiles), dim3(NT), 0, s, g, p->A, p->gcnt, p->gidx, p->flags);
+    hipLaunchKernelGGL(k_act_init, dim3(gv), dim3(NT), 0, s, g, p->A, p->gcnt, p->gidx, p->act, p->emask);
+    const int act_tiles = ((g.Lx + ACT_T - 1) / ACT_T) * ((g.Ly + ACT_T - 1) / ACT_T);
     for (int pass = 0;; pass++) {
         if (pass == MAX_PASSES) return EVPGPU_ERR_HIP;
         if ((st = clear_flag(p, 1))) return st;
-        hipLaunchKernelGGL(k_act_iter, dim3(gv), dim3(NT), 0, s, g, p->A, p->gcnt, p->gidx, p->act, p->flags + 1);
+        hipLaunchKernelGGL(k_act_iter, dim3(act_tiles), dim3(ACT_NT), 0, s, g, p->emask, p->act, p->flags + 1);
         if ((st = read_flags(p))) return st;
         p->st.activity_passes++;
         if (p->h_flags[0]) return pf_err(p->h_flags[0]);
@@ -738,19 +941,30 @@ int substep(evpgpu_pf *p, const Coef &P)
                                                      (int)nw, 0, end_bit, s));
     }
     p->st.rho_writes += nw;
-    // trigger fixed point
+    // trigger fixed point; tout holds the resets the previous pass used
     int *tin = p->trig0, *tout = p->trig1;
     hipLaunchKernelGGL(k_clear, dim3(gv), dim3(NT), 0, s, g.NV, tin);
     for (int pass = 0;; pass++) {
         if (pass == MAX_PASSES) return EVPGPU_ERR_HIP;
-        hipLaunchKernelGGL(k_avg, dim3(gv), dim3(NT), 0, s, g, p->A, p->wn, p->spos, p->sg, p->gid, p->defeng, tin,
-                           p->avg, p->cdd);
-        hipLaunchKernelGGL(k_update, dim3(gv), dim3(NT), 0, s, g, p->A, p->B, P, p->gcnt, p->gidx, p->nn, p->spos,
+        EVP_CHECK(hipEventRecord(p->eu[0], s));
+        hipLaunchKernelGGL(k_avg, dim3(tiles), dim3(NT), 0, s, g, p->A, p->wn, p->spos, p->sg, p->gid, p->defeng,
+                           tin, pass ? tout : (const int *)nullptr, p->avg, p->cdd);
+        EVP_CHECK(hipEventRecord(p->eu[1], s));
+        hipLaunchKernelGGL(k_update, dim3(tiles), dim3(NT), 0, s, g, p->A, p->B, P, p->gcnt, p->gidx, p->nn, p->spos,
                            p->sg, p->avg, p->cdd, p->keys[1], p->refs[1], nw, p->rho_in, tout, p->flags);
+        EVP_CHECK(hipEventRecord(p->eu[2], s));
         if ((st = clear_flag(p, 1))) return st;
         hipLaunchKernelGGL(k_diff, dim3(gv), dim3(NT), 0, s, g.NV, tin, tout, p->flags + 1);
         if ((st = read_flags(p))) return st;
         p->st.trigger_passes++;
+        {
+            float a = 0.f, u = 0.f;
+            (void)hipEventElapsedTime(&a, p->eu[0], p->eu[1]);
+            (void)hipEventElapsedTime(&u, p->eu[1], p->eu[2]);
+            p->st.ms_avg += a;
+            p->st.ms_update += u;
+            p->st.update_launches++;
+        }
         if (p->h_flags[0]) return pf_err(p->h_flags[0]);
         int *t = tin;
         tin = tout;
@@ -822,6 +1036,7 @@ extern "C" int evpgpu_pf_create(evpgpu_pf **out, int Lx, int Ly, int Lz, int n_r
     }
     bad |= dalloc(&p->defeng, g.NC) | dalloc(&p->gid, g.NC);
     bad |= dalloc(&p->gcnt, g.NV) | dalloc(&p->gidx, g.NV * CAP) | dalloc(&p->act, g.NV) | dalloc(&p->nn, g.NV);
+    bad |= dalloc(&p->emask, g.NV);
     bad |= dalloc(&p->spos, g.NV) | dalloc(&p->sg, g.NV) | dalloc(&p->wn, g.NV) | dalloc(&p->woff, g.NV);
     bad |= dalloc(&p->trig0, g.NV) | dalloc(&p->trig1, g.NV) | dalloc(&p->gid_new, g.NV);
     bad |= dalloc(&p->avg, g.NV * CAP) | dalloc(&p->cdd, g.NV * CAP);
@@ -837,6 +1052,7 @@ extern "C" int evpgpu_pf_create(evpgpu_pf **out, int Lx, int Ly, int Lz, int n_r
     bad |= hipStreamCreate(&p->stream) != hipSuccess;
     bad |= hipEventCreate(&p->ev0) != hipSuccess;
     bad |= hipEventCreate(&p->ev1) != hipSuccess;
+    for (int i = 0; i < 3; i++) bad |= hipEventCreate(&p->eu[i]) != hipSuccess;
     if (!bad) bad |= hipMemset(p->flags, 0, 3 * sizeof(int)) != hipSuccess;
     if (!bad) bad |= hipMemset(p->A.cnt, 0, g.NC * sizeof(int)) != hipSuccess;
     if (!bad) bad |= hipMemset(p->B.cnt, 0, g.NC * sizeof(int)) != hipSuccess;
@@ -857,7 +1073,7 @@ extern "C" int evpgpu_pf_destroy(evpgpu_pf *p)
         (void)hipFree(L->idx);
         (void)hipFree(L->val);
     }
-    void *dev[] = {p->defeng, p->gid, p->gcnt, p->gidx, p->act, p->nn,    p->spos,    p->sg,      p->wn,
+    void *dev[] = {p->defeng, p->gid, p->gcnt, p->gidx, p->act, p->emask, p->nn,    p->spos,    p->sg,      p->wn,
                    p->woff,   p->trig0, p->trig1, p->gid_new, p->avg, p->cdd, p->rho_in, p->flags, p->keys[0],
                    p->keys[1], p->refs[0], p->refs[1], p->tmp};
     for (void *q : dev) (void)hipFree(q);
@@ -866,6 +1082,8 @@ extern "C" int evpgpu_pf_destroy(evpgpu_pf *p)
     if (p->h_last) (void)hipHostFree(p->h_last);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
+    for (int i = 0; i < 3; i++)
+        if (p->eu[i]) (void)hipEventDestroy(p->eu[i]);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     free(p);
     return EVPGPU_OK;

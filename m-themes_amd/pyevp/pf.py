@@ -27,7 +27,8 @@ class Params(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("substeps", C.c_int), ("activity_passes", C.c_int), ("trigger_passes", C.c_int),
-                ("updated", C.c_longlong), ("rho_writes", C.c_longlong), ("ms", C.c_double)]
+                ("updated", C.c_longlong), ("rho_writes", C.c_longlong), ("ms", C.c_double),
+                ("ms_update", C.c_double), ("ms_avg", C.c_double), ("update_launches", C.c_int)]
 
 
 def params(d: dict) -> Params:

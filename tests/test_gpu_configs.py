@@ -99,12 +99,17 @@ def test_c3_256_converged_increment(c3):
 
 
 # ------------------------------------------------------------------ 64^3 --
-C4_ERR = 3e-8
+# Loop tolerance of the finite-difference checks: the agreement is limited by
+# how far the forward stops from the fixed point the adjoint differentiates.
+# At 1e-7 tau0, tau1, theta0, theta1 agree to <= 3e-7 and the rate exponent
+# to 1.7e-5; the exponent's check runs at 3e-8 (its perturbed forwards take
+# the pow() path of a non-integral exponent, about four times slower).
+C4_ERR = {"tau0": 1e-7, "tau1": 1e-7, "theta0": 1e-7, "theta1": 1e-7, "n": 3e-8}
 
 
-def c4_job(d, p=None):
+def c4_job(d, err, p=None):
     cfg = synth.config_c4()
-    cfg.err = C4_ERR
+    cfg.err = err
     cfg.iter_max = 40000
     job = lib.GpuJob(synth.write_job(str(d), cfg))
     job.set_newton_tolerances(1e-14, 1e-13)
@@ -124,42 +129,61 @@ def forced_szz(job, iters):
 
 
 @pytest.fixture(scope="module")
-def c4(tmp_path_factory):
-    """The recorded base forward, its adjoint gradient and a second job for
-    the finite differences."""
-    d = tmp_path_factory.mktemp("c4")
-    job, cfg = c4_job(d / "base")
+def c4_target(tmp_path_factory):
+    """the macroscopic sigma_33 curve of a forward at parameters +5 %"""
+    job, cfg = c4_job(tmp_path_factory.mktemp("c4t"), 1e-7)
+    job.set_params(0, job.get_params(0) * 1.05)
+    curve, _ = job.adjoint_record(cfg.increments, 5)
+    job.close()
+    return curve[:, 2].copy()
+
+
+def c4_base(d, err, target):
+    """The recorded base forward at loop tolerance err, its adjoint gradient of
+    L = sum_k (Sigma_33,k - target_k)^2 and a second job for the finite
+    differences."""
+    job, cfg = c4_job(d / "base", err)
     p0 = job.get_params(0)
     n = cfg.increments
-    # target: a seeded forward at parameters +5 %
-    job.set_params(0, p0 * 1.05)
-    target, _ = job.adjoint_record(n, 5)
-    target = target[:, 2].copy()
-    job.set_params(0, p0)
     t0 = time.perf_counter()
     curve, stats = job.adjoint_record(n, 5)
     t_fwd = time.perf_counter() - t0
     iters = [s["iters"] for s in stats]
-    say(f"c4: forward {t_fwd:.2f} s, Green iterations {iters}")
+    say(f"c4 (Err {err:g}): forward {t_fwd:.2f} s, Green iterations {iters}")
     seeds = np.zeros((n, 6))
     seeds[:, 2] = 2.0 * (curve[:, 2] - target)
     t0 = time.perf_counter()
     g, st = job.adjoint_run(seeds, cg_tol=1e-12, cg_max=4000)
     t_adj = time.perf_counter() - t0
-    say(f"c4: adjoint {t_adj:.2f} s (replay {st['replay_seconds']:.2f} s, CG {st['cg_iters']}), "
+    say(f"c4 (Err {err:g}): adjoint {t_adj:.2f} s (replay {st['replay_seconds']:.2f} s, CG {st['cg_iters']}), "
         f"ratio {t_adj / t_fwd:.3f}; gradient {g[0]}")
     job.close()
-    fj, _ = c4_job(d / "fd")
-    yield dict(p0=p0, target=target, iters=iters, grad=g[0], t_fwd=t_fwd, t_adj=t_adj, fj=fj)
-    fj.close()
+    fj, _ = c4_job(d / "fd", err)
+    return dict(p0=p0, target=target, iters=iters, grad=g[0], t_fwd=t_fwd, t_adj=t_adj, fj=fj)
 
 
-def test_c4_64_adjoint_within_three_forwards(c4):
-    assert c4["t_adj"] <= 3.0 * c4["t_fwd"]
+@pytest.fixture(scope="module")
+def c4_loose(tmp_path_factory, c4_target):
+    r = c4_base(tmp_path_factory.mktemp("c4l"), 1e-7, c4_target)
+    yield r
+    r["fj"].close()
+
+
+@pytest.fixture(scope="module")
+def c4_tight(tmp_path_factory, c4_target):
+    r = c4_base(tmp_path_factory.mktemp("c4s"), 3e-8, c4_target)
+    yield r
+    r["fj"].close()
+
+
+def test_c4_64_adjoint_within_three_forwards(c4_tight):
+    assert c4_tight["t_adj"] <= 3.0 * c4_tight["t_fwd"]
 
 
 @pytest.mark.parametrize("i", range(5), ids=list(lib.PARAM_NAMES))
-def test_c4_64_adjoint_gradient_vs_gpu_central_differences(c4, i):
+def test_c4_64_adjoint_gradient_vs_gpu_central_differences(request, i):
+    name = lib.PARAM_NAMES[i]
+    c4 = request.getfixturevalue("c4_tight" if C4_ERR[name] < 1e-7 else "c4_loose")
     p0, fj = c4["p0"], c4["fj"]
     h = 1e-6 * abs(p0[i])
     lv = []
@@ -169,10 +193,11 @@ def test_c4_64_adjoint_gradient_vs_gpu_central_differences(c4, i):
         fj.set_params(0, p)
         t0 = time.perf_counter()
         lv.append(np.sum((forced_szz(fj, c4["iters"]) - c4["target"]) ** 2))
-        say(f"c4: parameter {lib.PARAM_NAMES[i]} {'+-'[sgn < 0]}h forward {time.perf_counter() - t0:.1f} s")
+        say(f"c4: parameter {name} {'+-'[sgn < 0]}h forward {time.perf_counter() - t0:.1f} s")
     fd = (lv[0] - lv[1]) / (2 * h)
     g = c4["grad"][i]
-    say(f"c4: parameter {lib.PARAM_NAMES[i]}: adjoint {g:.10e} FD {fd:.10e} rel {abs(g - fd) / abs(fd):.2e}")
+    say(f"c4: parameter {name} (Err {C4_ERR[name]:g}): adjoint {g:.10e} FD {fd:.10e} "
+        f"rel {abs(g - fd) / abs(fd):.2e}")
     assert abs(g - fd) <= 1e-5 * abs(fd)
 
 
