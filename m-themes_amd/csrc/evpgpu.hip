@@ -773,8 +773,9 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
         return (iter < c->ld.iter_max) && (fmax(fabs(m.err_s), fabs(m.err_e)) > tol);
     };
     const bool mixed = c->precision == EVPGPU_MIXED;
-    if (mixed) EVP_CHECK(hipMemcpyAsync(c->Estart, c->E, 6 * c->N * sizeof(double),
-                                        hipMemcpyDeviceToDevice, c->stream));
+    // the strain at the start of the loop: the mixed loop's projection and
+    // the antisymmetric increment at its end use it
+    EVP_CHECK(hipMemcpyAsync(c->Estart, c->E, 6 * c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     const int ae = c->anchor_every > 0 ? c->anchor_every : 1;
     int imix = 0;
     auto mixed_step = [&](void) {
@@ -881,7 +882,14 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
         st = dd_evolve_launch(c, dd_params(c));
         if (st) return st;
     }
-    st = fft_inverse_real(c, c->acc, 3, c->rwork, 1.0 / (double)c->N);
+    // the antisymmetric part of the loop's Green outputs: at every wave vector
+    // but the split ones it follows from the strain change E - Estart (its
+    // spectrum is the sum of the outputs' symmetric parts); the split ones
+    // were accumulated in the loop
+    st = rsub_launch(c, c->E, c->Estart, 6 * c->N);
+    if (!st) st = fft_forward_to(c, c->Estart, 6, (double2 *)c->spec);
+    if (!st) st = green_skew_fill(c);
+    if (!st) st = fft_inverse_real(c, c->acc, 3, c->rwork, 1.0 / (double)c->N);
     if (st) return st;
     st = antisym_add_launch(c, m3);
     if (st) return st;

@@ -208,6 +208,7 @@ int fft_inverse_anchor(evpgpu_ctx *c, float *cA, double scale);
 int fft_inverse_set(evpgpu_ctx *c, double *target, const double *base, double scale);
 // green.hip
 int green_apply(evpgpu_ctx *c, int precision, int accumulate);
+int green_skew_fill(evpgpu_ctx *c);
 int green_full9(evpgpu_ctx *c, const double2 *in6, double2 *out9);
 int green_anchor(evpgpu_ctx *c);
 // fft_fused.hip
@@ -244,4 +245,5 @@ int fill_zero(evpgpu_ctx *c, double *p, long long n);
 int fill_zero_int(evpgpu_ctx *c, int *p, long long n);
 int to_f32_launch(evpgpu_ctx *c, const double *src, float *dst, long long n);
 int sub_launch(evpgpu_ctx *c, double *a, const double *b, long long n);
+int rsub_launch(evpgpu_ctx *c, const double *a, double *b, long long n);
 int copy_launch(evpgpu_ctx *c, const double *src, double *dst, long long n);

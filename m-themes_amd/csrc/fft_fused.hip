@@ -10,7 +10,8 @@
 //   k_x_green  one workgroup per run of TBX (y, kz) columns: all 6 components
 //              of those lines are transformed along x, the Green operator is
 //              applied at each wave vector (green_point.h, with the
-//              antisymmetric accumulator update), and transformed back;
+//              antisymmetric accumulator update at the split wave vectors),
+//              and transformed back;
 //   k_yz_c2r   the mirror of k_zy_r2c, ending in the strain update.
 //
 // The results equal fft.hip's up to rounding: the same half-spectrum
@@ -177,9 +178,12 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
     gstages<false, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
     // Green result at (x, column q0 + j) back into LDS, antisymmetric part
     // into the accumulator
-    auto put = [&](int x, int j, const C e6[6], const double wr[3], const double wi[3]) {
+    auto put = [&](int x, int j, const C e6[6], const double wr[3], const double wi[3], bool split) {
 #pragma unroll
         for (int c = 0; c < 6; c++) lds[(c * TBX + j) * PX + x] = e6[c];
+        // antisymmetric parts are kept at the split wave vectors only (the
+        // others follow from the increment's strain change at its end)
+        if (!split) return;
         const long long id = (long long)x * ncol + q0 + j;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -210,8 +214,9 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
 #pragma unroll
         for (int c = 0; c < 6; c++) s6[c] = lds[(c * TBX + j) * PX + x];
         const int y = q / nzh, kz = q - y * nzh;
-        gop::green_point<R>(c0s, x, y, kz, NX, ny, nz, s6, e6, wr, wi);
-        put(x, j, e6, wr, wi);
+        bool split;
+        gop::green_point<R>(c0s, x, y, kz, NX, ny, nz, s6, e6, wr, wi, split);
+        put(x, j, e6, wr, wi, split);
     }
     if (q0 == 0 && threadIdx.x == 0) {
         C s6[6], e6[6];
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
 #pragma unroll
         for (int c = 0; c < 6; c++) s6[c] = lds[c * TBX * PX];
         gop::green_zero<R>(c0s, G.s0, G.elast_bc, NX, ny, nz, s6, e6, wr, wi);
-        put(0, 0, e6, wr, wi);
+        put(0, 0, e6, wr, wi, false);
     }
     __syncthreads();
     gstages<true, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);

@@ -48,6 +48,13 @@ __global__ void k_sub(double *__restrict__ a, const double *__restrict__ b, long
     if (i < n) a[i] -= b[i];
 }
 
+// b = a - b
+__global__ void k_rsub(const double *__restrict__ a, double *__restrict__ b, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i] - b[i];
+}
+
 // E (6 comps) += dE (sym of Udot*dt), W (3 comps) += dW; creep: E = m6, W = w3
 __global__ void k_prologue(double *E, double *W, long long N, double d0, double d1, double d2,
                            double d3, double d4, double d5, double w0, double w1, double w2,
@@ -252,6 +259,12 @@ int copy_launch(evpgpu_ctx *c, const double *src, double *dst, long long n)
 {
     const long long n2 = n / 2;
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, c->stream, (const double2 *)src, (double2 *)dst, n2);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+int rsub_launch(evpgpu_ctx *c, const double *a, double *b, long long n)
+{
+    hipLaunchKernelGGL(k_rsub, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a, b, n);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 

@@ -34,6 +34,9 @@ using gop::gamma_apply;
 // MODE 1: write all 9 components to out9 (tests)
 // MODE 2: mixed loop, delta spectrum: in place symmetric, acc += antisym + skA
 // MODE 3: mixed loop, anchor spectrum: in place symmetric, skA = antisym
+// The antisymmetric parts are accumulated at the split wave vectors only:
+// everywhere else the increment's antisymmetric part follows from its
+// symmetric part at the end of the increment (green_skew_fill).
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 *acc,
                                                double2 *out9, float2 *skA, GreenParams P, int nx,
@@ -63,10 +66,12 @@ __global__ __launch_bounds__(256) void k_green(typename CT<R>::T *spec, double2 
     }
     C e6[6];
     double wr[3], wi[3];
+    bool split = false;
     if (id == 0) gop::green_zero<R>(c0s, P.s0, P.elast_bc, nx, ny, nz, s6, e6, wr, wi);
-    else gop::green_point<R>(c0s, x, y, kz, nx, ny, nz, s6, e6, wr, wi);
+    else gop::green_point<R>(c0s, x, y, kz, nx, ny, nz, s6, e6, wr, wi, split);
 #pragma unroll
     for (int c = 0; c < 6; c++) spec[c * NK + id] = e6[c];
+    if (!split) return;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         if (MODE == 3) {
@@ -147,7 +152,49 @@ __global__ __launch_bounds__(256) void k_project(double2 *spec, int nx, int ny, 
     }
 }
 
+
+// The antisymmetric part of an increment's Green outputs at a wave vector
+// that is not split, from the spectrum of its symmetric part e (the
+// increment's strain change): every output there is -u (x) n, so
+// e = sym(a x n) with a = 2 e.n - (n.e.n) n and the antisymmetric part is
+// skew(a x n) (23 13 12).  The split wave vectors and k = 0 keep the
+// accumulated values.
+__global__ __launch_bounds__(256) void k_skew_fill(const double2 *spec, double2 *acc, int nx, int ny, int nz,
+                                                   int nzh, long long NK)
+{
+    long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= NK || id == 0) return;
+    int kz = (int)(id % nzh);
+    int y = (int)((id / nzh) % ny);
+    int x = (int)(id / ((long long)nzh * ny));
+    double n[3], n2[3];
+    if (kspace::mode_dirs(x, y, kz, nx, ny, nz, n, n2)) return;
+    double2 e[6];
+    for (int c = 0; c < 6; c++) e[c] = spec[c * NK + id];
+    // e.n with e symmetric (11 22 33 23 13 12)
+    double2 en[3];
+    en[0] = make_double2(e[0].x * n[0] + e[5].x * n[1] + e[4].x * n[2], e[0].y * n[0] + e[5].y * n[1] + e[4].y * n[2]);
+    en[1] = make_double2(e[5].x * n[0] + e[1].x * n[1] + e[3].x * n[2], e[5].y * n[0] + e[1].y * n[1] + e[3].y * n[2]);
+    en[2] = make_double2(e[4].x * n[0] + e[3].x * n[1] + e[2].x * n[2], e[4].y * n[0] + e[3].y * n[1] + e[2].y * n[2]);
+    const double2 nen = make_double2(n[0] * en[0].x + n[1] * en[1].x + n[2] * en[2].x,
+                                     n[0] * en[0].y + n[1] * en[1].y + n[2] * en[2].y);
+    double2 a[3];
+    for (int i = 0; i < 3; i++) a[i] = make_double2(2.0 * en[i].x - nen.x * n[i], 2.0 * en[i].y - nen.y * n[i]);
+    acc[0 * NK + id] = make_double2(0.5 * (a[1].x * n[2] - a[2].x * n[1]), 0.5 * (a[1].y * n[2] - a[2].y * n[1]));
+    acc[1 * NK + id] = make_double2(0.5 * (a[0].x * n[2] - a[2].x * n[0]), 0.5 * (a[0].y * n[2] - a[2].y * n[0]));
+    acc[2 * NK + id] = make_double2(0.5 * (a[0].x * n[1] - a[1].x * n[0]), 0.5 * (a[0].y * n[1] - a[1].y * n[0]));
+}
 } // namespace
+
+// acc at the wave vectors that are not split: the antisymmetric part of the
+// increment from the spectrum of its strain change held in c->spec
+int green_skew_fill(evpgpu_ctx *c)
+{
+    unsigned nb = (unsigned)((c->NK + 255) / 256);
+    hipLaunchKernelGGL(k_skew_fill, dim3(nb), dim3(256), 0, c->stream, (const double2 *)c->spec, c->acc, c->nx,
+                       c->ny, c->nz, c->nzh, c->NK);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
 
 int green_project(evpgpu_ctx *c)
 {

@@ -182,9 +182,11 @@ __device__ __forceinline__ void green_zero(const R *ga, const double *s0, int el
 // FP64 real and imaginary parts) of e-hat at one wave vector k != 0 (the
 // caller takes k = 0 to green_zero: kept apart, its code does not add to the
 // registers of the per-point loops)
+// split: the wave vector sits on a Nyquist plane where n(-k) != -n(k)
+// (Gamma_eff mixes two directions there)
 template <typename R, typename C>
 __device__ __forceinline__ void green_point(const R *ga, int x, int y, int kz, int nx, int ny, int nz,
-                                            const C s6[6], C e6[6], double wr[3], double wi[3])
+                                            const C s6[6], C e6[6], double wr[3], double wi[3], bool &split)
 {
     C an[3];
 #pragma unroll
@@ -192,7 +194,8 @@ __device__ __forceinline__ void green_point(const R *ga, int x, int y, int kz, i
 #pragma unroll
     for (int c = 0; c < 3; c++) an[c] = C{(R)0, (R)0};
     double n1[3], n2[3];
-    if (!kspace::mode_dirs(x, y, kz, nx, ny, nz, n1, n2)) {
+    split = kspace::mode_dirs(x, y, kz, nx, ny, nz, n1, n2);
+    if (!split) {
         gamma_acc6<R>(ga, n1, s6, e6, an, (R)1);
     } else {
         gamma_acc6<R>(ga, n1, s6, e6, an, (R)0.5);
