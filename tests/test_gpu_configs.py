@@ -234,8 +234,10 @@ def test_c5_128_dd_loop_keeps_grain_ids(c5, part):
 
 
 def test_drx_gate_refuses_an_increment_past_the_nucleation_strain(tmp_path):
-    """Past e_vm = 4 the reference would nucleate DRX grains, which this build
-    does not: the next increment is refused with status 850."""
+    """Past e_vm = 4 the reference runs its DRX event inside the increment
+    driver (nucleation, phase field, microstructure update); the loop here
+    does not chain them (the event's pieces are their own entry points), so
+    the next increment is refused with status 850."""
     import ctypes as C
     cfg = synth.Config(n=8, grains=4, seed=3, increments=2, dd=synth.DDLaw())
     job = lib.GpuJob(synth.write_job(str(tmp_path), cfg), lib.FP64, law=lib.LAW_DD)

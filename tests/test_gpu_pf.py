@@ -162,3 +162,47 @@ def test_drx_nucleate_vs_oracle(shape, grains, seed):
     assert g["n_new"] == r["n_new"] and np.array_equal(g["vox"], r["vox"])
     for k in ("grain", "gid_rex", "nuclei", "kc", "rho_tot"):
         assert np.array_equal(g[k], r[k]), k
+
+
+def test_drx_event_pieces_on_a_dislocation_density_state(tmp_path):
+    """The event's pieces chained on the state of a real dislocation-density
+    run, GPU against oracle at each step: the strengths (seeded Weibull
+    draws), the nucleation check on the run's densities and grains, the
+    stored energy PF_XRX starts from (8 per slip system at the nuclei, the
+    density total elsewhere, src/evolution.c:973-1000) and ten phase-field
+    substeps; the grain field that comes out is the same bit for bit."""
+    from pyevp import synth
+    cfg = synth.Config(n=24, grains=16, seed=0x5EED0D, increments=3, dd=synth.DDLaw())
+    job = lib.GpuJob(synth.write_job(str(tmp_path), cfg), lib.FP64, law=lib.LAW_DD)
+    job.run(3)
+    dd = job.dd_state()
+    grain, _ = job.ids()
+    n_grains = int(len(job.grain_list()))
+    job.close()
+    shape = (cfg.n,) * 3
+    dens = [np.ascontiguousarray(dd[k].T).reshape((12,) + shape) for k in ("rho_s", "rho_m", "rho_g1", "rho_g2",
+                                                                        "rho_g3")]
+    g0 = grain.reshape(shape)
+    # strengths scaled so that a handful of boundary voxels nucleate (a
+    # Weibull scale 5 % above the largest rho_tot / 12)
+    tot = sum(d for d in dens[:2]).sum(0) + np.sqrt(dens[2] ** 2 + dens[3] ** 2 + dens[4] ** 2).sum(0)
+    kappa = gpf.weibull(0x5EED, np.full(shape, 1.05 * float((tot / 12).max())), 8.5)
+    rex0 = np.zeros(shape, np.int32)
+    kc0 = np.full(shape, 1.0)
+    r = o.drx_nucleate(g0, *dens, kappa, n_grains, 4, 1.0, 1.0, rex0, kc0)
+    g = gpf.nucleate(g0, *dens, kappa, n_grains, 4, 1.0, 1.0, rex0, kc0)
+    say(f"drx event: {r['n_new']} nuclei, {int((r['grain'] != g0).sum())} voxels repainted")
+    assert r["status"] == 0 and r["n_new"] > 0
+    for k in ("grain", "nuclei", "gid_rex", "kc", "rho_tot"):
+        assert np.array_equal(g[k], r[k]), k
+    rho_pf = np.where(g["nuclei"] == 1, 8.0 * 12, g["rho_tot"])
+    P = pf_cases.params(n_grains)
+    ra = np.zeros(60000)
+    ro = o.pf_xrx(g["grain"], rho_pf, 10, P, ra)
+    pf = gpf.PhaseField(shape, ra.size)
+    try:
+        gn, ra_g = pf.pf_xrx(g["grain"], rho_pf, 10, P, ra)
+        same(pf.lists(), ro, gn, ra_g)
+    finally:
+        pf.close()
+    say(f"drx event: grains changed by the phase field {int((gn != g['grain']).sum())}")
