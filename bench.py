@@ -80,12 +80,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(inp, precision, steps, warmup, law=lib.LAW_POWER):
+def run_gpu(inp, precision, steps, warmup, law=lib.LAW_POWER, loop_batch=None, timers=(0,), every=8):
+    """Timed increments; HIP-event timers `timers` (default: the Newton kernel's
+    alone) in one Green iteration of every `every` -- each event pair costs the
+    stream a few microseconds between kernels."""
     job = lib.GpuJob(inp, precision, law=law)
+    if loop_batch is not None:
+        job.set_loop_batch(loop_batch)
     if warmup > 0:
         job.run(warmup)
         job.reset()
-    job.set_timing(True)
+    job.set_timing(True, groups=timers, every=every)
     job.synchronize()
     t0 = time.perf_counter()
     it, stats = job.run(steps)
@@ -295,6 +300,14 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="profiling mode: one increment forced to this many Green iterations "
                          "per precision, then exit (for rocprofv3 counter runs)")
+    ap.add_argument("--timers", choices=["newton", "all"], default="newton",
+                    help="HIP-event timers in the timed region: the Newton kernel's alone "
+                         "(default), or every kernel group's (adds event overhead)")
+    ap.add_argument("--timer-every", type=int, default=8,
+                    help="the timers record one Green iteration in this many")
+    ap.add_argument("--loop-batch", type=int, default=None,
+                    help="Green iterations per batch queued ahead of the device (0: the host "
+                         "waits for every iteration; default: the library's, 4)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling mode: also run one plain device copy of this size (the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
@@ -352,7 +365,8 @@ def main():
         if args.precision not in ("both", name):
             continue
         log(f"bench: {name}: warmup {args.warmup}, timing {args.steps} increments")
-        r = run_gpu(inp, prec, args.steps, args.warmup)
+        r = run_gpu(inp, prec, args.steps, args.warmup, loop_batch=args.loop_batch,
+                    timers=(0,) if args.timers == "newton" else tuple(range(6)), every=args.timer_every)
         its = sum(r["iters"])
         r["vips"] = N * its / r["wall"]
         log(f"bench: {name}: iterations {r['iters']} ({sum(s['fp64_iters'] for s in r['stats'])} "
@@ -361,10 +375,13 @@ def main():
     main_key = "mixed" if "mixed" in res else "fp64"
     R = res[main_key]
 
-    # roofline of the dominant kernel: the per-voxel Newton update
-    ms, launches = R["timing"][0]
-    n_gas = None
-    g, p = None, None
+    # roofline of the dominant kernel: the per-voxel Newton update.  Its HIP
+    # events time a sample of the launches (one Green iteration in
+    # --timer-every); the work per launch is the average over all of them
+    # (one launch per Green iteration, plus the replay closing each increment)
+    ms, sampled = R["timing"][0]
+    avg_ms = ms / max(sampled, 1)
+    launches = sum(R["iters"]) + len(R["iters"])
     nit = sum(s["newton_iters"] for s in R["stats"])
     ntr = sum(s["ls_trials"] for s in R["stats"])
     j = lib.GpuJob(inp, lib.FP64)
@@ -372,10 +389,10 @@ def main():
     j.close()
     n_cryst = int(np.sum(p == 1))
     n_gas = N - n_cryst
-    flops = ALG_FLOPS_VOXEL * n_cryst * launches + ALG_FLOPS_ITER * nit
-    own = NEWTON_FLOPS_VOXEL * n_cryst * launches + NEWTON_FLOPS_ITER * nit + NEWTON_FLOPS_TRIAL * ntr
-    bytes_ = (NEWTON_BYTES_CRYSTAL * n_cryst + NEWTON_BYTES_GAS * n_gas) * launches
-    t_s = ms * 1e-3
+    flops = (ALG_FLOPS_VOXEL * n_cryst * launches + ALG_FLOPS_ITER * nit) / launches
+    own = (NEWTON_FLOPS_VOXEL * n_cryst * launches + NEWTON_FLOPS_ITER * nit + NEWTON_FLOPS_TRIAL * ntr) / launches
+    bytes_ = NEWTON_BYTES_CRYSTAL * n_cryst + NEWTON_BYTES_GAS * n_gas
+    t_s = avg_ms * 1e-3
     tf = flops / t_s / 1e12
     tf_own = own / t_s / 1e12
     gbs = bytes_ / t_s / 1e9
@@ -384,27 +401,30 @@ def main():
     traffic = NEWTON_TRAFFIC_128 if cfg.n == 128 else None
     per = dict(newton_iters_per_voxel_launch=nit / max(n_cryst * launches, 1),
                trials_per_voxel_launch=ntr / max(n_cryst * launches, 1),
-               flops_per_launch=flops / max(launches, 1), bytes_per_launch=bytes_ / max(launches, 1),
-               kernel_count=dict(tflops=tf_own, frac=tf_own / PEAK_FP64_TFS,
-                                 flops_per_launch=own / max(launches, 1)))
+               flops_per_launch=flops, bytes_per_launch=bytes_, launches=launches,
+               timed_launches=sampled,
+               kernel_count=dict(tflops=tf_own, frac=tf_own / PEAK_FP64_TFS, flops_per_launch=own))
     if tf / PEAK_FP64_TFS >= gbs / PEAK_HBM_GBS:
         roof = dict(bound="fp64-valu", achieved=tf, peak=PEAK_FP64_TFS, unit="TFLOP/s",
                     frac=tf / PEAK_FP64_TFS, traffic=traffic, kernel="k_newton",
-                    avg_launch_ms=ms / max(launches, 1), hbm_gbs=gbs, **per,
+                    avg_launch_ms=avg_ms, hbm_gbs=gbs, **per,
                     note="bound by the FP64 vector ALU (no MFMA instruction is used); peak is "
                          "the MI355X FP64 vector figure, 78.6 TFLOP/s; achieved counts the "
                          "reference's work per voxel (0.5 kflop + 4.7 kflop per Newton iteration), "
-                         "kernel_count the kernel's own smaller count")
+                         "kernel_count the kernel's own smaller count; avg_launch_ms from HIP "
+                         "events on the launches of one Green iteration in "
+                         f"{args.timer_every} over the timed region")
     else:
         roof = dict(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s",
                     frac=gbs / PEAK_HBM_GBS, traffic=traffic, kernel="k_newton",
-                    avg_launch_ms=ms / max(launches, 1), tflops=tf, **per)
+                    avg_launch_ms=avg_ms, tflops=tf, **per)
     kern = {}
     names = ["newton", "fft_forward", "green", "fft_inverse_update", "increment_pro_epilogue",
              "anchor_refresh"]
     for k, nm in enumerate(names):
         kms, kn = R["timing"][k]
-        kern[nm] = dict(ms_total=round(kms, 3), launches=kn, ms_avg=round(kms / max(kn, 1), 4))
+        if kn:
+            kern[nm] = dict(ms_total=round(kms, 3), launches=kn, ms_avg=round(kms / max(kn, 1), 4))
 
     out = {
         "metric": METRIC,
@@ -466,7 +486,8 @@ def main():
         dk = {}
         for k, nm in enumerate(names):
             kms, kn = r["timing"][k]
-            dk[nm] = dict(ms_total=round(kms, 3), launches=kn, ms_avg=round(kms / max(kn, 1), 4))
+            if kn:
+                dk[nm] = dict(ms_total=round(kms, 3), launches=kn, ms_avg=round(kms / max(kn, 1), 4))
         out["dd_law"] = {"value": vips, "unit": "voxel-iters/s", "precision": main_key,
                          "steps": args.dd_steps, "warmup": 0,
                          "ms_per_step": 1e3 * r["wall"] / args.dd_steps,

@@ -154,6 +154,14 @@ int evpgpu_set_newton_shortcut(evpgpu_ctx *ctx, int on);
  * of seven; the results agree with the seven-pass path to rounding.  Off
  * forces the seven passes.  Default on. */
 int evpgpu_set_fused_transforms(evpgpu_ctx *ctx, int on);
+/* The Green loop's control (get_smacro, the error norms and the loop test,
+ * src/evolution.c:473-547,1354,1415-1458) runs on the device after every
+ * iteration.  With the fused transforms the host queues the iterations in
+ * batches of `batch` (1..4), two batches in flight, and reads the loop's
+ * flags once per batch; the iterations queued past the loop's end do
+ * nothing.  0 waits for every iteration.  Results are identical either way.
+ * Default 4.  (No counterpart in the reference, whose loop is host code.) */
+int evpgpu_set_loop_batch(evpgpu_ctx *ctx, int batch);
 /* Mixed precision: iterations between FP64 transforms of the stress anchor
  * (default 16, at least 1).  No counterpart in the reference (FP64 only). */
 int evpgpu_set_anchor_interval(evpgpu_ctx *ctx, int iterations);
@@ -207,6 +215,12 @@ int evpgpu_increment(evpgpu_ctx *ctx, int forced_iters, evpgpu_stats *stats);
  * mixed-precision loop (FP64 transform of sigma_A).  Reports the summed
  * elapsed milliseconds and the launch count since the last reset.            */
 int evpgpu_set_timing(evpgpu_ctx *ctx, int on);
+/* The timers recorded while timing is on (bit k: timer k), in one Green
+ * iteration of every `every` (after evpgpu_set_timing: all timers, every
+ * iteration).  Every pair of events costs the stream a few microseconds
+ * between kernels, so a throughput measurement records only the timers it
+ * reports, on a sample of the iterations. */
+int evpgpu_set_timing_groups(evpgpu_ctx *ctx, unsigned groups, int every);
 /* Wait for all work of the context's stream and of the device. */
 int evpgpu_synchronize(evpgpu_ctx *ctx);
 int evpgpu_get_timing(evpgpu_ctx *ctx, int which, double *ms, long long *launches);

@@ -5,8 +5,10 @@
 //   R2C(sigma, 6 comps) -> Gamma_eff (in place, antisym part accumulated)
 //   -> C2R fused with  E += sym(dDisGradAvg) + re/N
 //   -> per-voxel Newton update + block partials -> one deterministic row sum
-// and the host reads 16 doubles back to finish get_smacro (src/evolution.c:
-// 473-547), the error normalisation and the loop test.
+// -> get_smacro (src/evolution.c:473-547), the error normalisation and the
+// loop test in a one-workgroup kernel (loopctl.hip).  The control stays on
+// the device, so the host queues iterations in batches and reads the loop's
+// flags once per batch, while the device runs the next one.
 //
 // Mixed precision splits sigma = sigma_A + delta.  Every anchor_every
 // iterations sigma_A = fp32(sigma) is transformed in FP64 and its Green
@@ -31,68 +33,48 @@
 // ---- kernel timing ----
 void tmark(evpgpu_ctx *c, int k, int end)
 {
-    if (!c->timing) return;
-    (void)hipEventRecord(c->ev[k][end], c->stream);
-    if (end) c->tmask |= 1u << k;
+    if (!c->timing || c->tskip || !((c->timing_groups >> k) & 1u)) return;
+    (void)hipEventRecord(c->ev[c->tslot][k][end], c->stream);
+    if (end) c->tmask[c->tslot] |= 1u << k;
 }
 
-// add the elapsed times of the event pairs recorded since the last collect
-// (after the stream has been synchronised); a pair never recorded is never
-// queried, so no error is left behind for the next launch check
-void tcollect(evpgpu_ctx *c)
+// add the elapsed times of the event pairs recorded in a slot since its last
+// collect (once the device has passed them), or drop them (keep == 0: the
+// launches of an iteration queued past the end of the loop, which returned
+// at once); a pair never recorded is never queried, so no error is left
+// behind for the next launch check
+void tcollect_slot(evpgpu_ctx *c, int slot, int keep)
 {
     if (!c->timing) return;
-    for (int k = 0; k < EVP_NTIMERS; k++) {
-        if (!(c->tmask & (1u << k))) continue;
+    for (int k = 0; k < EVP_NTIMERS && keep; k++) {
+        if (!(c->tmask[slot] & (1u << k))) continue;
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev[k][0], c->ev[k][1]) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, c->ev[slot][k][0], c->ev[slot][k][1]) == hipSuccess) {
             c->t_ms[k] += ms;
             c->t_n[k]++;
         } else {
             (void)hipGetLastError();
         }
     }
-    c->tmask = 0;
+    c->tmask[slot] = 0;
+}
+
+void tkernel(evpgpu_ctx *c, int k, hipEvent_t *start, hipEvent_t *stop)
+{
+    *start = *stop = nullptr;
+    if (!c->timing || c->tskip || !((c->timing_groups >> k) & 1u)) return;
+    *start = c->ev[c->tslot][k][0];
+    *stop = c->ev[c->tslot][k][1];
+    c->tmask[c->tslot] |= 1u << k;
+}
+
+// the current slot (after the stream has been synchronised)
+void tcollect(evpgpu_ctx *c)
+{
+    tcollect_slot(c, c->tslot, 1);
 }
 
 namespace {
-
-const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
-
-void basis(double B[6][3][3])
-{
-    memset(B, 0, sizeof(double) * 54);
-    B[1][0][0] = -1. * RSQ6; B[1][1][1] = -1. * RSQ6; B[1][2][2] = 2. * RSQ6;
-    B[0][0][0] = -1. * RSQ2; B[0][1][1] = RSQ2;
-    B[2][1][2] = RSQ2; B[2][2][1] = RSQ2;
-    B[3][0][2] = RSQ2; B[3][2][0] = RSQ2;
-    B[4][0][1] = RSQ2; B[4][1][0] = RSQ2;
-    B[5][0][0] = RSQ3; B[5][1][1] = RSQ3; B[5][2][2] = RSQ3;
-}
-
-// chg_basis opt 2 / chg_basis5 opt 1 restricted to n components
-void t2_to_v(const double t[9], double *v, int n)
-{
-    double B[6][3][3];
-    basis(B);
-    for (int i = 0; i < n; i++) {
-        double s = 0.0;
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) s += t[a * 3 + b] * B[i][a][b];
-        v[i] = s;
-    }
-}
-void v_to_t2(const double *v, double t[9], int n)
-{
-    double B[6][3][3];
-    basis(B);
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) {
-            double s = 0.0;
-            for (int i = 0; i < n; i++) s += v[i] * B[i][a][b];
-            t[a * 3 + b] = s;
-        }
-}
 
 // VonMises (src/kinematics.c:465-484)
 double vonmises(const double t[9])
@@ -228,52 +210,40 @@ NewtonParams newton_params(const evpgpu_ctx *c)
     return P;
 }
 
-// get_smacro (src/evolution.c:473-547) on the reduced sums; Q3 fixed
-void smacro(evpgpu_ctx *c, const double *red)
+// the loading constants of the loop control's get_smacro
+CtlConst ctl_const(const evpgpu_ctx *c)
 {
-    const double wgt = 1.0 / (double)c->N;
-    double s[6], s1[6];
-    for (int k = 0; k < 6; k++) {
-        s[k] = red[3 + k] * wgt;
-        s1[k] = red[9 + k] * (wgt / c->ld.wgt_ph1);
-    }
-    sym6_to_full(s, c->m.sig_avg);
-    sym6_to_full(s1, c->m.sig_avg1);
-    for (int i = 0; i < 6; i++) {
-        int ii = IJV[0][i], jj = IJV[1][i];
-        double d = 0.0;
-        if (c->ld.velgrad_bc_flag[i] == 0)
-            for (int k = 0; k < 6; k++) {
-                int kk = IJV[0][k], ll = IJV[1][k];
-                d += c->ld.s0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * c->ld.stress_bc_flag[k] *
-                     (c->scauchy[kk * 3 + ll] - c->m.sig_avg[kk * 3 + ll]);
-            }
-        c->dD[ii * 3 + jj] = d;
-    }
-    for (int i = 0; i < 9; i++) c->dD_acum[i] += c->dD[i];
-    double v6[6], dev[9];
-    t2_to_v(c->m.sig_avg, v6, 6);
-    v_to_t2(v6, dev, 5);
-    double tr = dev[0] + dev[4] + dev[8], a = 0.0;
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            double d = dev[i * 3 + j] - (i == j ? tr / 3.0 : 0.0);
-            a += (i == j) ? d * d : dev[i * 3 + j] * dev[i * 3 + j];
-        }
-    c->m.s_vm = sqrt(3. / 2. * a);
-    t2_to_v(c->m.sig_avg1, v6, 6);
-    v_to_t2(v6, dev, 5);
-    a = 0.0;
-    for (int i = 0; i < 9; i++) a += dev[i] * dev[i];
-    c->m.s_vm1 = sqrt(3. / 2. * a);
+    CtlConst K;
+    memset(&K, 0, sizeof(K));
+    memcpy(K.s0, c->ld.s0, sizeof(K.s0));
+    memcpy(K.scauchy, c->scauchy, sizeof(K.scauchy));
+    K.wgt_ph1 = c->ld.wgt_ph1;
+    K.scale = 1.0 / (double)c->N;
+    memcpy(K.velgrad_bc, c->ld.velgrad_bc_flag, sizeof(K.velgrad_bc));
+    memcpy(K.stress_bc, c->ld.stress_bc_flag, sizeof(K.stress_bc));
+    return K;
 }
 
-// one Green iteration; leaves errors and macros updated.  prec FP64 runs the
-// FFT/Green/C2R path in FP64 on sigma; MIXED runs it in FP32 on
-// delta = sigma - sigma_A (anchor refreshed first when refresh != 0).
-int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, double *ls_trials)
+int green_iteration_queue(evpgpu_ctx *c, int prec, int refresh, const CtlConst &K);
+
+// One Green iteration, queued: prec FP64 runs the FFT/Green/C2R path in FP64
+// on sigma; MIXED runs it in FP32 on delta = sigma - sigma_A (anchor
+// refreshed first when refresh != 0).  The strain update adds the dD of the
+// last control step: read from the control block by the fused kernels, from
+// the host copy of it otherwise (the loop then waits for every iteration).
+// Ends with the loop control's step.
+int green_iteration(evpgpu_ctx *c, int prec, int refresh, const CtlConst &K)
 {
-    const double scale = 1.0 / (double)c->N;
+    // the timers record one Green iteration in timing_every
+    c->tskip = c->timing && (c->tcount++ % c->timing_every) != 0;
+    int st = green_iteration_queue(c, prec, refresh, K);
+    c->tskip = 0;
+    return st;
+}
+
+int green_iteration_queue(evpgpu_ctx *c, int prec, int refresh, const CtlConst &K)
+{
+    const double scale = K.scale;
     int st = EVPGPU_OK;
     const bool fused = c->fused && fused_ok(c);
     if (prec == EVPGPU_MIXED && refresh) {
@@ -289,18 +259,13 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
         if (st) return st;
         tmark(c, 5, 1);
     }
-    double add[6];
-    const double *d = c->dD;
-    add[0] = d[0]; add[1] = d[4]; add[2] = d[8];
-    add[3] = (d[5] + d[7]) / 2.0;
-    add[4] = (d[2] + d[6]) / 2.0;
-    add[5] = (d[1] + d[3]) / 2.0;
     if (fused) {
-        for (int i = 0; i < 9; i++) c->dD_applied[i] += d[i];
-        st = prec == EVPGPU_MIXED ? fused_iteration_delta(c, c->sig, c->sigA, c->E, add, c->cA, scale)
-                                  : fused_iteration_fp64(c, c->sig, c->E, add, scale);
+        const double *addd = (const double *)((const char *)c->ctl + offsetof(LoopCtl, add));
+        st = prec == EVPGPU_MIXED ? fused_iteration_delta(c, c->sig, c->sigA, c->E, addd, c->cA, scale)
+                                  : fused_iteration_fp64(c, c->sig, c->E, addd, scale);
         if (st) return st;
     } else {
+    const double *add = c->h_ctl[0].add;
     tmark(c, 1, 0);
     st = prec == EVPGPU_MIXED ? fft_forward_delta(c, c->sig, c->sigA) : fft_forward(c, c->sig, 6, prec);
     if (st) return st;
@@ -309,7 +274,6 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
     st = prec == EVPGPU_MIXED ? green_delta(c) : green_apply(c, prec, 1);
     if (st) return st;
     tmark(c, 2, 1);
-    for (int i = 0; i < 9; i++) c->dD_applied[i] += d[i];
     tmark(c, 3, 0);
     st = prec == EVPGPU_MIXED ? fft_inverse_delta(c, c->E, add, c->cA, scale)
                               : fft_inverse_update(c, 6, prec, c->E, add, scale);
@@ -318,39 +282,95 @@ int green_iteration(evpgpu_ctx *c, int prec, int refresh, double *newton_iters, 
     }
     NewtonParams P = newton_params(c);
     P.dd_first = c->dd_first;
+    P.skip = c->skip;
     c->dd_first = 0;
     int nblk = 0;
-    tmark(c, 0, 0);
+    hipEvent_t e0, e1;
+    tkernel(c, 0, &e0, &e1);
     st = newton_launch(c, P, c->sig, c->sig_alt, c->E, c->epsp, c->edot, c->tran, c->dd_on ? c->dd_k : c->crss,
                        c->gamdot, c->phase, c->orient, c->sgr, c->N, c->partials, &nblk, c->d_status, nullptr,
-                       nullptr, c->newton_shortcut);
+                       nullptr, c->newton_shortcut, e0, e1);
     if (st) return st;
     {
         double *t = c->sig;
         c->sig = c->sig_alt;
         c->sig_alt = t;
     }
-    tmark(c, 0, 1);
-    // the 16 sums and the Newton status in one read-back
-    st = reduce_partials(c, c->partials, nblk, NRED, c->red, c->d_status);
+    return ctl_iter_launch(c, nblk);
+}
+
+// the phase-end test of the loop control (k_ctl_phase) on the host copy
+bool phase_ends(const LoopCtl &L, int limit, int conv)
+{
+    return !(L.f.iter < limit && (!conv || fmax(fabs(L.err_s), fabs(L.err_e)) > L.tol));
+}
+
+// One phase of the Green loop: iterations of one precision while
+// iter < limit (and, with conv, while the error norms exceed Err); first
+// runs at least one.  With the fused kernels and loop_batch > 0 the
+// iterations are queued in batches of loop_batch, two batches in flight: the
+// host reads the older batch's flags while the device runs the newer one,
+// and the iterations queued past the end of the phase return at once.
+// Otherwise the host waits for every iteration.  On return the host copy of
+// the control block (h_ctl[0]) is current and c->sig is the last stress.
+int run_phase(evpgpu_ctx *c, int prec, int limit, int conv, int first, const CtlConst &K, int *imix)
+{
+    int st = ctl_phase_launch(c, limit, conv, first);
     if (st) return st;
-    EVP_CHECK(hipMemcpyAsync(c->h_red, c->red, (NRED + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    EVP_CHECK(hipStreamSynchronize(c->stream));
-    tcollect(c);
-    if (c->h_red[NRED] != 0.0) return (int)c->h_red[NRED];
-    // a non-finite rate (|x|^n overflowing) makes the voxel's error norm and
-    // stress non-finite, and with them these block sums; the reference stops
-    // on such values (assert(!isnan), src/constitutive.c:617,653,700)
-    for (int k = 0; k < NRED; k++)
-        if (!isfinite(c->h_red[k])) return EVPGPU_ERR_NONFINITE;
-    c->m.err_e = c->h_red[1] * scale;
-    c->m.err_s = c->h_red[0] * scale;
-    *newton_iters += c->h_red[2];
-    *ls_trials += c->h_red[15];
-    smacro(c, c->h_red);
-    if (fabs(c->m.e_vm) < 1E-5) c->m.err_e = 0.0;
-    else c->m.err_e /= c->m.e_vm;
-    c->m.err_s /= c->m.s_vm;
+    if (!first && phase_ends(c->h_ctl[0], limit, conv)) return EVPGPU_OK;
+    const bool ahead = c->fused && fused_ok(c) && c->loop_batch > 0;
+    const int B = ahead ? c->loop_batch : 1, depth = ahead ? 2 : 1;
+    const int ae = c->anchor_every > 0 ? c->anchor_every : 1;
+    const int iter0 = c->h_ctl[0].f.iter;
+    c->skip = ahead ? (const int *)((const char *)c->ctl + offsetof(LoopCtl, f.done)) : nullptr;
+    long long sent = 0, seen = 0, queued = 0;
+    bool done = false;
+    while (!done && !st) {
+        while (!st && sent - seen < depth) {
+            for (int b = 0; b < B && !st; b++) {
+                c->tslot = (int)((sent * B + b) % EVP_NSLOT);
+                const int refresh = prec == EVPGPU_MIXED && (*imix)++ % ae == 0;
+                st = green_iteration(c, prec, refresh, K);
+                if (!st) queued++;
+            }
+            if (st) break;
+            LoopCtl *h = ahead ? &c->h_ctl[1 + sent % 2] : &c->h_ctl[0];
+            const size_t bytes = ahead ? sizeof(LoopFlags) : sizeof(LoopCtl);
+            if (hipMemcpyAsync(h, c->ctl, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipEventRecord(c->bev[sent % 2], c->stream) != hipSuccess)
+                st = EVPGPU_ERR_HIP;
+            sent++;
+        }
+        if (st) break;
+        if (hipEventSynchronize(c->bev[seen % 2]) != hipSuccess) {
+            st = EVPGPU_ERR_HIP;
+            break;
+        }
+        const LoopFlags f = (ahead ? c->h_ctl[1 + seen % 2] : c->h_ctl[0]).f;
+        for (int b = 0; b < B; b++) {
+            const long long j = seen * B + b;
+            tcollect_slot(c, (int)(j % EVP_NSLOT), iter0 + j < f.iter);
+        }
+        seen++;
+        done = f.done != 0;
+    }
+    // the batches still queued run nothing; wait for them, drop their timer
+    // pairs and bring the control block back
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !st) st = EVPGPU_ERR_HIP;
+    for (long long j = seen * B; j < sent * B; j++) tcollect_slot(c, (int)(j % EVP_NSLOT), 0);
+    c->tslot = 0;
+    c->skip = nullptr;
+    if (st) return st;
+    if (ahead) EVP_CHECK(hipMemcpy(&c->h_ctl[0], c->ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost));
+    const LoopCtl &L = c->h_ctl[0];
+    if (L.f.status) return L.f.status;
+    // every iteration carried out swapped the stress buffers once; the
+    // queued ones that returned at once did not
+    if ((queued - (L.f.iter - iter0)) % 2) {
+        double *t = c->sig;
+        c->sig = c->sig_alt;
+        c->sig_alt = t;
+    }
     return EVPGPU_OK;
 }
 
@@ -451,6 +471,12 @@ int evpgpu_create(evpgpu_ctx **out, int nx, int ny, int nz, int precision)
     if (!st) st = dalloc(&c->d_status, 1);
     if (!st && hipHostMalloc((void **)&c->h_red, (NRED + 1) * sizeof(double), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
     if (!st && hipHostMalloc((void **)&c->h_status, sizeof(int), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
+    if (!st && hipMalloc((void **)&c->ctl, sizeof(LoopCtl)) != hipSuccess) st = EVPGPU_ERR_NOMEM;
+    if (!st && hipMalloc((void **)&c->ctl_k, sizeof(CtlConst)) != hipSuccess) st = EVPGPU_ERR_NOMEM;
+    if (!st && hipHostMalloc((void **)&c->h_ctl, 3 * sizeof(LoopCtl), 0) != hipSuccess) st = EVPGPU_ERR_NOMEM;
+    for (int b = 0; b < 2 && !st; b++)
+        if (hipEventCreateWithFlags(&c->bev[b], hipEventDisableTiming) != hipSuccess) st = EVPGPU_ERR_HIP;
+    c->loop_batch = 4;
     if (!st) st = fft_twiddles(c);
     if (!st) {
         // fields start at zero (gamdot of gas voxels is never written)
@@ -472,15 +498,19 @@ int evpgpu_destroy(evpgpu_ctx *c)
     adjoint_free(c);
     void *ptrs[] = {c->sig, c->sig_alt, c->E, c->W, c->epsp, c->edot, c->tran, c->crss, c->gamacum, c->gamdot,
                     c->phase, c->grain, c->orient, c->sgr, c->spec, c->acc, c->rwork,
-                    c->sigA, c->cA, c->skA, c->Estart, c->partials, c->red, c->red2, c->d_status, c->tw64[0], c->tw64[1], c->tw64[2],
+                    c->sigA, c->cA, c->skA, c->Estart, c->partials, c->red, c->red2, c->d_status, c->ctl, c->ctl_k, c->tw64[0], c->tw64[1], c->tw64[2],
                     c->tw32[0], c->tw32[1], c->tw32[2], c->rho, c->dd_k, c->dd_tab};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
         if (ptrs[i]) (void)hipFree(ptrs[i]);
-    for (int k = 0; k < EVP_NTIMERS; k++)
-        for (int e = 0; e < 2; e++)
-            if (c->ev[k][e]) (void)hipEventDestroy(c->ev[k][e]);
+    for (int q = 0; q < EVP_NSLOT; q++)
+        for (int k = 0; k < EVP_NTIMERS; k++)
+            for (int e = 0; e < 2; e++)
+                if (c->ev[q][k][e]) (void)hipEventDestroy(c->ev[q][k][e]);
+    for (int b = 0; b < 2; b++)
+        if (c->bev[b]) (void)hipEventDestroy(c->bev[b]);
     free(c->h_sgr);
     if (c->h_red) (void)hipHostFree(c->h_red);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     free(c);
@@ -490,11 +520,18 @@ int evpgpu_destroy(evpgpu_ctx *c)
 int evpgpu_set_timing(evpgpu_ctx *c, int on)
 {
     if (!c) return EVPGPU_ERR_ARG;
-    if (on && !c->ev[0][0])
-        for (int k = 0; k < EVP_NTIMERS; k++)
-            for (int e = 0; e < 2; e++) EVP_CHECK(hipEventCreate(&c->ev[k][e]));
+    if (on && !c->ev[EVP_NSLOT - 1][EVP_NTIMERS - 1][1])
+        for (int q = 0; q < EVP_NSLOT; q++)
+            for (int k = 0; k < EVP_NTIMERS; k++)
+                for (int e = 0; e < 2; e++)
+                    if (!c->ev[q][k][e]) EVP_CHECK(hipEventCreate(&c->ev[q][k][e]));
     c->timing = on ? 1 : 0;
-    c->tmask = 0;
+    c->timing_groups = (1u << EVP_NTIMERS) - 1;
+    c->timing_every = 1;
+    c->tcount = 0;
+    c->tskip = 0;
+    c->tslot = 0;
+    memset(c->tmask, 0, sizeof(c->tmask));
     memset(c->t_ms, 0, sizeof(c->t_ms));
     memset(c->t_n, 0, sizeof(c->t_n));
     return EVPGPU_OK;
@@ -513,6 +550,22 @@ int evpgpu_get_timing(evpgpu_ctx *c, int which, double *ms, long long *launches)
     if (!c || which < 0 || which >= EVP_NTIMERS) return EVPGPU_ERR_ARG;
     if (ms) *ms = c->t_ms[which];
     if (launches) *launches = c->t_n[which];
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_timing_groups(evpgpu_ctx *c, unsigned groups, int every)
+{
+    if (!c || groups >= (1u << EVP_NTIMERS) || every < 1) return EVPGPU_ERR_ARG;
+    c->timing_groups = groups;
+    c->timing_every = every;
+    c->tcount = 0;
+    return EVPGPU_OK;
+}
+
+int evpgpu_set_loop_batch(evpgpu_ctx *c, int batch)
+{
+    if (!c || batch < 0 || batch > EVP_NSLOT / 2) return EVPGPU_ERR_ARG;
+    c->loop_batch = batch;
     return EVPGPU_OK;
 }
 
@@ -760,81 +813,66 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     tmark(c, 4, 1);
     EVP_CHECK(hipStreamSynchronize(c->stream));
     tcollect(c);
-    memset(c->dD, 0, sizeof(c->dD));
-    memset(c->dD_acum, 0, sizeof(c->dD_acum));
-    memset(c->dD_applied, 0, sizeof(c->dD_applied));
-    int iter = 0, it64 = 0;
-    m.err_e = 2. * c->ld.err_tol;
-    m.err_s = 2. * c->ld.err_tol;
-    const double tol = c->ld.err_tol;
-    double nit = 0.0, ntr = 0.0;
+    // the control block at the start of the loop: no macro strain correction
+    // yet, error norms above Err; the macro stresses stay the last
+    // increment's when the loop runs no iteration
+    LoopCtl init;
+    memset(&init, 0, sizeof(init));
+    init.tol = c->ld.err_tol;
+    init.e_vm = m.e_vm;
+    init.err_e = 2. * c->ld.err_tol;
+    init.err_s = 2. * c->ld.err_tol;
+    memcpy(init.sig_avg, m.sig_avg, sizeof(init.sig_avg));
+    memcpy(init.sig_avg1, m.sig_avg1, sizeof(init.sig_avg1));
+    init.s_vm = m.s_vm;
+    init.s_vm1 = m.s_vm1;
+    c->h_ctl[0] = init;
+    const CtlConst K = ctl_const(c);
+    st = ctl_init_launch(c, init, K);
+    if (st) return st;
     double t0 = now_s();
-    auto more = [&](void) {
-        return (iter < c->ld.iter_max) && (fmax(fabs(m.err_s), fabs(m.err_e)) > tol);
-    };
     const bool mixed = c->precision == EVPGPU_MIXED;
     // the strain at the start of the loop: the mixed loop's projection and
     // the antisymmetric increment at its end use it
     EVP_CHECK(hipMemcpyAsync(c->Estart, c->E, 6 * c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    const int ae = c->anchor_every > 0 ? c->anchor_every : 1;
-    int imix = 0;
-    auto mixed_step = [&](void) {
-        iter++;
-        int r = green_iteration(c, EVPGPU_MIXED, imix % ae == 0, &nit, &ntr);
-        imix++;
+    int imix = 0, it_mixed = 0;
+    // the loop's phases (limits count every iteration of the increment)
+    auto mixed_phase = [&](int limit, int conv) {
+        int r = run_phase(c, EVPGPU_MIXED, limit, conv, 0, K, &imix);
+        it_mixed = c->h_ctl[0].f.iter;
+        if (!r) r = project_increment(c);
         return r;
     };
-    auto fp64_step = [&](void) {
-        iter++;
-        it64++;
-        return green_iteration(c, EVPGPU_FP64, 0, &nit, &ntr);
+    auto fp64_phase = [&](int limit, int conv, int first) {
+        return run_phase(c, EVPGPU_FP64, limit, conv, first, K, &imix);
     };
     if (rep64 >= 0) {
-        if (mixed) {
-            for (int i = 0; i < rep_mix; i++) {
-                st = mixed_step();
-                if (st) return st;
-            }
-            st = project_increment(c);
-            if (st) return st;
-        }
-        for (int i = 0; i < rep64; i++) {
-            st = fp64_step();
-            if (st) return st;
-        }
+        if (mixed) st = mixed_phase(rep_mix, 0);
+        if (!st) st = fp64_phase(c->h_ctl[0].f.iter + rep64, 0, 0);
     } else if (forced > 0) {
-        if (mixed) {
-            while (iter < forced - 1) {
-                st = mixed_step();
-                if (st) return st;
-            }
-            st = project_increment(c);
-            if (st) return st;
-        }
-        while (iter < forced) {
-            st = fp64_step();
-            if (st) return st;
-        }
+        if (mixed) st = mixed_phase(forced - 1, 0);
+        if (!st) st = fp64_phase(forced, 0, 0);
     } else if (mixed) {
         // the FP64 sweep that closes the mixed loop counts against IterMax:
-        // the mixed iterations stop one short of it
-        while (more() && iter < c->ld.iter_max - 1) {
-            st = mixed_step();
-            if (st) return st;
-        }
-        st = project_increment(c);
-        if (st) return st;
-        // FP64 correction sweeps: at least one, until converged
-        do {
-            st = fp64_step();
-            if (st) return st;
-        } while (more());
+        // the mixed iterations stop one short of it; then FP64 correction
+        // sweeps, at least one, until converged
+        st = mixed_phase(c->ld.iter_max - 1, 1);
+        if (!st) st = fp64_phase(c->ld.iter_max, 1, 1);
     } else {
-        while (more()) {
-            st = fp64_step();
-            if (st) return st;
-        }
+        st = fp64_phase(c->ld.iter_max, 1, 0);
     }
+    if (st) return st;
+    const LoopCtl &L = c->h_ctl[0];
+    const int iter = L.f.iter, it64 = iter - it_mixed;
+    m.err_e = L.err_e;
+    m.err_s = L.err_s;
+    memcpy(m.sig_avg, L.sig_avg, sizeof(m.sig_avg));
+    memcpy(m.sig_avg1, L.sig_avg1, sizeof(m.sig_avg1));
+    m.s_vm = L.s_vm;
+    m.s_vm1 = L.s_vm1;
+    memcpy(c->dD, L.dD, sizeof(c->dD));
+    memcpy(c->dD_acum, L.dD_acum, sizeof(c->dD_acum));
+    memcpy(c->dD_applied, L.dD_applied, sizeof(c->dD_applied));
     // The loop's Newton launches may end an iteration whose step the
     // reference rejects before evaluating its trial point; gamdot comes from
     // that trial (Q5) and is read only by the epilogue, so the last launch is
@@ -844,12 +882,12 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
         NewtonParams P = newton_params(c);
         P.dd_first = c->dd_on && iter == 1;
         int nblk = 0;
-        tmark(c, 0, 0);
+        hipEvent_t e0, e1;
+        tkernel(c, 0, &e0, &e1);
         st = newton_launch(c, P, c->sig_alt, c->sig, c->E, c->epsp, c->edot, c->tran,
                            c->dd_on ? c->dd_k : c->crss, c->gamdot, c->phase, c->orient, c->sgr, c->N,
-                           c->partials, &nblk, c->d_status, nullptr, nullptr, 0);
+                           c->partials, &nblk, c->d_status, nullptr, nullptr, 0, e0, e1);
         if (st) return st;
-        tmark(c, 0, 1);
         EVP_CHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         EVP_CHECK(hipStreamSynchronize(c->stream));
         tcollect(c);
@@ -861,8 +899,8 @@ int increment_core(evpgpu_ctx *c, int forced, int rep_mix, int rep64, evpgpu_sta
     S.err_e = m.err_e;
     S.iters = iter;
     S.fp64_iters = it64;
-    S.newton_iters = nit;
-    S.ls_trials = ntr;
+    S.newton_iters = L.nit;
+    S.ls_trials = L.ntr;
 
     // epilogue (src/evolution.c:1469-1547)
     // antisymmetric displacement-gradient increment: uniform part + IFFT of the

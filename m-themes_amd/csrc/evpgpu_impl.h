@@ -49,6 +49,8 @@ struct NewtonParams {
     int dd_first;    // this launch makes each voxel's first rate call of the
                      // increment (the passing stress then uses the previous
                      // call's mobile density)
+    const int *skip; // device flag: the launch does nothing when it is set
+                     // (an iteration queued past the end of the Green loop)
 };
 
 struct GreenParams {
@@ -85,12 +87,44 @@ struct EpilogueParams {
 // timer slots: 0 Newton, 1 forward FFT, 2 Green, 3 inverse FFT + strain update,
 // 4 increment prologue/epilogue, 5 anchor refresh of the mixed-precision loop
 #define EVP_NTIMERS 6
+// timer slots per timer: one per Green iteration queued and not yet collected
+#define EVP_NSLOT 8
 
 #define NRED 16
 // epilogue partials: sum eps*wgt (6), sum edot*wgt (6)
 #define NRED_EPI 12
 // first-stage workgroups of the partial-sum reduction
 #define REDUCE_G 512
+
+// The Green loop's control, kept on the device so that the host can queue
+// iterations ahead of it: after each iteration's Newton reduction a
+// one-workgroup kernel finishes get_smacro, the error norms and the stopping
+// test (src/evolution.c:1415-1458), and sets `done` when the loop phase ends;
+// the kernels of the iterations queued after that see `done` and return.
+struct LoopFlags {
+    int iter;    // Green iterations carried out in the increment
+    int done;    // the phase has ended (converged, at its limit, or failed)
+    int status;  // EVPGPU_OK or the error that ended it
+    int pad;
+};
+struct LoopCtl {
+    LoopFlags f;
+    int limit;   // the phase runs while iter < limit ...
+    int conv;    // ... and, when conv, while max(|err_s|, |err_e|) > tol
+    double tol, e_vm;
+    double dD[9], dD_acum[9], dD_applied[9];
+    double add[6];              // the symmetric part of dD, as the strain update adds it
+    double sig_avg[9], sig_avg1[9], s_vm, s_vm1;
+    double err_s, err_e;
+    double nit, ntr;            // Newton iterations and line-search trials, summed
+};
+// the loading constants get_smacro reads
+struct CtlConst {
+    double s0[81], scauchy[9];
+    double wgt_ph1, scale;
+    int velgrad_bc[6], stress_bc[6];
+};
+static_assert(sizeof(LoopCtl) % 8 == 0 && sizeof(CtlConst) % 8 == 0, "control blocks are whole doubles");
 
 struct evpgpu_ctx {
     int device;
@@ -148,10 +182,25 @@ struct evpgpu_ctx {
     int *d_status;
     int *h_status;
 
-    // kernel timing (HIP events), see evpgpu_set_timing
+    // device-side loop control (loopctl.hip)
+    LoopCtl *ctl;     // device
+    CtlConst *ctl_k;  // device: the increment's constants
+    LoopCtl *h_ctl;   // pinned host: [0] the last full copy, [1..2] the flags of
+                      // the two batches in flight (f only)
+    hipEvent_t bev[2];
+    const int *skip;  // &ctl->f.done while iterations are queued ahead, else null
+    int loop_batch;   // iterations per queued batch (0: one at a time, waited for)
+
+    // kernel timing (HIP events), see evpgpu_set_timing; one set of pairs per
+    // slot, the slot of the iteration being queued in tslot
     int timing;
-    unsigned tmask;   // pairs recorded since the last collect
-    hipEvent_t ev[EVP_NTIMERS][2];
+    unsigned timing_groups; // the timers recorded (bit k: timer k) ...
+    int timing_every;       // ... in one Green iteration of every timing_every
+    long long tcount;       // Green iterations queued since timing was set
+    int tskip;              // the iteration being queued is not timed
+    int tslot;
+    unsigned tmask[EVP_NSLOT]; // pairs recorded since the slot's last collect
+    hipEvent_t ev[EVP_NSLOT][EVP_NTIMERS][2];
     double t_ms[EVP_NTIMERS];
     long long t_n[EVP_NTIMERS];
 
@@ -215,6 +264,11 @@ int green_anchor(evpgpu_ctx *c);
 // HIP-event timing (no-ops unless evpgpu_set_timing is on)
 void tmark(evpgpu_ctx *c, int k, int end);
 void tcollect(evpgpu_ctx *c);
+// collect (keep != 0) or drop the pairs recorded in one slot
+void tcollect_slot(evpgpu_ctx *c, int slot, int keep);
+// the event pair of timer k in the current slot for a launch that records
+// its own start and end (hipExtLaunchKernelGGL), or nulls when not timing
+void tkernel(evpgpu_ctx *c, int k, hipEvent_t *start, hipEvent_t *stop);
 
 bool fused_ok(const evpgpu_ctx *c);
 // the Green operator's parameters from the context's loading
@@ -230,13 +284,22 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
                   const double *epsp, double *edot, const double *tran, const double *crss,
                   double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status,
-                  double *err_vox, int *niter_vox, int fast);
+                  double *err_vox, int *niter_vox, int fast, hipEvent_t ev_start = nullptr,
+                  hipEvent_t ev_stop = nullptr);
 // the Newton kernel's parameters of a context (evpgpu.hip)
 NewtonParams context_newton_params(const evpgpu_ctx *c);
 // column sums of partials[nblk][nval] into out[0..nval); with status, also
 // out[nval] = *status (out must then hold nval + 1 values)
 int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, double *out,
                     const int *status = nullptr);
+// the first stage alone (into c->red2, skipped when *skip is set); returns
+// the number of stage-1 rows in *g
+int reduce_stage1(evpgpu_ctx *c, const double *partials, int nblk, int nval, const int *skip, int *g);
+// loopctl.hip: the device-side Green-loop control
+int ctl_init_launch(evpgpu_ctx *c, const LoopCtl &init, const CtlConst &K);
+int ctl_phase_launch(evpgpu_ctx *c, int limit, int conv, int first);
+// the Newton partials of one iteration -> get_smacro, norms, stopping test
+int ctl_iter_launch(evpgpu_ctx *c, int nblk);
 // fields.hip
 int prologue_launch(evpgpu_ctx *c, const double *dE6, int creep, const double *macro_sym6);
 int antisym_add_launch(evpgpu_ctx *c, const double *dW3);

@@ -46,11 +46,13 @@ template <typename R, int NY, int NZ, int IN>
 __global__ __launch_bounds__(NTH) void k_zy_r2c(const double *__restrict__ in, const float *__restrict__ inA,
                                                 typename CT<R>::T *__restrict__ out, long long N, long long NK,
                                                 int nx, const typename CT<R>::T *__restrict__ twy,
-                                                const typename CT<R>::T *__restrict__ twz)
+                                                const typename CT<R>::T *__restrict__ twz,
+                                                const int *__restrict__ skip)
 {
     typedef Plane<R, NY, NZ> PL;
     typedef typename PL::C C;
     constexpr int M = PL::M, P = PL::P;
+    if (skip && *skip) return;
     __shared__ C lds[NY * P];
     const int x = blockIdx.x, comp = blockIdx.y;
     const long long base = comp * N + (long long)x * NY * NZ;
@@ -88,19 +90,22 @@ __global__ __launch_bounds__(NTH) void k_zy_r2c(const double *__restrict__ in, c
 }
 
 // MODE 0: target += add + re*scale; 1: target = re*scale; 2: target += add +
-// fa + re*scale; 3: fo = (float)(re*scale); 4: target = base + re*scale
+// fa + re*scale; 3: fo = (float)(re*scale); 4: target = base + re*scale.
+// add is addd[comp] when addd is given (device memory), else a0..a5.
 template <typename R, int NY, int NZ, int MODE>
 __global__ __launch_bounds__(NTH) void k_yz_c2r(const typename CT<R>::T *__restrict__ in, double *target,
                                                 const float *__restrict__ fa, float *__restrict__ fo,
                                                 const double *__restrict__ basef, long long N, long long NK,
                                                 double a0, double a1, double a2, double a3, double a4,
-                                                double a5, double scale,
+                                                double a5, const double *__restrict__ addd, double scale,
                                                 const typename CT<R>::T *__restrict__ twy,
-                                                const typename CT<R>::T *__restrict__ twz)
+                                                const typename CT<R>::T *__restrict__ twz,
+                                                const int *__restrict__ skip)
 {
     typedef Plane<R, NY, NZ> PL;
     typedef typename PL::C C;
     constexpr int M = PL::M, P = PL::P;
+    if (skip && *skip) return;
     __shared__ C lds[NY * P];
     const int x = blockIdx.x, comp = blockIdx.y;
     const C *src = in + comp * NK + (long long)x * NY * P;
@@ -113,7 +118,8 @@ __global__ __launch_bounds__(NTH) void k_yz_c2r(const typename CT<R>::T *__restr
     }
     __syncthreads();
     gstages<true, M, 0, 1, NY, P, 1, 0, NTH, 2>(lds, twz);
-    const double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
+    const double add = addd ? addd[comp]
+                            : comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
     const long long base = comp * N + (long long)x * NY * NZ;
     for (int i = threadIdx.x; i < NY * M; i += NTH) {
         const int y = i / M, n = i % M;
@@ -142,11 +148,13 @@ __global__ __launch_bounds__(NTH) void k_yz_c2r(const typename CT<R>::T *__restr
 template <typename R, int NX, int TBX, int GMODE>
 __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__ spec, double2 *__restrict__ acc,
                                                  float2 *__restrict__ skA, GreenParams G, int ny, int nz,
-                                                 int nzh, long long NK, const typename CT<R>::T *__restrict__ twx)
+                                                 int nzh, long long NK, const typename CT<R>::T *__restrict__ twx,
+                                                 const int *__restrict__ skip)
 {
     typedef typename CT<R>::T C;
     constexpr int PX = NX + 1;
     constexpr int NL = 6 * TBX;
+    if (skip && *skip) return;
     __shared__ C lds[NL * PX];
     __shared__ R c0s[36];
     if (threadIdx.x < 36) c0s[threadIdx.x] = (R)G.ga[threadIdx.x];
@@ -288,15 +296,17 @@ int zy(evpgpu_ctx *c, const double *in, const float *inA)
     C *spec = (C *)c->spec;
 #define CALLP(NY_, NZ_)                                                                                   \
     hipLaunchKernelGGL((k_zy_r2c<R, NY_, NZ_, IN>), dim3(c->nx, 6), dim3(NTH), 0, c->stream, in, inA, spec, \
-                       c->N, c->NK, c->nx, twp<R>(c, 1), twp<R>(c, 2))
+                       c->N, c->NK, c->nx, twp<R>(c, 1), twp<R>(c, 2), c->skip)
     PLANE_DISPATCH(CALLP);
 #undef CALLP
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 
+// addc: host values of the constant added per component (or null: none);
+// addd: the same in device memory (read by the kernel, overrides addc)
 template <typename R, int MODE>
 int yz(evpgpu_ctx *c, double *target, const double *addc, double scale, const float *fa, float *fo,
-       const double *basef)
+       const double *basef, const double *addd = nullptr)
 {
     typedef typename CT<R>::T C;
     const C *spec = (const C *)c->spec;
@@ -305,8 +315,8 @@ int yz(evpgpu_ctx *c, double *target, const double *addc, double scale, const fl
         for (int i = 0; i < 6; i++) a[i] = addc[i];
 #define CALLP(NY_, NZ_)                                                                                    \
     hipLaunchKernelGGL((k_yz_c2r<R, NY_, NZ_, MODE>), dim3(c->nx, 6), dim3(NTH), 0, c->stream, spec, target, \
-                       fa, fo, basef, c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5], scale, twp<R>(c, 1),  \
-                       twp<R>(c, 2))
+                       fa, fo, basef, c->N, c->NK, a[0], a[1], a[2], a[3], a[4], a[5], addd, scale,          \
+                       twp<R>(c, 1), twp<R>(c, 2), c->skip)
     PLANE_DISPATCH(CALLP);
 #undef CALLP
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
@@ -322,7 +332,7 @@ int xg(evpgpu_ctx *c)
         constexpr int T = tbx<R, NX_>();                                                                 \
         hipLaunchKernelGGL((k_x_green<R, NX_, T, GMODE>), dim3((c->ny * c->nzh + T - 1) / T), dim3(NTX), 0,  \
                            c->stream, (C *)c->spec, c->acc, c->skA, G, c->ny, c->nz, c->nzh, c->NK,      \
-                           twp<R>(c, 0));                                                                \
+                           twp<R>(c, 0), c->skip);                                                       \
     }
     LINE_DISPATCH(CALLL);
 #undef CALLL
@@ -344,6 +354,7 @@ bool fused_ok(const evpgpu_ctx *c)
 }
 
 // FP64 iteration: spec = R2C(sigma) ; Green (acc += antisym) ; E += add + C2R/N
+// (add: 6 values in device memory)
 int fused_iteration_fp64(evpgpu_ctx *c, const double *sig, double *E, const double *add, double scale)
 {
     tmark(c, 1, 0);
@@ -353,13 +364,13 @@ int fused_iteration_fp64(evpgpu_ctx *c, const double *sig, double *E, const doub
     if (!st) st = xg<double, 0>(c);
     tmark(c, 2, 1);
     tmark(c, 3, 0);
-    if (!st) st = yz<double, 0>(c, E, add, scale, nullptr, nullptr, nullptr);
+    if (!st) st = yz<double, 0>(c, E, nullptr, scale, nullptr, nullptr, nullptr, add);
     tmark(c, 3, 1);
     return st;
 }
 
 // mixed loop, delta step (FP32): spec = R2C(sigma - sigma_A) ; Green (acc += antisym + skA) ;
-// E += add + c_A + C2R/N
+// E += add + c_A + C2R/N (add: 6 values in device memory)
 int fused_iteration_delta(evpgpu_ctx *c, const double *sig, const float *sigA, double *E, const double *add,
                           const float *cA, double scale)
 {
@@ -370,7 +381,7 @@ int fused_iteration_delta(evpgpu_ctx *c, const double *sig, const float *sigA, d
     if (!st) st = xg<float, 2>(c);
     tmark(c, 2, 1);
     tmark(c, 3, 0);
-    if (!st) st = yz<float, 2>(c, E, add, scale, cA, nullptr, nullptr);
+    if (!st) st = yz<float, 2>(c, E, nullptr, scale, cA, nullptr, nullptr, add);
     tmark(c, 3, 1);
     return st;
 }

@@ -26,8 +26,10 @@ __global__ void k_fill_int(int *p, long long n)
     if (i < n) p[i] = 0;
 }
 
-__global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, long long n)
+__global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, long long n,
+                         const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = (float)src[i];
 }
@@ -251,7 +253,8 @@ int fill_zero_int(evpgpu_ctx *c, int *p, long long n)
 
 int to_f32_launch(evpgpu_ctx *c, const double *src, float *dst, long long n)
 {
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, src, dst, n);
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, src, dst, n,
+                       c->skip);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }
 

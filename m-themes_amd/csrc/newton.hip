@@ -27,6 +27,8 @@
 // registers with compile-time indices only.  One wave per SIMD: the solve
 // holds about 420 registers, so the code is shaped for latency (next Schmid
 // vector read ahead, constants hoisted, no scalar loops on the hot path).
+#include <hip/hip_ext.h>
+
 #include "evpgpu_impl.h"
 #include "linesearch.h"
 
@@ -487,6 +489,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
     const double *__restrict__ crss, double *__restrict__ gamdot, double *__restrict__ partials,
     int *__restrict__ status, double *__restrict__ err_vox, int *__restrict__ niter_vox)
 {
+    if (P.skip && *P.skip) return;
     __shared__ double slab[EVPGPU_NSYSMX * 5 * NB_THREADS];
     // C0 and S0 (packed upper triangles) as broadcast LDS reads: kept in
     // scalar registers they do not fit, and each use of a spilled one costs
@@ -671,19 +674,25 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
 constexpr int RED_MAXV = 16;
 
 // status (stage 2 only, may be null): the Newton status word, appended as
-// out[nval] so that one copy brings back the sums and the status
+// out[nval] so that one copy brings back the sums and the status.  skip (may
+// be null): the launch does nothing when *skip is set.
 __global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ partials, int nblk,
                                                      int nval, int per, double *__restrict__ out,
-                                                     const int *__restrict__ status)
+                                                     const int *__restrict__ status,
+                                                     const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     if (status && blockIdx.x == 0 && threadIdx.x == 0) out[nval] = (double)*status;
     __shared__ double sh[256];
     const int v = threadIdx.x % RED_MAXV, r0 = threadIdx.x / RED_MAXV;
     const long long b0 = (long long)blockIdx.x * per;
     const long long b1 = min((long long)nblk, b0 + per);
     double s = 0.0;
-    if (v < nval)
+    if (v < nval) {
+        // unrolled so that the loads go out together (the sum keeps its order)
+#pragma unroll 4
         for (long long b = b0 + r0; b < b1; b += 256 / RED_MAXV) s += partials[b * nval + v];
+    }
     sh[threadIdx.x] = s;
     __syncthreads();
     if (threadIdx.x < RED_MAXV) {
@@ -699,7 +708,7 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
                   const double *epsp, double *edot, const double *tran, const double *crss,
                   double *gamdot, const int *phase, const int *orient, const double *sgr,
                   long long n, double *partials, int *nblk_out, int *status, double *err_vox,
-                  int *niter_vox, int fast)
+                  int *niter_vox, int fast, hipEvent_t ev_start, hipEvent_t ev_stop)
 {
     int nblk = (int)((n + NB_THREADS - 1) / NB_THREADS);
     // the integral-exponent variants need every crystal phase to have one;
@@ -727,10 +736,12 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
         }
         if (same && (n0 == 10 || n0 == 20)) ne = n0;
     }
-#define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, P, n, phase, orient, sgr, sig_in, sig, E, \
-                    epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
-#define LAUNCH(A, B, F) hipLaunchKernelGGL((k_newton<A, B, F, 0>), NEWTON_ARGS)
-#define LAUNCH_NE(E, F) hipLaunchKernelGGL((k_newton<RM_FAST, true, F, E>), NEWTON_ARGS)
+    // the events (when given) take the kernel's own start and end times: no
+    // marker packets between the launches
+#define NEWTON_ARGS dim3(nblk), dim3(NB_THREADS), 0, c->stream, ev_start, ev_stop, 0, P, n, phase, orient, sgr, \
+                    sig_in, sig, E, epsp, edot, tran, crss, gamdot, partials, status, err_vox, niter_vox
+#define LAUNCH(A, B, F) hipExtLaunchKernelGGL((k_newton<A, B, F, 0>), NEWTON_ARGS)
+#define LAUNCH_NE(E, F) hipExtLaunchKernelGGL((k_newton<RM_FAST, true, F, E>), NEWTON_ARGS)
 #define LAUNCH_RM(B, F)                                                                  \
     do {                                                                                 \
         if (ne == 10) LAUNCH_NE(10, F);                                                  \
@@ -765,8 +776,22 @@ int reduce_partials(evpgpu_ctx *c, const double *partials, int nblk, int nval, d
     int per = (nblk + g - 1) / g;
     g = (nblk + per - 1) / per;
     hipLaunchKernelGGL(k_reduce_rows, dim3(g), dim3(256), 0, c->stream, partials, nblk, nval, per, c->red2,
-                       (const int *)nullptr);
+                       (const int *)nullptr, (const int *)nullptr);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->stream, (const double *)c->red2, g, nval, g,
-                       out, status);
+                       out, status, (const int *)nullptr);
+    return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
+}
+
+// the first stage of reduce_partials alone: c->red2 = g rows of nval sums
+// (the second stage's input, summed the same way by the loop control)
+int reduce_stage1(evpgpu_ctx *c, const double *partials, int nblk, int nval, const int *skip, int *g_out)
+{
+    if (nval > RED_MAXV) return EVPGPU_ERR_ARG;
+    int g = nblk < REDUCE_G ? nblk : REDUCE_G;
+    int per = (nblk + g - 1) / g;
+    g = (nblk + per - 1) / per;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(g), dim3(256), 0, c->stream, partials, nblk, nval, per, c->red2,
+                       (const int *)nullptr, skip);
+    *g_out = g;
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
 }

@@ -89,6 +89,8 @@ EVPGPU_SYMBOLS = [
     "evpgpu_newton_device_noalloc",
     "evpgpu_set_anchor_interval", "evpgpu_set_newton_tolerances", "evpgpu_set_newton_shortcut",
     "evpgpu_set_fused_transforms",
+    "evpgpu_set_loop_batch",
+    "evpgpu_set_timing_groups",
     "evpgpu_set_phases", "evpgpu_set_loading", "evpgpu_upload_microstructure",
     "evpgpu_upload_state", "evpgpu_download_state", "evpgpu_set_macro", "evpgpu_get_macro",
     "evpgpu_increment", "evpgpu_set_timing", "evpgpu_synchronize", "evpgpu_get_timing", "evpgpu_fft_r2c_host", "evpgpu_fft_c2r_host", "evpgpu_green_host",
@@ -142,6 +144,8 @@ def gpu():
         L.evpgpu_set_newton_tolerances.argtypes = [VP, C.c_double, C.c_double]
         L.evpgpu_set_newton_shortcut.argtypes = [VP, C.c_int]
         L.evpgpu_set_fused_transforms.argtypes = [VP, C.c_int]
+        L.evpgpu_set_loop_batch.argtypes = [VP, C.c_int]
+        L.evpgpu_set_timing_groups.argtypes = [VP, C.c_uint, C.c_int]
         L.evpgpu_set_phases.argtypes = [VP, C.POINTER(Phase), C.c_int]
         L.evpgpu_set_loading.argtypes = [VP, C.POINTER(Loading)]
         L.evpgpu_upload_microstructure.argtypes = [VP, IP, IP, IP, C.c_int, DP]
@@ -250,8 +254,15 @@ class GpuJob:
     def reset(self):
         _chk(evp().evp_job_reset(self.h), "evp_job_reset")
 
-    def set_timing(self, on: bool):
+    def set_timing(self, on: bool, groups=None, every: int = 1):
+        """HIP-event timers on or off; groups: the timer indices to record (all by
+        default), in one Green iteration of every `every`."""
         _chk(gpu().evpgpu_set_timing(self.ctx, 1 if on else 0), "evpgpu_set_timing")
+        if on and (groups is not None or every != 1):
+            mask = 0
+            for k in (range(6) if groups is None else groups):
+                mask |= 1 << int(k)
+            _chk(gpu().evpgpu_set_timing_groups(self.ctx, mask, int(every)), "evpgpu_set_timing_groups")
 
     def timing(self, which: int):
         """(summed ms, launches) of kernel group `which` (0 Newton, 1 FFT fwd, 2 Green,
@@ -279,6 +290,10 @@ class GpuJob:
 
     def set_fused_transforms(self, on: bool):
         _chk(gpu().evpgpu_set_fused_transforms(self.ctx, int(bool(on))), "evpgpu_set_fused_transforms")
+
+    def set_loop_batch(self, batch: int):
+        """Green iterations per batch queued ahead of the device (0: wait for each)."""
+        _chk(gpu().evpgpu_set_loop_batch(self.ctx, int(batch)), "evpgpu_set_loop_batch")
 
     def set_anchor_interval(self, iterations: int):
         _chk(gpu().evpgpu_set_anchor_interval(self.ctx, iterations), "evpgpu_set_anchor_interval")

@@ -311,6 +311,23 @@ def test_kernel_timing_groups(job_factory, fused, prec):
         assert t[5][1] == 0
 
 
+def test_kernel_timing_sample(job_factory):
+    """The Newton timer alone, on one Green iteration in 4: about a quarter of
+    the launches are timed (the increment's closing replay always), no other
+    timer records, and the sampled average is a launch duration."""
+    inp = job_factory("timing16", n=16, grains=8, seed=23, increments=1)
+    job = lib.GpuJob(inp, lib.MIXED)
+    job.set_timing(True, groups=[0], every=4)
+    it, _ = job.run(1)
+    ms, n = job.timing(0)
+    assert abs((n - 1) - it[0] / 4) <= 2, (n, it)
+    assert 0.0 < ms / n < 50.0
+    for k in range(1, 6):
+        assert job.timing(k)[1] == 0
+    with pytest.raises(RuntimeError):
+        job.set_timing(True, groups=[0], every=0)
+
+
 def test_newton_shortcut_is_exact(job_factory):
     """Ending certainly-rejected Newton iterations early changes nothing: the
     whole state after 3 increments is bit-identical with the shortcut off."""
@@ -326,6 +343,35 @@ def test_newton_shortcut_is_exact(job_factory):
     A, B = a.state(), b.state()
     for k in A:
         assert np.array_equal(A[k], B[k]), k
+
+
+@pytest.mark.parametrize("prec", [lib.FP64, lib.MIXED])
+def test_loop_batches_are_exact(job_factory, prec):
+    """The loop control runs on the device, so the host may queue iterations
+    ahead of it: batches of 1, 2 or 4 iterations (two in flight, the ones
+    queued past the loop's end doing nothing) give the state, the iteration
+    counts and the Newton/line-search sums bit for bit of a host that waits
+    for every iteration (batch 0), in converged and forced increments."""
+    inp = job_factory("batch16", n=16, grains=8, seed=29, increments=4, pore_fraction=0.02)
+    runs = []
+    for batch in (0, 1, 2, 4):
+        job = lib.GpuJob(inp, prec)
+        job.set_loop_batch(batch)
+        it, st = job.run(3)
+        itf, stf = job.run(1, forced=5)
+        runs.append((list(it) + list(itf), [x["newton_iters"] for x in st + stf],
+                     [x["ls_trials"] for x in st + stf], [x["fp64_iters"] for x in st + stf],
+                     job.state(), job.macro()))
+    it0, nit0, ntr0, f0, s0, m0 = runs[0]
+    assert it0[-1] == 5
+    for it, nit, ntr, f, s, m in runs[1:]:
+        assert it == it0 and nit == nit0 and ntr == ntr0 and f == f0
+        for k in s0:
+            assert np.array_equal(s[k], s0[k]), k
+        for k in ("sig_avg", "s_vm", "e_vm", "err_s", "err_e"):
+            assert np.array_equal(np.asarray(m[k]), np.asarray(m0[k])), k
+    with pytest.raises(RuntimeError):
+        lib.GpuJob(inp, prec).set_loop_batch(5)
 
 
 def test_mixed_precision_stress_error(job_factory):
