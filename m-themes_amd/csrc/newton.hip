@@ -495,24 +495,35 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
     // scalar registers they do not fit, and each use of a spilled one costs
     // two lane reads
     __shared__ double k0[42];
+    long long v = (long long)blockIdx.x * NB_THREADS + threadIdx.x;
+    // the voxel's phase and orientation index go out with the C0/S0 loads,
+    // unconditionally (a lane past the end reads the last voxel's and drops
+    // them), so that one wait covers all four: with one wave per SIMD every
+    // wait on memory stalls the SIMD
+    const long long vc = v < N ? v : N - 1;
+    const int ph_raw = phase[vc], ori = orient[vc];
+    const double c0v = P.c0p[threadIdx.x % 21], s0v = P.s0p[threadIdx.x % 21];
+    const int ph = v < N ? ph_raw : 1;
     if (threadIdx.x < 21) {
-        k0[threadIdx.x] = P.c0p[threadIdx.x];
-        k0[21 + threadIdx.x] = P.s0p[threadIdx.x];
+        k0[threadIdx.x] = c0v;
+        k0[21 + threadIdx.x] = s0v;
     }
     __syncthreads();
-    long long v = (long long)blockIdx.x * NB_THREADS + threadIdx.x;
     // this lane's err_s, err_e, Newton iterations and line-search trials
     double r_cs = 0.0, r_ce = 0.0, r_it = 0.0, r_tr = 0.0;
     int pl = 0;
     if (v < N) {
-        pl = phase[v] - 1;
+        pl = ph - 1;
         pl = pl < 0 ? 0 : (pl > 1 ? 1 : pl);
     }
+    // the two phases' gas flags as a select of kernel arguments, not an
+    // indexed (memory) read
+    const bool gas_l = pl == 0 ? P.is_gas[0] != 0 : P.is_gas[1] != 0;
     // Crystal voxels, one phase of the wave at a time: inside the loop the
     // phase index p is wave-uniform, so the rate-law constants, slip-system
     // counts and crystal axes are scalar operands and the slip-system loops
     // branch on scalars (a wave holds one crystal phase almost always).
-    bool pending = v < N && !P.is_gas[pl];
+    bool pending = v < N && !gas_l;
     // every pass clears at least the lead lane, so it ends after one pass
     // per distinct phase (two at most)
     for (unsigned long long todo = __ballot(pending); todo; todo = __ballot(pending)) {
@@ -525,35 +536,54 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             pending = false;
             Vox V;
             RateIn in;
-            double t[6];
+            // every load of the voxel's inputs first -- stress, plastic and
+            // total strain, the orientation's S, the rotation and the
+            // per-system CRSS (all 12: the arrays hold them whatever the
+            // phase's count) -- then the arithmetic: one wait on memory
+            // instead of one per group
+            double t[6], tp[6], te[6], T9[9], c1[EVPGPU_NSYSMX], c2[EVPGPU_NSYSMX];
 #pragma unroll
             for (int c = 0; c < 6; c++) t[c] = sig_in[c * N + v];
+#pragma unroll
+            for (int c = 0; c < 6; c++) tp[c] = epsp[c * N + v];
+#pragma unroll
+            for (int c = 0; c < 6; c++) te[c] = E[c * N + v];
+            const double *sv = sgr + 36LL * ori;
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = i; j < 6; j++) V.S[PK(i, j)] = sv[i * 6 + j];
+#pragma unroll
+            for (int q = 0; q < 9; q++) T9[q] = tran[q * N + v];
+            const long long S12 = (long long)EVPGPU_NSYSMX * N;
+#pragma unroll
+            for (int s = 0; s < EVPGPU_NSYSMX; s++) {
+                if (RM == RM_DISL) {
+                    c1[s] = crss[(P.dd_first ? 3 * S12 : 0) + s * N + v];
+                } else {
+                    c1[s] = crss[(s * 2) * N + v];
+                    c2[s] = CSYM ? c1[s] : crss[(s * 2 + 1) * N + v];
+                }
+            }
             // |lambda|_F over the full 3x3, row-major order
             V.signorm = sqrt(t[0] * t[0] + t[5] * t[5] + t[4] * t[4] + t[5] * t[5] + t[1] * t[1] +
                              t[3] * t[3] + t[4] * t[4] + t[3] * t[3] + t[2] * t[2]);
             sym_to_v6(t, V.xl);
             {
                 double e6[6], s6[6];
-#pragma unroll
-                for (int c = 0; c < 6; c++) t[c] = epsp[c * N + v];
-                sym_to_v6(t, e6);
-#pragma unroll
-                for (int c = 0; c < 6; c++) t[c] = E[c * N + v];
-                sym_to_v6(t, s6);
+                sym_to_v6(tp, e6);
+                sym_to_v6(te, s6);
 #pragma unroll
                 for (int i = 0; i < 6; i++) V.bp[i] = e6[i] - s6[i];
             }
-            const double *sv = sgr + 36LL * orient[v];
-#pragma unroll
-            for (int i = 0; i < 6; i++)
-#pragma unroll
-                for (int j = i; j < 6; j++) V.S[PK(i, j)] = sv[i * 6 + j];
-            const int ns = P.nsys[p];
+            // the fast variants have every crystal phase on 12 systems: a
+            // compile-time count keeps the per-system loops free of branches
+            const int ns = RM == RM_FAST ? EVPGPU_NSYSMX : P.nsys[p];
             // Schmid tensors in sample axes: sym((T n) x (T b)) in the 5-D basis
             // (update_schmid, src/kinematics.c:430-463), into this lane's LDS column
             double T[3][3];
 #pragma unroll
-            for (int q = 0; q < 9; q++) T[q / 3][q % 3] = tran[q * N + v];
+            for (int q = 0; q < 9; q++) T[q / 3][q % 3] = T9[q];
             in.sc = slab + threadIdx.x;
             in.k0 = k0;
             for (int s = 0; s < ns; s++) {
@@ -577,18 +607,16 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             if (RM == RM_DISL) {
                 // crss holds the increment's constants: passing stress,
                 // 1/(tau_0 + tau_cut), gam_o, the first call's passing stress
-                const long long S = (long long)EVPGPU_NSYSMX * N;
                 in.tpn = crss + v;
                 in.tps = N;
 #pragma unroll
-                for (int s = 0; s < EVPGPU_NSYSMX; s++)
-                    in.itf[s] = s < ns ? crss[(P.dd_first ? 3 * S : 0) + s * N + v] : 0.0;
+                for (int s = 0; s < EVPGPU_NSYSMX; s++) in.itf[s] = s < ns ? c1[s] : 0.0;
             } else {
 #pragma unroll
             for (int s = 0; s < EVPGPU_NSYSMX; s++) {
                 if (s < ns) {
-                    in.itf[s] = 1.0 / crss[(s * 2) * N + v];
-                    in.itb[s] = CSYM ? in.itf[s] : 1.0 / crss[(s * 2 + 1) * N + v];
+                    in.itf[s] = 1.0 / c1[s];
+                    in.itb[s] = CSYM ? in.itf[s] : 1.0 / c2[s];
                 } else {
                     in.itf[s] = in.itb[s] = 1.0;
                 }
@@ -630,7 +658,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
     red[2] = r_it;
     red[15] = r_tr;
     if (v < N) {
-        if (P.is_gas[pl]) {
+        if (gas_l) {
 #pragma unroll
             for (int c = 0; c < 6; c++) {
                 sig[c * N + v] = 0.0;
