@@ -145,11 +145,25 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
                                       const double s[6], double ed[5], double kk[21],
                                       double gd[EVPGPU_NSYSMX])
 {
+    // (the per-system test stays a branch on the phase's count even where it
+    // is always 12: without it the compiler interleaves the systems'
+    // arithmetic and the fast variant spills)
     const int ns = P.nsys[p];
     const int intn = P.int_n[p];
     // RM_FAST: the phase's rate constants, as the general path multiplies
     // them ((gam0 n) p1) itau
     const double g0 = P.gam0[p][0], g0n = P.gam0[p][0] * P.nrs[p][0];
+    // RM_DISL: 1/(tau_0 + tau_cut) and gam_o of every system, loaded
+    // together before the loop (read at their use, under the per-lane test
+    // of the passing stress, each active system waited on its own loads)
+    double itcs[EVPGPU_NSYSMX], g0ks[EVPGPU_NSYSMX];
+    if (RM == RM_DISL) {
+#pragma unroll
+        for (int k = 0; k < EVPGPU_NSYSMX; k++) {
+            itcs[k] = in.tpn[(EVPGPU_NSYSMX + k) * in.tps];
+            g0ks[k] = in.tpn[(2 * EVPGPU_NSYSMX + k) * in.tps];
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 5; i++) ed[i] = 0.;
     double scn[5];
@@ -171,12 +185,10 @@ __device__ __forceinline__ void srate(const NewtonParams &P, int p, const RateIn
             if (RM == RM_DISL) {
                 // gam = gam_o sinh((|rss| - tau_pass)/(tau_0 + tau_cut)) sign(rss)
                 // above the passing stress, 0 below (FORWARD_PLUS_BACKWARD)
-                // (1/(tau_0 + tau_cut) and gam_o are read where used: held in
-                // registers they would spill)
                 const double a = fabs(r);
                 if (a > in.itf[k]) {
-                    const double itc = in.tpn[(EVPGPU_NSYSMX + k) * in.tps];
-                    const double g0k = in.tpn[(2 * EVPGPU_NSYSMX + k) * in.tps];
+                    const double itc = itcs[k];
+                    const double g0k = g0ks[k];
                     const double x = (a - in.itf[k]) * itc;
                     // sinh and cosh from one expm1: with m = e^x - 1 and
                     // q = e^-x, sinh x = m (1 + q) / 2 (no cancellation at
@@ -578,7 +590,7 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             }
             // the fast variants have every crystal phase on 12 systems: a
             // compile-time count keeps the per-system loops free of branches
-            const int ns = RM == RM_FAST ? EVPGPU_NSYSMX : P.nsys[p];
+            const int ns = RM == RM_FAST || RM == RM_DISL ? EVPGPU_NSYSMX : P.nsys[p];
             // Schmid tensors in sample axes: sym((T n) x (T b)) in the 5-D basis
             // (update_schmid, src/kinematics.c:430-463), into this lane's LDS column
             double T[3][3];
@@ -751,6 +763,10 @@ int newton_launch(evpgpu_ctx *c, const NewtonParams &P, const double *sig_in, do
         if (!one) fastrm = false;
     }
     const int rm = !intn ? RM_POW : (fastrm ? RM_FAST : RM_LOOP);
+    // the dislocation-density variant assumes 12 systems in every crystal phase
+    if (P.dd)
+        for (int q = 0; q < 2; q++)
+            if (!P.is_gas[q] && P.nsys[q] != EVPGPU_NSYSMX) return EVPGPU_ERR_ARG;
     // compile-time exponents for the common n = 10 and n = 20 (every crystal
     // phase the same n, forward = backward CRSS)
     int ne = 0;
