@@ -282,6 +282,13 @@ __device__ __forceinline__ bool chol6_solve(double k[21], double b[6])
     return true;
 }
 
+// An empty asm statement that takes x in a register: the value must be
+// ready at this point (its load is not moved below it)
+__device__ __forceinline__ void keep_reg(const double &x)
+{
+    __asm__ volatile("" ::"v"(x));
+}
+
 // Per-voxel constants of the Newton solve: the compliance S = LU_inv_66(C_gr)
 // (symmetric; its upper triangle), lambda and eps_p - E in the B-basis.
 struct Vox {
@@ -598,7 +605,11 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
             for (int q = 0; q < 9; q++) T[q / 3][q % 3] = T9[q];
             in.sc = slab + threadIdx.x;
             in.k0 = k0;
-            for (int s = 0; s < ns; s++) {
+            // unrolled (straight-line where the count is the constant 12), so
+            // that the loads above stay ahead of this arithmetic
+#pragma unroll
+            for (int s = 0; s < EVPGPU_NSYSMX; s++) {
+                if (s >= ns) break;
                 double tn[3], tb[3];
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
@@ -615,6 +626,18 @@ __global__ __launch_bounds__(NB_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1
                 col[2 * NB_THREADS] = P12 * RSQ2 + P12 * RSQ2;
                 col[3 * NB_THREADS] = P02 * RSQ2 + P02 * RSQ2;
                 col[4 * NB_THREADS] = P01 * RSQ2 + P01 * RSQ2;
+            }
+            // the voxel's other inputs are taken here, after the Schmid
+            // arithmetic: left alone, the compiler sinks their loads past the
+            // Newton loop's entry test and the wave waits on memory twice
+#pragma unroll
+            for (int i = 0; i < 6; i++) keep_reg(V.bp[i]);
+#pragma unroll
+            for (int i = 0; i < 21; i++) keep_reg(V.S[i]);
+#pragma unroll
+            for (int k = 0; k < EVPGPU_NSYSMX; k++) {
+                keep_reg(c1[k]);
+                if (!CSYM && RM != RM_DISL) keep_reg(c2[k]);
             }
             if (RM == RM_DISL) {
                 // crss holds the increment's constants: passing stress,
