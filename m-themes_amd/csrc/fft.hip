@@ -109,10 +109,16 @@ __global__ __launch_bounds__(NT) void k_fft_strided(typename CT<R>::T *data, int
     const int o = tile / nbx;
     const long long base = (long long)(o / O2) * S1 + (long long)(o % O2) * S2 + t0;
     const int nt = min(TB, ninner - t0);
-    for (int e = threadIdx.x; e < L * TB; e += NT) {
-        int i = e / TB, t = e % TB;
-        if (t < nt) lds[t * P + i] = data[base + i * es + t];
-    }
+    staged_copy<NT, (L * TB + NT - 1) / NT, sizeof(R) == 8 ? 8 : 16, C>(
+        L * TB,
+        [&](int e) {
+            const int i = e / TB, t = e % TB;
+            return t < nt ? data[base + i * es + t] : C{};
+        },
+        [&](int e, C v) {
+            const int i = e / TB, t = e % TB;
+            if (t < nt) lds[t * P + i] = v;
+        });
     __syncthreads();
     stages<INV, L, 0, 1>(lds, tw);
     for (int e = threadIdx.x; e < L * TB; e += NT) {
@@ -137,25 +143,27 @@ __global__ __launch_bounds__(NT) void k_fft_z_r2c(const double *in, const float 
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TB;
     const long long off = comp * N + line0 * L; // even: the pairs below are aligned
-    for (int e = threadIdx.x; e < M * TB; e += NT) {
-        const int t = e / M, n = e % M;
-        R a, b;
-        if (IN == 0) {
-            double2 v = reinterpret_cast<const double2 *>(in + off)[e];
-            a = (R)v.x;
-            b = (R)v.y;
-        } else if (IN == 1) {
-            double2 v = reinterpret_cast<const double2 *>(in + off)[e];
-            float2 w = reinterpret_cast<const float2 *>(inA + off)[e];
-            a = (R)(v.x - (double)w.x);
-            b = (R)(v.y - (double)w.y);
-        } else {
-            float2 w = reinterpret_cast<const float2 *>(inA + off)[e];
-            a = (R)w.x;
-            b = (R)w.y;
-        }
-        lds[t * P + n] = C{a, b};
-    }
+    staged_copy<NT, (M * TB + NT - 1) / NT, IN == 1 ? 8 : 16, C>(
+        M * TB,
+        [&](int e) {
+            R a, b;
+            if (IN == 0) {
+                double2 v = reinterpret_cast<const double2 *>(in + off)[e];
+                a = (R)v.x;
+                b = (R)v.y;
+            } else if (IN == 1) {
+                double2 v = reinterpret_cast<const double2 *>(in + off)[e];
+                float2 w = reinterpret_cast<const float2 *>(inA + off)[e];
+                a = (R)(v.x - (double)w.x);
+                b = (R)(v.y - (double)w.y);
+            } else {
+                float2 w = reinterpret_cast<const float2 *>(inA + off)[e];
+                a = (R)w.x;
+                b = (R)w.y;
+            }
+            return C{a, b};
+        },
+        [&](int e, C v) { lds[(e / M) * P + e % M] = v; });
     __syncthreads();
     // the M-point transforms: the length-L table read at stride 2
     gstages<false, M, 0, 1, TB, P, 1, 0, NT, 2>(lds, tw);
@@ -188,7 +196,8 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, d
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TB;
     const C *src = in + comp * NK + line0 * P;
-    for (int e = threadIdx.x; e < P * TB; e += NT) lds[e] = src[e];
+    staged_copy<NT, (P * TB + NT - 1) / NT, sizeof(R) == 8 ? 8 : 16, C>(
+        P * TB, [&](int e) { return src[e]; }, [&](int e, C v) { lds[e] = v; });
     __syncthreads();
     for (int i = threadIdx.x; i < TB * (M / 2 + 1); i += NT)
         split_inverse<R, M>(lds + (i / (M / 2 + 1)) * P, i % (M / 2 + 1), tw);

@@ -12,6 +12,31 @@ template <typename R> struct CT;
 template <> struct CT<double> { typedef double2 T; };
 template <> struct CT<float> { typedef float2 T; };
 
+// Move `total` elements into LDS: element e = threadIdx.x + u NTH is
+// load(e) (a value of type T), then store(e, value).  A loop that stores
+// each value as it arrives waits on memory once per element; here a thread's
+// loads go out together, CH at a time (CH bounds the registers they hold),
+// and are waited for once per chunk.
+template <int NTH, int PER, int CH, typename T, typename LoadF, typename StoreF>
+__device__ __forceinline__ void staged_copy(int total, LoadF load, StoreF store)
+{
+#pragma unroll
+    for (int u0 = 0; u0 < PER; u0 += CH) {
+        T v[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + (u0 + u) * NTH;
+            v[u] = T{};
+            if (u0 + u < PER && e < total) v[u] = load(e);
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + (u0 + u) * NTH;
+            if (u0 + u < PER && e < total) store(e, v[u]);
+        }
+    }
+}
+
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return {a.x - b.x, a.y - b.y}; }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b)
