@@ -104,6 +104,10 @@ __global__ __launch_bounds__(NT) void k_fft_strided(typename CT<R>::T *data, int
     typedef typename CT<R>::T C;
     constexpr int P = L + 1;
     __shared__ C lds[TB * P];
+    // the twiddle table in LDS, loaded with the tile (from memory, every
+    // stage waited on its twiddles)
+    __shared__ C s_tw[L];
+    for (int i = threadIdx.x; i < L; i += NT) s_tw[i] = tw[i];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int t0 = (tile % nbx) * TB;
     const int o = tile / nbx;
@@ -120,7 +124,7 @@ __global__ __launch_bounds__(NT) void k_fft_strided(typename CT<R>::T *data, int
             if (t < nt) lds[t * P + i] = v;
         });
     __syncthreads();
-    stages<INV, L, 0, 1>(lds, tw);
+    stages<INV, L, 0, 1>(lds, (const C *)s_tw);
     for (int e = threadIdx.x; e < L * TB; e += NT) {
         int i = e / TB, t = e % TB;
         if (t < nt) data[base + i * es + t] = lds[t * P + i];
@@ -140,6 +144,8 @@ __global__ __launch_bounds__(NT) void k_fft_z_r2c(const double *in, const float 
     typedef typename CT<R>::T C;
     constexpr int M = L / 2, P = M + 1;
     __shared__ C lds[TB * P];
+    __shared__ C s_tw[L]; // the twiddle table, loaded with the lines
+    for (int i = threadIdx.x; i < L; i += NT) s_tw[i] = tw[i];
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TB;
     const long long off = comp * N + line0 * L; // even: the pairs below are aligned
@@ -166,9 +172,9 @@ __global__ __launch_bounds__(NT) void k_fft_z_r2c(const double *in, const float 
         [&](int e, C v) { lds[(e / M) * P + e % M] = v; });
     __syncthreads();
     // the M-point transforms: the length-L table read at stride 2
-    gstages<false, M, 0, 1, TB, P, 1, 0, NT, 2>(lds, tw);
+    gstages<false, M, 0, 1, TB, P, 1, 0, NT, 2>(lds, (const C *)s_tw);
     for (int i = threadIdx.x; i < TB * (M / 2 + 1); i += NT)
-        split_forward<R, M>(lds + (i / (M / 2 + 1)) * P, i % (M / 2 + 1), tw);
+        split_forward<R, M>(lds + (i / (M / 2 + 1)) * P, i % (M / 2 + 1), (const C *)s_tw);
     __syncthreads();
     C *dst = out + comp * NK + line0 * P;
     for (int e = threadIdx.x; e < P * TB; e += NT) dst[e] = lds[e];
@@ -193,6 +199,8 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, d
     typedef typename CT<R>::T C;
     constexpr int M = L / 2, P = M + 1;
     __shared__ C lds[TB * P];
+    __shared__ C s_tw[L]; // the twiddle table, loaded with the lines
+    for (int i = threadIdx.x; i < L; i += NT) s_tw[i] = tw[i];
     const int comp = blockIdx.y;
     const long long line0 = (long long)blockIdx.x * TB;
     const C *src = in + comp * NK + line0 * P;
@@ -200,9 +208,9 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(const typename CT<R>::T *in, d
         P * TB, [&](int e) { return src[e]; }, [&](int e, C v) { lds[e] = v; });
     __syncthreads();
     for (int i = threadIdx.x; i < TB * (M / 2 + 1); i += NT)
-        split_inverse<R, M>(lds + (i / (M / 2 + 1)) * P, i % (M / 2 + 1), tw);
+        split_inverse<R, M>(lds + (i / (M / 2 + 1)) * P, i % (M / 2 + 1), (const C *)s_tw);
     __syncthreads();
-    gstages<true, M, 0, 1, TB, P, 1, 0, NT, 2>(lds, tw);
+    gstages<true, M, 0, 1, TB, P, 1, 0, NT, 2>(lds, (const C *)s_tw);
     const double add = comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
     const long long off = comp * N + line0 * L;
     for (int e = threadIdx.x; e < M * TB; e += NT) {

@@ -54,6 +54,11 @@ __global__ __launch_bounds__(NTH) void k_zy_r2c(const double *__restrict__ in, c
     constexpr int M = PL::M, P = PL::P;
     if (skip && *skip) return;
     __shared__ C lds[NY * P];
+    // the twiddle tables in LDS, loaded with the plane: read from memory at
+    // every stage, each stage waited on them
+    __shared__ C s_twy[NY], s_twz[NZ];
+    if (threadIdx.x < NY) s_twy[threadIdx.x] = twy[threadIdx.x];
+    else if (threadIdx.x < NY + NZ) s_twz[threadIdx.x - NY] = twz[threadIdx.x - NY];
     const int x = blockIdx.x, comp = blockIdx.y;
     const long long base = comp * N + (long long)x * NY * NZ;
     // every load of the plane goes out before the first wait (a loop that
@@ -90,14 +95,14 @@ __global__ __launch_bounds__(NTH) void k_zy_r2c(const double *__restrict__ in, c
     }
     __syncthreads();
     // z: NY rows of M packed values (twiddles of length M = the nz table at stride 2)
-    gstages<false, M, 0, 1, NY, P, 1, 0, NTH, 2>(lds, twz);
+    gstages<false, M, 0, 1, NY, P, 1, 0, NTH, 2>(lds, (const C *)s_twz);
     for (int i = threadIdx.x; i < NY * (M / 2 + 1); i += NTH) {
         const int y = i / (M / 2 + 1), k = i % (M / 2 + 1);
-        split_forward<R, M>(lds + y * P, k, twz);
+        split_forward<R, M>(lds + y * P, k, (const C *)s_twz);
     }
     __syncthreads();
     // y: P columns of length NY
-    gstages<false, NY, 0, 1, P, 1, P, 1, NTH, 1>(lds, twy);
+    gstages<false, NY, 0, 1, P, 1, P, 1, NTH, 1>(lds, (const C *)s_twy);
     C *dst = out + comp * NK + (long long)x * NY * P;
     for (int i = threadIdx.x; i < NY * P; i += NTH) dst[i] = lds[i];
 }
@@ -120,6 +125,10 @@ __global__ __launch_bounds__(NTH) void k_yz_c2r(const typename CT<R>::T *__restr
     constexpr int M = PL::M, P = PL::P;
     if (skip && *skip) return;
     __shared__ C lds[NY * P];
+    // the twiddle tables in LDS, loaded with the spectrum plane
+    __shared__ C s_twy[NY], s_twz[NZ];
+    if (threadIdx.x < NY) s_twy[threadIdx.x] = twy[threadIdx.x];
+    else if (threadIdx.x < NY + NZ) s_twz[threadIdx.x - NY] = twz[threadIdx.x - NY];
     const int x = blockIdx.x, comp = blockIdx.y;
     const C *src = in + comp * NK + (long long)x * NY * P;
     {
@@ -143,13 +152,13 @@ __global__ __launch_bounds__(NTH) void k_yz_c2r(const typename CT<R>::T *__restr
         }
     }
     __syncthreads();
-    gstages<true, NY, 0, 1, P, 1, P, 1, NTH, 1>(lds, twy);
+    gstages<true, NY, 0, 1, P, 1, P, 1, NTH, 1>(lds, (const C *)s_twy);
     for (int i = threadIdx.x; i < NY * (M / 2 + 1); i += NTH) {
         const int y = i / (M / 2 + 1), k = i % (M / 2 + 1);
-        split_inverse<R, M>(lds + y * P, k, twz);
+        split_inverse<R, M>(lds + y * P, k, (const C *)s_twz);
     }
     __syncthreads();
-    gstages<true, M, 0, 1, NY, P, 1, 0, NTH, 2>(lds, twz);
+    gstages<true, M, 0, 1, NY, P, 1, 0, NTH, 2>(lds, (const C *)s_twz);
     const double add = addd ? addd[comp]
                             : comp == 0 ? a0 : comp == 1 ? a1 : comp == 2 ? a2 : comp == 3 ? a3 : comp == 4 ? a4 : a5;
     const long long base = comp * N + (long long)x * NY * NZ;
@@ -191,6 +200,9 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
     if (skip && *skip) return;
     __shared__ C lds[NL * PX];
     __shared__ R c0s[36];
+    // the twiddle table in LDS, loaded with the tile
+    __shared__ C s_tw[NX];
+    for (int i = threadIdx.x; i < NX; i += NTX) s_tw[i] = twx[i];
     if (threadIdx.x < 36) c0s[threadIdx.x] = (R)G.ga[threadIdx.x];
     // The x lines are the columns q = y*nzh + kz of the [x][y][kz] half
     // spectrum; a workgroup takes TBX consecutive columns q0.. (runs that
@@ -229,7 +241,7 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
         }
     }
     __syncthreads();
-    gstages<false, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
+    gstages<false, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, (const C *)s_tw);
     // Green result at (x, column q0 + j) back into LDS, antisymmetric part
     // into the accumulator
     auto put = [&](int x, int j, const C e6[6], const double wr[3], const double wi[3], bool split) {
@@ -281,7 +293,7 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
         put(0, 0, e6, wr, wi, false);
     }
     __syncthreads();
-    gstages<true, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, twx);
+    gstages<true, NX, 0, 1, NL, PX, 1, 0, NTX, 1>(lds, (const C *)s_tw);
     for (int i = threadIdx.x; i < NL * NX; i += NTX) {
         const int c = i / (TBX * NX), r = i % (TBX * NX), x = r / TBX, j = r % TBX;
         if (j < nk) spec[c * NK + (long long)x * ncol + q0 + j] = lds[(c * TBX + j) * PX + x];
@@ -297,7 +309,8 @@ template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[a
 // Green phases then overlap)
 template <typename R, int NX> constexpr int tbx()
 {
-    constexpr int per = 6 * (NX + 1) * (int)sizeof(typename CT<R>::T), cap = LDS_MAX / 3 - 1024;
+    constexpr int per = 6 * (NX + 1) * (int)sizeof(typename CT<R>::T),
+                  cap = LDS_MAX / 3 - 1024 - NX * (int)sizeof(typename CT<R>::T); // less the twiddle table
     return per * 16 <= cap ? 16 : per * 8 <= cap ? 8 : per * 4 <= cap ? 4 : 2;
 }
 
