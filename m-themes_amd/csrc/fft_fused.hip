@@ -187,6 +187,19 @@ __global__ __launch_bounds__(NTH) void k_yz_c2r(const typename CT<R>::T *__restr
     }
 }
 
+// LDS pitch of an x line (complex values).  The tile is written and read
+// column-fastest: lane l takes column l % TBX of x = l / TBX.  The LDS serves
+// the lanes of a group at once -- 32 lanes of 8-byte (FP32 complex) or 16 of
+// 16-byte (FP64 complex) accesses -- and element j * PX + x of a group of
+// G lanes lands in slot (j PX + x) mod G: distinct for every lane when
+// PX = G / TBX (mod G), the smallest such pitch >= NX.
+template <typename R> constexpr int px(int nx, int tbx)
+{
+    const int G = sizeof(R) == 4 ? 32 : 16;
+    const int want = (G / tbx) % G;
+    return nx + ((want - nx % G) % G + G) % G;
+}
+
 // GMODE 0: acc += antisym; 2: acc += antisym + skA; 3: skA = antisym
 template <typename R, int NX, int TBX, int GMODE>
 __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__ spec, double2 *__restrict__ acc,
@@ -195,7 +208,7 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
                                                  const int *__restrict__ skip)
 {
     typedef typename CT<R>::T C;
-    constexpr int PX = NX + 1;
+    constexpr int PX = px<R>(NX, TBX);
     constexpr int NL = 6 * TBX;
     if (skip && *skip) return;
     __shared__ C lds[NL * PX];
@@ -272,7 +285,9 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
         // keep the 36 Green coefficients in LDS: hoisted out of this loop
         // they would occupy 72 registers for the whole loop
         __asm__ volatile("" ::: "memory");
-        const int x = i / TBX, j = i % TBX;
+        // consecutive lanes take consecutive x of one column: conflict-free
+        // LDS rows, and a wave-uniform (y, kz)
+        const int x = i % NX, j = i / NX;
         const int q = q0 + j;
         if (j >= nk || (q == 0 && x == 0)) continue; // k = 0 below
         C s6[6], e6[6];
@@ -301,17 +316,21 @@ __global__ __launch_bounds__(NTX) void k_x_green(typename CT<R>::T *__restrict__
 }
 
 template <typename R> const typename CT<R>::T *twp(evpgpu_ctx *c, int axis);
+
 template <> const double2 *twp<double>(evpgpu_ctx *c, int axis) { return c->tw64[axis]; }
 template <> const float2 *twp<float>(evpgpu_ctx *c, int axis) { return c->tw32[axis]; }
 
-// lines per x-green workgroup: 6 components x TBX lines x (nx+1) in LDS,
+// lines per x-green workgroup: 6 components x TBX lines x px() in LDS,
 // sized so that three workgroups share a CU's LDS (their load, transform and
 // Green phases then overlap)
 template <typename R, int NX> constexpr int tbx()
 {
-    constexpr int per = 6 * (NX + 1) * (int)sizeof(typename CT<R>::T),
-                  cap = LDS_MAX / 3 - 1024 - NX * (int)sizeof(typename CT<R>::T); // less the twiddle table
-    return per * 16 <= cap ? 16 : per * 8 <= cap ? 8 : per * 4 <= cap ? 4 : 2;
+    constexpr int cs = (int)sizeof(typename CT<R>::T);
+    constexpr int cap = LDS_MAX / 3 - 1024 - NX * cs; // less the twiddle table
+    return 6 * 16 * px<R>(NX, 16) * cs <= cap  ? 16
+           : 6 * 8 * px<R>(NX, 8) * cs <= cap  ? 8
+           : 6 * 4 * px<R>(NX, 4) * cs <= cap  ? 4
+                                               : 2;
 }
 
 // plane (ny = nz) and line (nx) lengths of the fused kernels: powers of two
