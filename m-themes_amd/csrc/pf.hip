@@ -464,24 +464,55 @@ __device__ inline void avg_box(const Dims &g, const Lists &A, long long c, long 
         a[k] = 0.0;
         cn[k] = 0;
     }
-    for (int i = 0; i < 7; ++i)
-        for (int j = 0; j < 7; ++j)
-            for (int m = 0; m < 7; ++m) {
-                const int q = ((tx + i) * HY + (ty + j)) * HZ + (tz + m);
-                const int gp = s_gid[q];
-                bool hit = false;
+    if constexpr (M > 4) {
+        // long lists (rare): the cell-by-cell loop, which holds fewer registers
+        for (int i = 0; i < 7; ++i)
+            for (int j = 0; j < 7; ++j)
+                for (int m = 0; m < 7; ++m) {
+                    const int q = ((tx + i) * HY + (ty + j)) * HZ + (tz + m);
+                    const int gp = s_gid[q];
+                    bool hit = false;
 #pragma unroll
-                for (int k = 0; k < M; k++) hit = hit || gp == gk[k];
-                if (!hit) continue;
-                double e = s_de[q];
-                if (s_tf[q] && vid(g, x - 3 + i, y - 3 + j, z - 3 + m) < d) e = 100.0;
+                    for (int k = 0; k < M; k++) hit = hit || gp == gk[k];
+                    if (!hit) continue;
+                    double e = s_de[q];
+                    if (s_tf[q] && vid(g, x - 3 + i, y - 3 + j, z - 3 + m) < d) e = 100.0;
 #pragma unroll
-                for (int k = 0; k < M; k++)
-                    if (gp == gk[k]) {
-                        a[k] += e;
-                        cn[k] += 1;
-                    }
+                    for (int k = 0; k < M; k++)
+                        if (gp == gk[k]) {
+                            a[k] += e;
+                            cn[k] += 1;
+                        }
+                }
+    } else {
+        for (int i = 0; i < 7; ++i)
+            for (int j = 0; j < 7; ++j) {
+                // one z row of the box: its 7 cells' LDS reads go out together,
+                // then the sums in the cells' order (the reads are unconditional,
+                // the sums predicated)
+                const int q0 = ((tx + i) * HY + (ty + j)) * HZ + tz;
+                int gp[7];
+                double ev[7];
+                unsigned char tf[7];
+#pragma unroll
+                for (int m = 0; m < 7; ++m) {
+                    gp[m] = s_gid[q0 + m];
+                    ev[m] = s_de[q0 + m];
+                    tf[m] = s_tf[q0 + m];
+                }
+#pragma unroll
+                for (int m = 0; m < 7; ++m) {
+                    double e = ev[m];
+                    if (tf[m] && vid(g, x - 3 + i, y - 3 + j, z - 3 + m) < d) e = 100.0;
+#pragma unroll
+                    for (int k = 0; k < M; k++)
+                        if (gp[m] == gk[k]) {
+                            a[k] += e;
+                            cn[k] += 1;
+                        }
+                }
             }
+    }
 #pragma unroll
     for (int k = 0; k < M; k++)
         if (k < n) {
