@@ -233,25 +233,74 @@ __global__ __launch_bounds__(ACT_NT) void k_act_iter(Dims g, const int *emask, i
 __global__ __launch_bounds__(NT) void k_slot(Dims g, Lists A, const int *gcnt, const int *gidx, const int *act, int *nn, int *spos,
                        int *sg, int *wn, int *nupd, int *err)
 {
+    // The loads go out in two batches instead of one chain per neighbour:
+    // first the voxel's own list (its first OWN entries whatever the count:
+    // the lists hold CAP slots per cell) and the 9 early neighbours' activity
+    // flags and candidate counts, then each active neighbour's last candidate
+    // (the first one the search tries).
+    constexpr int OWN = 4;
     for (long long d = blockIdx.x * (long long)NT + threadIdx.x; d < g.NV; d += (long long)gridDim.x * NT) {
         int x, y, z;
         vox(g, d, x, y, z);
         const long long c = cix(g, x, y, z);
         const int n0 = A.cnt[c];
-        const bool relabel = n0 > 0 && A.val[(long long)(n0 - 1) * g.NC + c] < PF_EPS;
-        const int nf = relabel ? n0 - 1 : n0;
-        int found = 0;
-        for (int e = 0; e < 9 && !found; e++) {
+        int own[OWN];
+        double ownv[OWN];
+#pragma unroll
+        for (int f = 0; f < OWN; f++) {
+            own[f] = A.idx[(long long)f * g.NC + c];
+            ownv[f] = A.val[(long long)f * g.NC + c];
+        }
+        long long u9[9];
+        bool in9[9];
+        int a9[9], n9[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
             const int ux = x + EARLY[e][0], uy = y + EARLY[e][1], uz = z + EARLY[e][2];
-            if (!interior(g, ux, uy, uz)) continue;
-            const long long u = vid(g, ux, uy, uz);
-            if (!act[u]) continue;
-            const int n = gcnt[u];
+            in9[e] = interior(g, ux, uy, uz);
+            u9[e] = in9[e] ? vid(g, ux, uy, uz) : d;
+            a9[e] = act[u9[e]];
+            n9[e] = gcnt[u9[e]];
+        }
+        int top9[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            top9[e] = 0;
+            if (in9[e] && a9[e] && n9[e] > 0) top9[e] = gidx[(long long)(n9[e] - 1) * g.NV + u9[e]];
+        }
+        double lastv = 1.0;
+        int lasti = 0;
+        if (n0 > 0) {
+#pragma unroll
+            for (int f = 0; f < OWN; f++)
+                if (f == n0 - 1) {
+                    lastv = ownv[f];
+                    lasti = own[f];
+                }
+            if (n0 > OWN) {
+                lastv = A.val[(long long)(n0 - 1) * g.NC + c];
+                lasti = A.idx[(long long)(n0 - 1) * g.NC + c];
+            }
+        }
+        const bool relabel = n0 > 0 && lastv < PF_EPS;
+        const int nf = relabel ? n0 - 1 : n0;
+        // is grain gr one of the first nf entries of the voxel's list?
+        auto fixed_grain = [&](int gr) {
+            bool fixed = false;
+#pragma unroll
+            for (int f = 0; f < OWN; f++) fixed = fixed || (f < nf && own[f] == gr);
+            for (int f = OWN; f < nf; f++) fixed = fixed || A.idx[(long long)f * g.NC + c] == gr;
+            return fixed;
+        };
+        int found = 0;
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            if (found || !in9[e] || !a9[e]) continue;
+            const long long u = u9[e];
+            const int n = n9[e];
             for (int k = n - 1; k >= 0; k--) {
-                const int gr = gidx[(long long)k * g.NV + u];
-                bool fixed = false;
-                for (int f = 0; f < nf; f++) fixed = fixed || A.idx[(long long)f * g.NC + c] == gr;
-                if (!fixed) {
+                const int gr = k == n - 1 ? top9[e] : gidx[(long long)k * g.NV + u];
+                if (!fixed_grain(gr)) {
                     found = gr;
                     break;
                 }
@@ -260,7 +309,7 @@ __global__ __launch_bounds__(NT) void k_slot(Dims g, Lists A, const int *gcnt, c
         int n = n0, p = -1, s = 0;
         if (relabel) {
             p = n0 - 1;
-            s = found ? found : A.idx[(long long)(n0 - 1) * g.NC + c];
+            s = found ? found : lasti;
         } else if (found) {
             if (n0 == CAP) {
                 atomicOr(err, PFE_CAP); // the appended entry has no room
