@@ -189,15 +189,15 @@ __global__ __launch_bounds__(NTH) void k_yz_c2r(const typename CT<R>::T *__restr
 
 // LDS pitch of an x line (complex values).  The tile is written and read
 // column-fastest: lane l takes column l % TBX of x = l / TBX.  The LDS serves
-// the lanes of a group at once -- 32 lanes of 8-byte (FP32 complex) or 16 of
-// 16-byte (FP64 complex) accesses -- and element j * PX + x of a group of
-// G lanes lands in slot (j PX + x) mod G: distinct for every lane when
-// PX = G / TBX (mod G), the smallest such pitch >= NX.
+// 32 lanes of 8-byte (FP32 complex) accesses at once, and element j * PX + x
+// lands in bank pair (j PX + x) mod 32: distinct for every lane of the 32
+// when PX = 32 / TBX (mod 32), the smallest such pitch >= NX.  FP64 keeps
+// NX + 1 (the same rule for its 16-byte accesses measured no faster).
 template <typename R> constexpr int px(int nx, int tbx)
 {
-    const int G = sizeof(R) == 4 ? 32 : 16;
-    const int want = (G / tbx) % G;
-    return nx + ((want - nx % G) % G + G) % G;
+    if (sizeof(R) != 4) return nx + 1;
+    const int want = (32 / tbx) % 32;
+    return nx + ((want - nx % 32) % 32 + 32) % 32;
 }
 
 // GMODE 0: acc += antisym; 2: acc += antisym + skA; 3: skA = antisym
