@@ -355,9 +355,10 @@ int run_phase(evpgpu_ctx *c, int prec, int limit, int conv, int first, const Ctl
         done = f.done != 0;
     }
     // the batches still queued run nothing; wait for them, drop their timer
-    // pairs and bring the control block back
+    // pairs (every slot not collected above, a batch cut short by an error
+    // included) and bring the control block back
     if (hipStreamSynchronize(c->stream) != hipSuccess && !st) st = EVPGPU_ERR_HIP;
-    for (long long j = seen * B; j < sent * B; j++) tcollect_slot(c, (int)(j % EVP_NSLOT), 0);
+    for (int q = 0; q < EVP_NSLOT; q++) tcollect_slot(c, q, 0);
     c->tslot = 0;
     c->skip = nullptr;
     if (st) return st;
