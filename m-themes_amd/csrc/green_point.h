@@ -106,10 +106,10 @@ __device__ __forceinline__ void green_tensor(const R *c0s, const double *s0, int
 // from the 6 stress components directly (no 3x3 tensors held: fewer registers
 // in the fused x-line kernel)
 template <typename R, typename C>
-__device__ __forceinline__ void gamma_acc6(const R *ga, const double nd[3], const C s6[6], C e6[6],
-                                           C an[3], R w)
+__device__ __forceinline__ void gamma_acc6(const R *ga, const R nd[3], const C s6[6], C e6[6], C an[3],
+                                           R w)
 {
-    const R n0 = (R)nd[0], n1 = (R)nd[1], n2 = (R)nd[2];
+    const R n0 = nd[0], n1 = nd[1], n2 = nd[2];
     const R nn[6] = {n0 * n0, n1 * n1, n2 * n2, n1 * n2, n0 * n2, n0 * n1};
     R a[6];
 #pragma unroll
@@ -183,7 +183,8 @@ __device__ __forceinline__ void green_zero(const R *ga, const double *s0, int el
 // caller takes k = 0 to green_zero: kept apart, its code does not add to the
 // registers of the per-point loops)
 // split: the wave vector sits on a Nyquist plane where n(-k) != -n(k)
-// (Gamma_eff mixes two directions there)
+// (Gamma_eff mixes two directions there).  The directions are in R: FP64 the
+// reference's arithmetic, FP32 the FP32 form of kspace::mode_dirs.
 template <typename R, typename C>
 __device__ __forceinline__ void green_point(const R *ga, int x, int y, int kz, int nx, int ny, int nz,
                                             const C s6[6], C e6[6], double wr[3], double wi[3], bool &split)
@@ -193,7 +194,7 @@ __device__ __forceinline__ void green_point(const R *ga, int x, int y, int kz, i
     for (int c = 0; c < 6; c++) e6[c] = C{(R)0, (R)0};
 #pragma unroll
     for (int c = 0; c < 3; c++) an[c] = C{(R)0, (R)0};
-    double n1[3], n2[3];
+    R n1[3], n2[3];
     split = kspace::mode_dirs(x, y, kz, nx, ny, nz, n1, n2);
     if (!split) {
         gamma_acc6<R>(ga, n1, s6, e6, an, (R)1);

@@ -7,11 +7,15 @@
 //     forward function;
 //   * the Newton kernel's line-search backtracking rule
 //     (m-themes_amd/csrc/linesearch.h), checked against the oracle's and
-//     against the models it minimises on every branch.
+//     against the models it minimises on every branch;
+//   * the FP32 wave-vector directions and Nyquist split test
+//     (m-themes_amd/csrc/spectral.h), checked against the FP64 ones over
+//     whole half spectra.
 #include <string.h>
 
 #include "adjoint_voxel.h"
 #include "linesearch.h"
+#include "spectral.h"
 
 using namespace adj;
 
@@ -166,4 +170,39 @@ extern "C" double lsc_backtrack(double alam, double alam2, double f_new, double 
                                 double slope)
 {
     return ls_backtrack(alam, alam2, f_new, f2, f_old, slope);
+}
+
+// Every wave vector k != 0 of the nx x ny x nz half spectrum through both
+// forms of kspace::mode_dirs.  out: [0] wave vectors, [1] split in FP64,
+// [2] split decisions that differ, [3] max |n_f32 - n_f64| over n1 and, where
+// split, n2 (each up to sign: the Green operator is even in n).
+extern "C" void ksc_sweep(int nx, int ny, int nz, double *out)
+{
+    double tot = 0, splits = 0, bad = 0, dmax = 0;
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int kz = 0; kz <= nz / 2; kz++) {
+                if (x + y + kz == 0) continue;
+                double d1[3], d2[3];
+                float f1[3], f2[3];
+                const bool sd = kspace::mode_dirs(x, y, kz, nx, ny, nz, d1, d2);
+                const bool sf = kspace::mode_dirs(x, y, kz, nx, ny, nz, f1, f2);
+                tot += 1;
+                splits += sd;
+                bad += sd != sf;
+                for (int v = 0; v < (sd ? 2 : 1); v++) {
+                    const double *d = v ? d2 : d1;
+                    const float *f = v ? f2 : f1;
+                    double ep = 0, em = 0;
+                    for (int m = 0; m < 3; m++) {
+                        ep = fmax(ep, fabs(f[m] - d[m]));
+                        em = fmax(em, fabs(f[m] + d[m]));
+                    }
+                    dmax = fmax(dmax, fmin(ep, em));
+                }
+            }
+    out[0] = tot;
+    out[1] = splits;
+    out[2] = bad;
+    out[3] = dmax;
 }
