@@ -178,9 +178,17 @@ __global__ __launch_bounds__(256) void k_loop_ctl(const double *__restrict__ red
     else if (t < NL + NK) blk[t] = reinterpret_cast<const double *>(Kp)[t - NL];
     int st = t == 0 ? *status : 0;
     const int v = t % NRED, r0 = t / NRED;
+    // every row this thread sums is loaded before the first add (one trip to
+    // memory: the rows were written by workgroups on all the XCDs), then
+    // summed in row order
+    constexpr int STEP = 256 / NRED, TRIPS = (REDUCE_G + STEP - 1) / STEP;
+    double x[TRIPS];
+#pragma unroll
+    for (int k = 0; k < TRIPS; k++) x[k] = r0 + k * STEP < g ? red2[(r0 + k * STEP) * NRED + v] : 0.0;
     double s = 0.0;
-#pragma unroll 8
-    for (int b = r0; b < g; b += 256 / NRED) s += red2[b * NRED + v];
+#pragma unroll
+    for (int k = 0; k < TRIPS; k++)
+        if (r0 + k * STEP < g) s += x[k];
     sh[t] = s;
     __syncthreads();
     const LoopCtl &Ls = *reinterpret_cast<const LoopCtl *>(blk);
@@ -249,6 +257,7 @@ int ctl_iter_launch(evpgpu_ctx *c, int nblk)
     int g = 0;
     int st = reduce_stage1(c, c->partials, nblk, NRED, c->skip, &g);
     if (st) return st;
+    if (g < 1 || g > REDUCE_G) return EVPGPU_ERR_ARG; // k_loop_ctl loads at most REDUCE_G rows
     hipLaunchKernelGGL(k_loop_ctl, dim3(1), dim3(256), 0, c->stream, (const double *)c->red2, g,
                        (const int *)c->d_status, c->ctl, (const CtlConst *)c->ctl_k);
     return hipGetLastError() == hipSuccess ? EVPGPU_OK : EVPGPU_ERR_HIP;
