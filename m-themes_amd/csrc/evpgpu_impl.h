@@ -7,6 +7,7 @@
 #include <stdlib.h>
 
 #include "../../include/evpgpu.h"
+#include "loopctl_step.h"
 
 #define EVP_CHECK(x)                                                   \
     do {                                                               \
@@ -14,10 +15,6 @@
         if (e_ != hipSuccess) return EVPGPU_ERR_HIP;                   \
     } while (0)
 
-// Lebensohn-Tome basis constants (src/kinematics.c:22-24)
-#define RSQ2 0.70710678118654744
-#define RSQ3 0.57735026918962584
-#define RSQ6 0.40824829046386304
 
 // ---------------------------------------------------------------- params --
 // Per-launch constants of the per-voxel stress update (passed by value: they
@@ -96,35 +93,6 @@ struct EpilogueParams {
 // first-stage workgroups of the partial-sum reduction
 #define REDUCE_G 512
 
-// The Green loop's control, kept on the device so that the host can queue
-// iterations ahead of it: after each iteration's Newton reduction a
-// one-workgroup kernel finishes get_smacro, the error norms and the stopping
-// test (src/evolution.c:1415-1458), and sets `done` when the loop phase ends;
-// the kernels of the iterations queued after that see `done` and return.
-struct LoopFlags {
-    int iter;    // Green iterations carried out in the increment
-    int done;    // the phase has ended (converged, at its limit, or failed)
-    int status;  // EVPGPU_OK or the error that ended it
-    int pad;
-};
-struct LoopCtl {
-    LoopFlags f;
-    int limit;   // the phase runs while iter < limit ...
-    int conv;    // ... and, when conv, while max(|err_s|, |err_e|) > tol
-    double tol, e_vm;
-    double dD[9], dD_acum[9], dD_applied[9];
-    double add[6];              // the symmetric part of dD, as the strain update adds it
-    double sig_avg[9], sig_avg1[9], s_vm, s_vm1;
-    double err_s, err_e;
-    double nit, ntr;            // Newton iterations and line-search trials, summed
-};
-// the loading constants get_smacro reads
-struct CtlConst {
-    double s0[81], scauchy[9];
-    double wgt_ph1, scale;
-    int velgrad_bc[6], stress_bc[6];
-};
-static_assert(sizeof(LoopCtl) % 8 == 0 && sizeof(CtlConst) % 8 == 0, "control blocks are whole doubles");
 
 struct evpgpu_ctx {
     int device;

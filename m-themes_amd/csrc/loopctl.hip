@@ -14,125 +14,13 @@
 //
 // The arithmetic is the host code's it replaces, operation for operation
 // (built with -ffp-contract=off; IEEE division and square root), so the
-// stopping decisions and every value are the same.
+// stopping decisions and every value are the same; get_smacro's part lives in
+// loopctl_step.h, whose host build equals the oracle's bit for bit.
 #include <math.h>
 
 #include "evpgpu_impl.h"
 
 namespace {
-
-constexpr int IJV0[6] = {0, 1, 2, 1, 0, 0};
-constexpr int IJV1[6] = {0, 1, 2, 2, 2, 1};
-
-// the basis of chg_basis (src/kinematics.c:14-174): B[i][a][b]
-__device__ __forceinline__ double basis_entry(int i, int a, int b)
-{
-    if (i == 0) return a == 0 && b == 0 ? -1. * RSQ2 : (a == 1 && b == 1 ? RSQ2 : 0.0);
-    if (i == 1) return a == b ? (a == 2 ? 2. * RSQ6 : -1. * RSQ6) : 0.0;
-    if (i == 2) return (a == 1 && b == 2) || (a == 2 && b == 1) ? RSQ2 : 0.0;
-    if (i == 3) return (a == 0 && b == 2) || (a == 2 && b == 0) ? RSQ2 : 0.0;
-    if (i == 4) return (a == 0 && b == 1) || (a == 1 && b == 0) ? RSQ2 : 0.0;
-    return a == b ? RSQ3 : 0.0;
-}
-
-// chg_basis opt 2 (first n components)
-__device__ __forceinline__ void t2_to_v(const double t[9], double *v, int n)
-{
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        if (i >= n) break;
-        double s = 0.0;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int b = 0; b < 3; b++) s += t[a * 3 + b] * basis_entry(i, a, b);
-        v[i] = s;
-    }
-}
-
-// chg_basis opt 1 (first n components)
-__device__ __forceinline__ void v_to_t2(const double *v, double t[9], int n)
-{
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-            double s = 0.0;
-#pragma unroll
-            for (int i = 0; i < 6; i++) {
-                if (i >= n) break;
-                s += v[i] * basis_entry(i, a, b);
-            }
-            t[a * 3 + b] = s;
-        }
-}
-
-__device__ __forceinline__ void sym6_to_full(const double s[6], double t[9])
-{
-    t[0] = s[0]; t[4] = s[1]; t[8] = s[2];
-    t[5] = t[7] = s[3];
-    t[2] = t[6] = s[4];
-    t[1] = t[3] = s[5];
-}
-
-// get_smacro (src/evolution.c:473-547) on the reduced sums
-__device__ __forceinline__ void smacro(const CtlConst &K, LoopCtl &L, const double *red)
-{
-    const double wgt = K.scale;
-    double s[6], s1[6];
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        s[k] = red[3 + k] * wgt;
-        s1[k] = red[9 + k] * (wgt / K.wgt_ph1);
-    }
-    sym6_to_full(s, L.sig_avg);
-    sym6_to_full(s1, L.sig_avg1);
-    double dD[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) dD[i] = L.dD[i];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int ii = IJV0[i], jj = IJV1[i];
-        double d = 0.0;
-        if (K.velgrad_bc[i] == 0) {
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                const int kk = IJV0[k], ll = IJV1[k];
-                d += K.s0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * K.stress_bc[k] *
-                     (K.scauchy[kk * 3 + ll] - L.sig_avg[kk * 3 + ll]);
-            }
-        }
-        dD[ii * 3 + jj] = d;
-    }
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        L.dD[i] = dD[i];
-        L.dD_acum[i] += dD[i];
-    }
-    double v6[6], dev[9];
-    t2_to_v(L.sig_avg, v6, 6);
-    v_to_t2(v6, dev, 5);
-    double tr = dev[0] + dev[4] + dev[8], a = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            double d = dev[i * 3 + j] - (i == j ? tr / 3.0 : 0.0);
-            a += (i == j) ? d * d : dev[i * 3 + j] * dev[i * 3 + j];
-        }
-    L.s_vm = sqrt(3. / 2. * a);
-    t2_to_v(L.sig_avg1, v6, 6);
-    v_to_t2(v6, dev, 5);
-    a = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; i++) a += dev[i] * dev[i];
-    L.s_vm1 = sqrt(3. / 2. * a);
-}
-
-__device__ __forceinline__ int phase_done(const LoopCtl &L)
-{
-    return !(L.f.iter < L.limit && (!L.conv || fmax(fabs(L.err_s), fabs(L.err_e)) > L.tol));
-}
 
 __global__ void k_ctl_init(LoopCtl *L, LoopCtl init, CtlConst *Kd, CtlConst K)
 {
@@ -149,7 +37,7 @@ __global__ void k_ctl_phase(LoopCtl *Lp, int limit, int conv, int first)
     LoopCtl L = *Lp;
     L.limit = limit;
     L.conv = conv;
-    L.f.done = L.f.status ? 1 : (first ? 0 : phase_done(L));
+    L.f.done = L.f.status ? 1 : (first ? 0 : lctl::phase_done(L));
     Lp->limit = limit;
     Lp->conv = conv;
     Lp->f.done = L.f.done;
@@ -222,7 +110,7 @@ __global__ __launch_bounds__(256) void k_loop_ctl(const double *__restrict__ red
     L.err_s = red[0] * K.scale;
     L.nit += red[2];
     L.ntr += red[15];
-    smacro(K, L, red);
+    lctl::smacro(K, L, red);
     if (fabs(L.e_vm) < 1E-5) L.err_e = 0.0;
     else L.err_e /= L.e_vm;
     L.err_s /= L.s_vm;
@@ -234,7 +122,7 @@ __global__ __launch_bounds__(256) void k_loop_ctl(const double *__restrict__ red
     L.add[4] = (d[2] + d[6]) / 2.0;
     L.add[5] = (d[1] + d[3]) / 2.0;
     L.f.iter++;
-    L.f.done = phase_done(L);
+    L.f.done = lctl::phase_done(L);
     *Lp = L;
 }
 

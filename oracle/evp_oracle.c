@@ -1463,10 +1463,55 @@ void orc_job_green(const orc_job *J, const double *ksig_r, const double *ksig_i,
                     (double(*)[3])(out_i + 9 * (size_t)v));
 }
 
-/* get_smacro (src/evolution.c:473-547), Q3 fixed */
-static void get_smacro(orc_job *J)
+/* get_smacro (src/evolution.c:473-547), Q3 fixed, from the column sums of
+ * the stress over all voxels (sum) and over phase-1 voxels (sum1) */
+void orc_smacro_core(const double sum[3][3], const double sum1[3][3], double WGT, double Wgt_ph1,
+                     const double S0[3][3][3][3], const double Scauchy[3][3], const int VelGrad_BC_Flag[6],
+                     const int Stress_BC_Flag[6], double SigAvg[3][3], double SigAvg1[3][3],
+                     double dDisGradAvg[3][3], double dDisGradAvg_acum[3][3], double *s_vm_out,
+                     double *s_vm1_out)
 {
     static const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            SigAvg[a][b] = sum[a][b] * WGT;
+            SigAvg1[a][b] = sum1[a][b] * (WGT / Wgt_ph1);
+        }
+    for (int i = 0; i < 6; i++) {
+        int ii = IJV[0][i], jj = IJV[1][i];
+        dDisGradAvg[ii][jj] = 0.0;
+        if (VelGrad_BC_Flag[i] == 0)
+            for (int k = 0; k < 6; k++) {
+                int kk = IJV[0][k], ll = IJV[1][k];
+                dDisGradAvg[ii][jj] += S0[ii][jj][kk][ll] * Stress_BC_Flag[k] * (Scauchy[kk][ll] - SigAvg[kk][ll]);
+            }
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) dDisGradAvg_acum[a][b] += dDisGradAvg[a][b];
+    double s6[6], s5[5], dev[3][3];
+    orc_t2_to_v6((const double(*)[3])SigAvg, s6);
+    for (int i = 0; i < 5; i++) s5[i] = s6[i];
+    orc_v5_to_t2(s5, dev);
+    double tr = dev[0][0] + dev[1][1] + dev[2][2];
+    double s_vm = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            if (a != b) s_vm += dev[a][b] * dev[a][b];
+            else s_vm += (dev[a][b] - tr / 3.0) * (dev[a][b] - tr / 3.0);
+        }
+    *s_vm_out = sqrt(3. / 2. * s_vm);
+    orc_t2_to_v6((const double(*)[3])SigAvg1, s6);
+    for (int i = 0; i < 5; i++) s5[i] = s6[i];
+    orc_v5_to_t2(s5, dev);
+    double s_vm1 = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) s_vm1 += dev[a][b] * dev[a][b];
+    *s_vm1_out = sqrt(3. / 2. * s_vm1);
+}
+
+static void get_smacro(orc_job *J)
+{
+    double sum[3][3], sum1[3][3];
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) {
             double s = 0.0, s1 = 0.0;
@@ -1474,40 +1519,13 @@ static void get_smacro(orc_job *J)
                 s += J->Sig[v][a][b];
                 if (J->phase[v] == 1) s1 += J->Sig[v][a][b];
             }
-            J->SigAvg[a][b] = s * J->WGT;
-            J->SigAvg1[a][b] = s1 * (J->WGT / J->Wgt_ph1);
+            sum[a][b] = s;
+            sum1[a][b] = s1;
         }
-    for (int i = 0; i < 6; i++) {
-        int ii = IJV[0][i], jj = IJV[1][i];
-        J->dDisGradAvg[ii][jj] = 0.0;
-        if (J->VelGrad_BC_Flag[i] == 0)
-            for (int k = 0; k < 6; k++) {
-                int kk = IJV[0][k], ll = IJV[1][k];
-                J->dDisGradAvg[ii][jj] += J->S0[ii][jj][kk][ll] * J->Stress_BC_Flag[k] *
-                                          (J->Scauchy[kk][ll] - J->SigAvg[kk][ll]);
-            }
-    }
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) J->dDisGradAvg_acum[a][b] += J->dDisGradAvg[a][b];
-    double s6[6], s5[5], dev[3][3];
-    orc_t2_to_v6(J->SigAvg, s6);
-    for (int i = 0; i < 5; i++) s5[i] = s6[i];
-    orc_v5_to_t2(s5, dev);
-    double tr = dev[0][0] + dev[1][1] + dev[2][2];
-    J->s_vm = 0.0;
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) {
-            if (a != b) J->s_vm += dev[a][b] * dev[a][b];
-            else J->s_vm += (dev[a][b] - tr / 3.0) * (dev[a][b] - tr / 3.0);
-        }
-    J->s_vm = sqrt(3. / 2. * J->s_vm);
-    orc_t2_to_v6(J->SigAvg1, s6);
-    for (int i = 0; i < 5; i++) s5[i] = s6[i];
-    orc_v5_to_t2(s5, dev);
-    J->s_vm1 = 0.0;
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) J->s_vm1 += dev[a][b] * dev[a][b];
-    J->s_vm1 = sqrt(3. / 2. * J->s_vm1);
+    orc_smacro_core((const double(*)[3])sum, (const double(*)[3])sum1, J->WGT, J->Wgt_ph1,
+                    (const double(*)[3][3][3])J->S0, (const double(*)[3])J->Scauchy, J->VelGrad_BC_Flag,
+                    J->Stress_BC_Flag, J->SigAvg, J->SigAvg1, J->dDisGradAvg, J->dDisGradAvg_acum, &J->s_vm,
+                    &J->s_vm1);
 }
 
 static double now_s(void)

@@ -10,11 +10,15 @@
 //     against the models it minimises on every branch;
 //   * the FP32 wave-vector directions and Nyquist split test
 //     (m-themes_amd/csrc/spectral.h), checked against the FP64 ones over
-//     whole half spectra.
+//     whole half spectra;
+//   * the Green loop control's get_smacro step
+//     (m-themes_amd/csrc/loopctl_step.h), checked against the oracle's bit
+//     for bit.
 #include <string.h>
 
 #include "adjoint_voxel.h"
 #include "linesearch.h"
+#include "loopctl_step.h"
 #include "spectral.h"
 
 using namespace adj;
@@ -205,4 +209,35 @@ extern "C" void ksc_sweep(int nx, int ny, int nz, double *out)
     out[1] = splits;
     out[2] = bad;
     out[3] = dmax;
+}
+
+// lctl::smacro on one set of reduced sums: red[3..8] the stress sums over
+// all voxels, red[9..14] over phase-1 voxels (11 22 33 23 13 12).  dD and
+// dD_acum (9 each, row-major) are updated in place; out receives sig_avg (9),
+// sig_avg1 (9), s_vm and s_vm1.
+extern "C" void lcc_smacro(const double *s0, const double *scauchy, double wgt_ph1, double scale,
+                           const int *velgrad_bc, const int *stress_bc, const double *red, double *dD,
+                           double *dD_acum, double *out)
+{
+    CtlConst K;
+    LoopCtl L;
+    memset(&K, 0, sizeof(K));
+    memset(&L, 0, sizeof(L));
+    memcpy(K.s0, s0, sizeof(K.s0));
+    memcpy(K.scauchy, scauchy, sizeof(K.scauchy));
+    K.wgt_ph1 = wgt_ph1;
+    K.scale = scale;
+    for (int i = 0; i < 6; i++) {
+        K.velgrad_bc[i] = velgrad_bc[i];
+        K.stress_bc[i] = stress_bc[i];
+    }
+    memcpy(L.dD, dD, sizeof(L.dD));
+    memcpy(L.dD_acum, dD_acum, sizeof(L.dD_acum));
+    lctl::smacro(K, L, red);
+    memcpy(dD, L.dD, sizeof(L.dD));
+    memcpy(dD_acum, L.dD_acum, sizeof(L.dD_acum));
+    memcpy(out, L.sig_avg, sizeof(L.sig_avg));
+    memcpy(out + 9, L.sig_avg1, sizeof(L.sig_avg1));
+    out[18] = L.s_vm;
+    out[19] = L.s_vm1;
 }
