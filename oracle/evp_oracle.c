@@ -19,10 +19,12 @@
  *   875-1108 (Gradient_ShearRate), 1346-1520 (DislocationEvolution);
  *   src/init.c:179-203, 593-702, 821-846, 1025, 1073-1125 (its set-up)
  *
- * Parity pins: tests/test_oracle_pins.py and tests/test_oracle_dd.py.  The
- * SSD rate expression of DislocationEvolution (dd_evolution) is parity
- * unpinned: it is restated term by term and only checked for the signs and
- * growth the law implies (its GND rates are pinned by a closed form).
+ * Parity pins: tests/test_oracle_pins.py, tests/test_oracle_plastic.py
+ * (the plastic whole loop against exact single-crystal and laminate
+ * solutions), tests/test_oracle_line_search.py (every line-search trial),
+ * tests/test_oracle_dd.py (the DD law; the SSD rate of dd_evolution one term
+ * at a time against its physics, the GND rates by a closed form) and
+ * tests/test_loopctl_step.py (get_smacro's core against the device's step).
  *
  * Dispositions of the reference's quirks (shared with the HIP path):
  *   Q1  the power-law driver is reconstructed: the live loop of
