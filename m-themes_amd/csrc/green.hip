@@ -21,6 +21,7 @@
 // accumulator that the increment epilogue inverts once (they only feed the
 // lattice spin of update_orient).
 #include "evpgpu_impl.h"
+#include "fft_core.h"
 #include "green_point.h"
 #include "spectral.h"
 

@@ -11,9 +11,15 @@
 // k = 0, -S0 : sigma-hat when ElastBC == 0, else zero.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <math.h>
 
-#include "fft_core.h"
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define GPHD __host__ __device__ __forceinline__
+#else
+#define GPHD static inline
+#endif
+
 #include "spectral.h"
 
 namespace gop {
@@ -21,7 +27,7 @@ namespace gop {
 // e = Gamma(n) s for complex s (3x3 full), e accumulated with weight w;
 // ga: the 36 coefficients of the symmetric A = C0 : (n x n) (GreenParams)
 template <typename R, typename C>
-__device__ __forceinline__ void gamma_apply(const R *ga, const double nd[3], const C s[3][3],
+GPHD void gamma_apply(const R *ga, const double nd[3], const C s[3][3],
                                             C e[3][3], R w)
 {
     R n[3] = {(R)nd[0], (R)nd[1], (R)nd[2]};
@@ -67,7 +73,7 @@ __device__ __forceinline__ void gamma_apply(const R *ga, const double nd[3], con
 
 // the full (non-symmetric) e-hat of one wave vector
 template <typename R, typename C>
-__device__ __forceinline__ void green_tensor(const R *c0s, const double *s0, int elast_bc, int x, int y,
+GPHD void green_tensor(const R *c0s, const double *s0, int elast_bc, int x, int y,
                                              int kz, int nx, int ny, int nz, const C s6[6], C e[3][3])
 {
     C s[3][3] = {{s6[0], s6[5], s6[4]}, {s6[5], s6[1], s6[3]}, {s6[4], s6[3], s6[2]}};
@@ -106,7 +112,7 @@ __device__ __forceinline__ void green_tensor(const R *c0s, const double *s0, int
 // from the 6 stress components directly (no 3x3 tensors held: fewer registers
 // in the fused x-line kernel)
 template <typename R, typename C>
-__device__ __forceinline__ void gamma_acc6(const R *ga, const R nd[3], const C s6[6], C e6[6], C an[3],
+GPHD void gamma_acc6(const R *ga, const R nd[3], const C s6[6], C e6[6], C an[3],
                                            R w)
 {
     const R n0 = nd[0], n1 = nd[1], n2 = nd[2];
@@ -152,7 +158,7 @@ __device__ __forceinline__ void gamma_acc6(const R *ga, const R nd[3], const C s
 
 // e6 / antisymmetric parts from a full e-hat
 template <typename R, typename C>
-__device__ __forceinline__ void split_e(const C e[3][3], C e6[6], double wr[3], double wi[3])
+GPHD void split_e(const C e[3][3], C e6[6], double wr[3], double wi[3])
 {
     const int ia[6] = {0, 1, 2, 1, 0, 0}, ib[6] = {0, 1, 2, 2, 2, 1};
 #pragma unroll
@@ -170,7 +176,7 @@ __device__ __forceinline__ void split_e(const C e[3][3], C e6[6], double wr[3], 
 
 // the k = 0 term: -S0 : sigma-hat when ElastBC == 0, else zero
 template <typename R, typename C>
-__device__ __forceinline__ void green_zero(const R *ga, const double *s0, int elast_bc, int nx, int ny,
+GPHD void green_zero(const R *ga, const double *s0, int elast_bc, int nx, int ny,
                                            int nz, const C s6[6], C e6[6], double wr[3], double wi[3])
 {
     C e[3][3];
@@ -186,7 +192,7 @@ __device__ __forceinline__ void green_zero(const R *ga, const double *s0, int el
 // (Gamma_eff mixes two directions there).  The directions are in R: FP64 the
 // reference's arithmetic, FP32 the FP32 form of kspace::mode_dirs.
 template <typename R, typename C>
-__device__ __forceinline__ void green_point(const R *ga, int x, int y, int kz, int nx, int ny, int nz,
+GPHD void green_point(const R *ga, int x, int y, int kz, int nx, int ny, int nz,
                                             const C s6[6], C e6[6], double wr[3], double wi[3], bool &split)
 {
     C an[3];

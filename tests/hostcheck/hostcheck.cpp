@@ -13,10 +13,14 @@
 //     whole half spectra;
 //   * the Green loop control's get_smacro step
 //     (m-themes_amd/csrc/loopctl_step.h), checked against the oracle's bit
-//     for bit.
+//     for bit;
+//   * the Green operator at one wave vector of the half spectrum
+//     (m-themes_amd/csrc/green_point.h, FP64 and FP32), checked against the
+//     oracle's full-grid operator.
 #include <string.h>
 
 #include "adjoint_voxel.h"
+#include "green_point.h"
 #include "linesearch.h"
 #include "loopctl_step.h"
 #include "spectral.h"
@@ -240,4 +244,49 @@ extern "C" void lcc_smacro(const double *s0, const double *scauchy, double wgt_p
     memcpy(out + 9, L.sig_avg1, sizeof(L.sig_avg1));
     out[18] = L.s_vm;
     out[19] = L.s_vm1;
+}
+
+
+namespace {
+struct cplx64 {
+    double x, y;
+};
+struct cplx32 {
+    float x, y;
+};
+
+template <typename R, typename C>
+int green_point_as(const double *ga36, int x, int y, int kz, int nx, int ny, int nz, const double *s12,
+                   double *e12, double *w6)
+{
+    R ga[36];
+    for (int i = 0; i < 36; i++) ga[i] = (R)ga36[i];
+    C s6[6], e6[6];
+    for (int c = 0; c < 6; c++) s6[c] = C{(R)s12[2 * c], (R)s12[2 * c + 1]};
+    double wr[3], wi[3];
+    bool split = false;
+    gop::green_point<R>(ga, x, y, kz, nx, ny, nz, s6, e6, wr, wi, split);
+    for (int c = 0; c < 6; c++) {
+        e12[2 * c] = (double)e6[c].x;
+        e12[2 * c + 1] = (double)e6[c].y;
+    }
+    for (int c = 0; c < 3; c++) {
+        w6[c] = wr[c];
+        w6[3 + c] = wi[c];
+    }
+    return split ? 1 : 0;
+}
+} // namespace
+
+// gop::green_point at half-spectrum entry (x, y, kz) != 0 in FP64 (fp32 = 0)
+// or FP32: ga36 the Green coefficients A = C0 : (n x n) as the library builds
+// them, s12 the stress transform (11 22 33 23 13 12, real and imaginary
+// parts interleaved); e12 the symmetric part of e-hat in the same layout,
+// w6 the antisymmetric part (23 13 12 real, then imaginary).  Returns the
+// split flag.
+extern "C" int gpc_green_point(int fp32, const double *ga36, int x, int y, int kz, int nx, int ny, int nz,
+                               const double *s12, double *e12, double *w6)
+{
+    if (fp32) return green_point_as<float, cplx32>(ga36, x, y, kz, nx, ny, nz, s12, e12, w6);
+    return green_point_as<double, cplx64>(ga36, x, y, kz, nx, ny, nz, s12, e12, w6);
 }
